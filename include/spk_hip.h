@@ -1,0 +1,147 @@
+/*
+ * spk_hip.h -- C ABI of libspk_hip.so: SPPARKS' strict-mode Metropolis colour
+ * sweep (`sweep color/strict`, app_style potts / ising) on one MI355X.
+ *
+ * Plain C: pointers, sizes and status codes only.  Every call returns 0 on
+ * success or a negative SPK_HIP_E* code; spk_hip_last_error() then holds a
+ * thread-local message.  Device memory belongs to the handle; host arrays
+ * belong to the caller and are copied.  Each handle owns one HIP stream and
+ * is not thread-safe; distinct handles may be driven from distinct threads.
+ *
+ * Host arrays use the reference's site order: 0-based index
+ * g = id-1 = i + nx*(j + ny*k), x fastest (src/create_sites.cpp:344-350).
+ * On the device, spins and seeds are held colour-blocked (all sites of colour
+ * 1, then colour 2, ...), which is invisible through this interface.
+ *
+ * Reference interfaces replaced (paths relative to the SPPARKS source tree):
+ *   - the per-colour sweep `(this->*sweep)(n, list)` bound to
+ *     AppLattice::sweep_nomask_strict / sweep_mask_strict
+ *     (src/app_lattice.h:165-170, src/app_lattice.cpp:305-314, :674-677,
+ *     :796-819), which calls AppPotts / AppIsing::site_event_rejection and
+ *     site_energy (src/app_lattice.h:73-74, src/app_potts.cpp:97-146,
+ *     src/app_ising.cpp:88-136);
+ *   - the per-site RandomPark seeds `siteseeds` (src/app_lattice.h:119,
+ *     src/app_lattice.cpp:287-295, src/random_park.cpp:67-86);
+ *   - DiagEnergy::compute and AppLattice::stats reductions
+ *     (src/diag_energy.cpp:53-66, src/app_lattice.cpp:917-934).
+ */
+#ifndef SPK_HIP_H
+#define SPK_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SPK_HIP_ABI_VERSION 1
+
+/* status codes */
+#define SPK_HIP_OK        0
+#define SPK_HIP_EINVAL   -1   /* bad argument / incommensurate lattice    */
+#define SPK_HIP_EDEVICE  -2   /* HIP runtime error                        */
+#define SPK_HIP_ENOMEM   -3   /* device or host allocation failed         */
+#define SPK_HIP_ESTATE   -4   /* call not valid in the handle's state     */
+#define SPK_HIP_EVALUE   -5   /* site values outside the app's range      */
+
+/* app styles (src/app_potts.h:18-19, src/app_ising.h:18-19) */
+#define SPK_HIP_POTTS 0
+#define SPK_HIP_ISING 1
+
+/* lattice styles (src/lattice.cpp:44-56): colours 2 / 4 / 2 / 8 */
+#define SPK_HIP_SQ_4N  0
+#define SPK_HIP_SQ_8N  1
+#define SPK_HIP_SC_6N  2
+#define SPK_HIP_SC_26N 3
+
+typedef struct spk_hip_desc {
+  int app;             /* SPK_HIP_POTTS or SPK_HIP_ISING                  */
+  int lattice;         /* SPK_HIP_SQ_4N ... SPK_HIP_SC_26N                */
+  int nx, ny, nz;      /* periodic box in lattice units; nz = 1 in 2d     */
+  int nspins;          /* Potts Q (`app_style potts Q`); ignored for Ising */
+  double temperature;  /* kT (`temperature` command); 0.0 = T=0 branch    */
+} spk_hip_desc;
+
+typedef struct spk_hip_lattice spk_hip_lattice;   /* opaque handle */
+
+/* Thread-local message for the last failing call on this thread. */
+const char *spk_hip_last_error(void);
+
+/* Number of colours the strict sweep uses for `d` (Lattice::ncolors,
+   src/lattice.cpp:187-228); SPK_HIP_EINVAL when an edge is odd. */
+int spk_hip_ncolors(const spk_hip_desc *d);
+
+/* Metropolis table of a temperature (host only, no device needed):
+   tab[d] = exp((ei-ef)*t_inverse) for ef-ei = d = 1..maxd, t_inverse =
+   1.0/temperature, exactly as src/app_potts.cpp:130 / src/app_ising.cpp:120
+   evaluate it with this process's libm; thr[d] = the smallest Park-Miller
+   state s with (1.0/2147483647)*s > tab[d] (2147483647 if none), so that the
+   reference's rejection `uniform() > tab[d]` is `s >= thr[d]`.  thr[0] gets
+   the smallest s with (1.0/2147483647)*s >= 0.5 (the Ising proposal cut).
+   Either output may be NULL; arrays hold maxd+1 entries. */
+int spk_hip_metropolis_table(double temperature, int maxd, double *tab, int32_t *thr);
+
+/* The `seed S` command's master generator (RanMars::init / uniform,
+   src/random_mars.cpp:43-105; src/input.cpp:942-951), host only: writes the
+   first n uniform() draws that follow init(S).  A strict run that starts
+   with `set site range` uses draw 0 for the set generator
+   (RandomPark(double), src/random_park.cpp:45-49; src/set.cpp:598) and draw
+   4 as the site-seed base rseed (src/app_lattice.cpp:274-292). */
+int spk_hip_master_draws(int seed, int n, double *out);
+
+/* Allocate device state for one lattice (AppLattice::init's colour sets and
+   siteseeds, src/app_lattice.cpp:190-295).  Spins start at 1, seeds at 1. */
+int spk_hip_create(const spk_hip_desc *d, spk_hip_lattice **out);
+void spk_hip_destroy(spk_hip_lattice *h);
+
+/* Change kT between runs (AppLattice::set_temperature,
+   src/app_lattice.cpp:893-898); rebuilds the Metropolis table. */
+int spk_hip_set_temperature(spk_hip_lattice *h, double temperature);
+
+/* Copy spins and/or seeds (either pointer may be NULL), length nx*ny*nz,
+   reference site order.  Upload checks 1 <= spin <= Q (AppPotts::init_app,
+   src/app_potts.cpp:78-91) and 1 <= seed <= 2^31-2. */
+int spk_hip_upload(spk_hip_lattice *h, const int32_t *spin, const int32_t *seed);
+int spk_hip_download(spk_hip_lattice *h, int32_t *spin, int32_t *seed);
+
+/* Device-side initial state.
+   set_range: `set site range lo hi` with loop all (src/set.cpp:582-627):
+     site id g gets irandom(hi-lo+1)-1+lo from a RandomPark started at
+     seed0 and advanced g times (Park-Miller jump-ahead).
+   set_value: `set site value v`.
+   init_seeds: siteseeds[i] = tagreset(rseed, id[i], 100)
+     (src/app_lattice.cpp:287-295, src/random_park.cpp:67-73). */
+int spk_hip_set_range(spk_hip_lattice *h, int seed0, int lo, int hi);
+int spk_hip_set_value(spk_hip_lattice *h, int value);
+int spk_hip_init_seeds(spk_hip_lattice *h, double rseed);
+
+/* Run `nsweeps` strict sweeps (colours 1..K each), as nsweeps iterations of
+   AppLattice::iterate_rejection (src/app_lattice.cpp:632-711).
+   naccept_total (nullable): accepted flips over the call (added, not set).
+   naccept_per_sweep / energy_per_sweep (nullable, length nsweeps):
+   cumulative accepted flips since the call began and the total energy
+   (DiagEnergy, src/diag_energy.cpp:53-66) after each sweep. */
+int spk_hip_sweeps(spk_hip_lattice *h, int nsweeps, int64_t *naccept_total,
+                   int64_t *naccept_per_sweep, double *energy_per_sweep);
+
+/* Total energy: sum over sites of #unlike neighbours
+   (src/diag_energy.cpp:53-66, exact in double). */
+int spk_hip_energy(spk_hip_lattice *h, double *energy);
+
+/* Block until all work queued on the handle's stream has finished. */
+int spk_hip_sync(spk_hip_lattice *h);
+
+/* Launch configuration of the per-colour sweep kernel (results do not
+   depend on it: colour/strict is decomposition independent).
+   block = threads per workgroup (64..1024, multiple of 64). */
+int spk_hip_set_launch(spk_hip_lattice *h, int block);
+
+/* Device-side timing of the last spk_hip_sweeps call: milliseconds between
+   HIP events recorded on the handle's stream around the kernel sequence,
+   and the number of sweep-kernel launches it made. */
+int spk_hip_last_timing(spk_hip_lattice *h, double *ms, int64_t *nlaunch);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

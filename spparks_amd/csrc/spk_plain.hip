@@ -1,0 +1,519 @@
+// spk_plain.hip -- libspk_hip.so, single-lattice path: the per-colour strict
+// sweep kernel, the fused energy / Naccept reductions, device-side initial
+// state (set site range, tagreset seeds via Park-Miller jump-ahead), layout
+// conversion, and the C ABI declared in include/spk_hip.h.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "spk_device.h"
+#include "spk_host.h"
+
+using namespace spk;
+
+// ==================================================================== kernels
+
+// One colour pass: every site of colour `color` performs one strict event.
+// Thread t -> sub-grid site (a,b,e) of that colour (x fastest), so the
+// wavefront's own spin/seed loads and stores are unit-stride.
+template <int LAT>
+__global__ __launch_bounds__(1024) void k_sweep_color(int32_t *__restrict__ spin,
+                                                      int32_t *__restrict__ seed,
+                                                      Geom g, Rule r, int color,
+                                                      unsigned long long *__restrict__ acc) {
+  constexpr int NN = NeighCount<LAT>::value;
+  __shared__ int s_thr[32];
+  if (threadIdx.x < 28) s_thr[threadIdx.x] = r.thr[threadIdx.x];
+  __syncthreads();
+
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  int changed = 0, dE = 0;
+  if (t < g.M) {
+    const int ti = (int)t;
+    const int a = ti % g.hx;
+    const int rest = ti / g.hx;
+    const int b = rest % g.hy;
+    const int e = rest / g.hy;
+    int32_t nb[NN];
+    gather<LAT>(spin, g, color, a, b, e, nb);
+    const int64_t self = (int64_t)color * g.M + t;
+    const int32_t old = spin[self];
+    SiteOutcome o = site_event<NN>(r, s_thr, old, seed[self], nb);
+    seed[self] = o.seed;
+    if (o.changed) spin[self] = o.spin;
+    changed = o.changed;
+    dE = o.dE;
+  }
+  block_accumulate(changed, dE, acc, blockIdx.x & (kAccSlots - 1));
+}
+
+// Total energy: sum over all sites of #unlike neighbours.
+template <int LAT>
+__global__ __launch_bounds__(256) void k_energy(const int32_t *__restrict__ spin, Geom g,
+                                                unsigned long long *__restrict__ acc) {
+  constexpr int NN = NeighCount<LAT>::value;
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  int eng = 0;
+  if (t < (int64_t)g.M * g.K) {
+    const int c = (int)(t / g.M);
+    const int ti = (int)(t - (int64_t)c * g.M);
+    const int a = ti % g.hx;
+    const int rest = ti / g.hx;
+    const int b = rest % g.hy;
+    const int e = rest / g.hy;
+    int32_t nb[NN];
+    gather<LAT>(spin, g, c, a, b, e, nb);
+    const int32_t me = spin[t];
+#pragma unroll
+    for (int m = 0; m < NN; m++) eng += (nb[m] != me);
+  }
+  block_accumulate(eng, 0, acc, blockIdx.x & (kAccSlots - 1));
+}
+
+// natural order <-> colour-blocked order
+__global__ void k_to_blocked(const int32_t *__restrict__ nat, int32_t *__restrict__ blk, Geom g) {
+  const int64_t n = (int64_t)g.M * g.K;
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  const int i = (int)(t % g.nx);
+  const int64_t r = t / g.nx;
+  const int j = (int)(r % g.ny);
+  const int k = (int)(r / g.ny);
+  blk[blocked_index(g, i, j, k)] = nat[t];
+}
+
+__global__ void k_to_natural(const int32_t *__restrict__ blk, int32_t *__restrict__ nat, Geom g) {
+  const int64_t n = (int64_t)g.M * g.K;
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  const int i = (int)(t % g.nx);
+  const int64_t r = t / g.nx;
+  const int j = (int)(r % g.ny);
+  const int k = (int)(r / g.ny);
+  nat[t] = blk[blocked_index(g, i, j, k)];
+}
+
+// counts values outside [lo, hi]
+__global__ void k_count_outside(const int32_t *__restrict__ v, int64_t n, int lo, int hi,
+                                unsigned long long *__restrict__ bad) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  int out = 0;
+  if (t < n) out = (v[t] < lo || v[t] > hi);
+  out = wave_sum(out);
+  if ((threadIdx.x & 63) == 0 && out) atomicAdd(bad, (unsigned long long)out);
+}
+
+// `set site range lo hi`, loop all: site id gid = t+1 draws
+// irandom(range) from a generator advanced gid times from seed0
+// (src/set.cpp:598-627); 16807^gid mod m from the binary powers in pw.
+struct PowTable { uint32_t p[32]; };
+
+__global__ void k_set_range(int32_t *__restrict__ spin, Geom g, int seed0, int lo, int range,
+                            PowTable pw) {
+  const int64_t n = (int64_t)g.M * g.K;
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  const uint64_t gid = (uint64_t)t + 1;
+  uint32_t s = (uint32_t)seed0;
+#pragma unroll
+  for (int bit = 0; bit < 32; bit++)
+    if ((gid >> bit) & 1u) s = pm_mulmod(s, pw.p[bit]);
+  double u = kAM * (double)(int32_t)s;            // RandomPark::uniform
+  int v = (int)(u * (double)range) + 1;           // irandom(range)
+  if (v > range) v = range;
+  const int i = (int)(t % g.nx);
+  const int64_t r = t / g.nx;
+  const int j = (int)(r % g.ny);
+  const int k = (int)(r / g.ny);
+  spin[blocked_index(g, i, j, k)] = v - 1 + lo;
+}
+
+__global__ void k_fill(int32_t *__restrict__ v, int64_t n, int32_t value) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n) v[t] = value;
+}
+
+// siteseeds = tagreset(rseed, id, 100) (src/random_park.cpp:67-73).
+// xbase = rseed*IM is formed on the host; x = xbase + id is one rounded
+// addition as in the reference (no contraction: built with
+// -ffp-contract=off).  For 0 <= x < 2*IM, fmod(x, IM) is x or the exact
+// difference x-IM (Sterbenz).  The 100 warm-up steps are one multiply by
+// 16807^100 mod m.
+__global__ void k_init_seeds(int32_t *__restrict__ seed, Geom g, double xbase, uint32_t a100) {
+  const int64_t n = (int64_t)g.M * g.K;
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  const double im = (double)kIM;
+  double x = xbase + (double)(t + 1);
+  while (x >= im) x -= im;
+  int32_t s = (int32_t)x;
+  if (s == 0) s = 1;
+  s = (int32_t)pm_mulmod((uint32_t)s, a100);
+  const int i = (int)(t % g.nx);
+  const int64_t r = t / g.nx;
+  const int j = (int)(r % g.ny);
+  const int k = (int)(r / g.ny);
+  seed[blocked_index(g, i, j, k)] = s;
+}
+
+// ===================================================================== host
+
+struct spk_hip_lattice {
+  spk_hip_desc d;
+  Geom g;
+  Rule rule;
+  int64_t N = 0;
+  int device = 0;
+  hipStream_t stream = nullptr;
+  int32_t *spin = nullptr, *seed = nullptr, *stage = nullptr;
+  unsigned long long *acc = nullptr;   // [cap][kAccSlots][2]
+  int acc_cap = 0;
+  int block = 256;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  double last_ms = 0.0;
+  int64_t last_nlaunch = 0;
+  bool energy_valid = false;
+  int64_t energy = 0;                  // total energy, exact integer
+  std::vector<unsigned long long> hacc;
+};
+
+namespace {
+
+int launch_sweep(spk_hip_lattice *h, int color, unsigned long long *acc) {
+  const int bs = h->block;
+  const unsigned grid = (unsigned)((h->g.M + bs - 1) / bs);
+  switch (h->g.lat) {
+    case SPK_HIP_SQ_4N:
+      hipLaunchKernelGGL(k_sweep_color<SPK_HIP_SQ_4N>, dim3(grid), dim3(bs), 0, h->stream,
+                         h->spin, h->seed, h->g, h->rule, color, acc);
+      break;
+    case SPK_HIP_SQ_8N:
+      hipLaunchKernelGGL(k_sweep_color<SPK_HIP_SQ_8N>, dim3(grid), dim3(bs), 0, h->stream,
+                         h->spin, h->seed, h->g, h->rule, color, acc);
+      break;
+    case SPK_HIP_SC_6N:
+      hipLaunchKernelGGL(k_sweep_color<SPK_HIP_SC_6N>, dim3(grid), dim3(bs), 0, h->stream,
+                         h->spin, h->seed, h->g, h->rule, color, acc);
+      break;
+    default:
+      hipLaunchKernelGGL(k_sweep_color<SPK_HIP_SC_26N>, dim3(grid), dim3(bs), 0, h->stream,
+                         h->spin, h->seed, h->g, h->rule, color, acc);
+      break;
+  }
+  return check_hip(hipGetLastError(), "sweep kernel launch");
+}
+
+int launch_energy(spk_hip_lattice *h, unsigned long long *acc) {
+  const int bs = 256;
+  const unsigned grid = (unsigned)((h->N + bs - 1) / bs);
+  switch (h->g.lat) {
+    case SPK_HIP_SQ_4N:
+      hipLaunchKernelGGL(k_energy<SPK_HIP_SQ_4N>, dim3(grid), dim3(bs), 0, h->stream, h->spin, h->g, acc);
+      break;
+    case SPK_HIP_SQ_8N:
+      hipLaunchKernelGGL(k_energy<SPK_HIP_SQ_8N>, dim3(grid), dim3(bs), 0, h->stream, h->spin, h->g, acc);
+      break;
+    case SPK_HIP_SC_6N:
+      hipLaunchKernelGGL(k_energy<SPK_HIP_SC_6N>, dim3(grid), dim3(bs), 0, h->stream, h->spin, h->g, acc);
+      break;
+    default:
+      hipLaunchKernelGGL(k_energy<SPK_HIP_SC_26N>, dim3(grid), dim3(bs), 0, h->stream, h->spin, h->g, acc);
+      break;
+  }
+  return check_hip(hipGetLastError(), "energy kernel launch");
+}
+
+unsigned grid_for(int64_t n, int bs) { return (unsigned)((n + bs - 1) / bs); }
+
+int ensure_acc(spk_hip_lattice *h, int nsweeps) {
+  if (nsweeps <= h->acc_cap) return SPK_HIP_OK;
+  int cap = h->acc_cap ? h->acc_cap : 16;
+  while (cap < nsweeps) cap *= 2;
+  unsigned long long *p = nullptr;
+  if (hipMalloc(&p, (size_t)cap * kAccSlots * 2 * sizeof(unsigned long long)) != hipSuccess)
+    return set_error(SPK_HIP_ENOMEM, "device allocation of %d accumulator rows failed", cap);
+  if (h->acc) (void)hipFree(h->acc);
+  h->acc = p;
+  h->acc_cap = cap;
+  h->hacc.resize((size_t)cap * kAccSlots * 2);
+  return SPK_HIP_OK;
+}
+
+// sums the slots of accumulator row `row` (already copied to h->hacc)
+void row_sum(const spk_hip_lattice *h, int row, int64_t *x, int64_t *y) {
+  unsigned long long sx = 0, sy = 0;
+  const unsigned long long *p = h->hacc.data() + (size_t)row * kAccSlots * 2;
+  for (int s = 0; s < kAccSlots; s++) { sx += p[2 * s]; sy += p[2 * s + 1]; }
+  *x = (int64_t)sx;
+  *y = (int64_t)sy;
+}
+
+int compute_energy(spk_hip_lattice *h, int64_t *e) {
+  int rc = ensure_acc(h, 1);
+  if (rc) return rc;
+  const size_t bytes = kAccSlots * 2 * sizeof(unsigned long long);
+  if ((rc = check_hip(hipMemsetAsync(h->acc, 0, bytes, h->stream), "memset"))) return rc;
+  if ((rc = launch_energy(h, h->acc))) return rc;
+  if ((rc = check_hip(hipMemcpyAsync(h->hacc.data(), h->acc, bytes, hipMemcpyDeviceToHost, h->stream),
+                      "energy readback")))
+    return rc;
+  if ((rc = check_hip(hipStreamSynchronize(h->stream), "energy sync"))) return rc;
+  int64_t y;
+  row_sum(h, 0, e, &y);
+  h->energy = *e;
+  h->energy_valid = true;
+  return SPK_HIP_OK;
+}
+
+int validate_on_device(spk_hip_lattice *h, const int32_t *buf, int lo, int hi, int64_t *nbad) {
+  int rc = ensure_acc(h, 1);
+  if (rc) return rc;
+  if ((rc = check_hip(hipMemsetAsync(h->acc, 0, sizeof(unsigned long long), h->stream), "memset")))
+    return rc;
+  hipLaunchKernelGGL(k_count_outside, dim3(grid_for(h->N, 256)), dim3(256), 0, h->stream, buf, h->N, lo,
+                     hi, h->acc);
+  if ((rc = check_hip(hipGetLastError(), "validate launch"))) return rc;
+  unsigned long long v = 0;
+  if ((rc = check_hip(hipMemcpyAsync(&v, h->acc, sizeof(v), hipMemcpyDeviceToHost, h->stream), "readback")))
+    return rc;
+  if ((rc = check_hip(hipStreamSynchronize(h->stream), "validate sync"))) return rc;
+  *nbad = (int64_t)v;
+  return SPK_HIP_OK;
+}
+
+}  // namespace
+
+// ====================================================================== ABI
+
+extern "C" {
+
+int spk_hip_ncolors(const spk_hip_desc *d) {
+  Geom g;
+  return make_geom(d, &g) ? SPK_HIP_EINVAL : g.K;
+}
+
+int spk_hip_create(const spk_hip_desc *d, spk_hip_lattice **out) {
+  if (!d || !out) return set_error(SPK_HIP_EINVAL, "null argument");
+  *out = nullptr;
+  Geom g;
+  int rc = make_geom(d, &g);
+  if (rc) return rc;
+  Rule r;
+  if ((rc = make_rule(d->app, d->nspins, d->lattice, d->temperature, &r))) return rc;
+  spk_hip_lattice *h = new (std::nothrow) spk_hip_lattice();
+  if (!h) return set_error(SPK_HIP_ENOMEM, "host allocation failed");
+  h->d = *d;
+  h->g = g;
+  h->rule = r;
+  h->N = (int64_t)g.M * g.K;
+  if ((rc = check_hip(hipGetDevice(&h->device), "hipGetDevice")) ||
+      (rc = check_hip(hipStreamCreate(&h->stream), "hipStreamCreate")) ||
+      (rc = check_hip(hipEventCreate(&h->ev0), "hipEventCreate")) ||
+      (rc = check_hip(hipEventCreate(&h->ev1), "hipEventCreate"))) {
+    spk_hip_destroy(h);
+    return rc;
+  }
+  const size_t bytes = (size_t)h->N * sizeof(int32_t);
+  if (hipMalloc(&h->spin, bytes) != hipSuccess || hipMalloc(&h->seed, bytes) != hipSuccess ||
+      hipMalloc(&h->stage, bytes) != hipSuccess) {
+    spk_hip_destroy(h);
+    return set_error(SPK_HIP_ENOMEM, "device allocation of %lld sites failed", (long long)h->N);
+  }
+  hipLaunchKernelGGL(k_fill, dim3(grid_for(h->N, 256)), dim3(256), 0, h->stream, h->spin, h->N, 1);
+  hipLaunchKernelGGL(k_fill, dim3(grid_for(h->N, 256)), dim3(256), 0, h->stream, h->seed, h->N, 1);
+  if ((rc = check_hip(hipStreamSynchronize(h->stream), "create sync"))) {
+    spk_hip_destroy(h);
+    return rc;
+  }
+  *out = h;
+  return SPK_HIP_OK;
+}
+
+void spk_hip_destroy(spk_hip_lattice *h) {
+  if (!h) return;
+  if (h->stream) (void)hipStreamSynchronize(h->stream);
+  if (h->spin) (void)hipFree(h->spin);
+  if (h->seed) (void)hipFree(h->seed);
+  if (h->stage) (void)hipFree(h->stage);
+  if (h->acc) (void)hipFree(h->acc);
+  if (h->ev0) (void)hipEventDestroy(h->ev0);
+  if (h->ev1) (void)hipEventDestroy(h->ev1);
+  if (h->stream) (void)hipStreamDestroy(h->stream);
+  delete h;
+}
+
+int spk_hip_set_temperature(spk_hip_lattice *h, double temperature) {
+  if (!h) return set_error(SPK_HIP_EINVAL, "null handle");
+  Rule r;
+  int rc = make_rule(h->d.app, h->d.nspins, h->d.lattice, temperature, &r);
+  if (rc) return rc;
+  h->rule = r;
+  h->d.temperature = temperature;
+  return SPK_HIP_OK;
+}
+
+int spk_hip_upload(spk_hip_lattice *h, const int32_t *spin, const int32_t *seed) {
+  if (!h) return set_error(SPK_HIP_EINVAL, "null handle");
+  const size_t bytes = (size_t)h->N * sizeof(int32_t);
+  int rc;
+  const int qmax = h->d.app == SPK_HIP_POTTS ? h->d.nspins : 2;
+  if (spin) {
+    if ((rc = check_hip(hipMemcpyAsync(h->stage, spin, bytes, hipMemcpyHostToDevice, h->stream), "upload")))
+      return rc;
+    int64_t nbad = 0;
+    if ((rc = validate_on_device(h, h->stage, 1, qmax, &nbad))) return rc;
+    if (nbad) return set_error(SPK_HIP_EVALUE, "One or more sites have invalid values (%lld)", (long long)nbad);
+    hipLaunchKernelGGL(k_to_blocked, dim3(grid_for(h->N, 256)), dim3(256), 0, h->stream, h->stage, h->spin,
+                       h->g);
+    if ((rc = check_hip(hipGetLastError(), "layout kernel"))) return rc;
+    h->energy_valid = false;
+  }
+  if (seed) {
+    if ((rc = check_hip(hipMemcpyAsync(h->stage, seed, bytes, hipMemcpyHostToDevice, h->stream), "upload")))
+      return rc;
+    int64_t nbad = 0;
+    if ((rc = validate_on_device(h, h->stage, 1, kIM - 1, &nbad))) return rc;
+    if (nbad) return set_error(SPK_HIP_EVALUE, "%lld seeds outside [1, 2^31-2]", (long long)nbad);
+    hipLaunchKernelGGL(k_to_blocked, dim3(grid_for(h->N, 256)), dim3(256), 0, h->stream, h->stage, h->seed,
+                       h->g);
+    if ((rc = check_hip(hipGetLastError(), "layout kernel"))) return rc;
+  }
+  return check_hip(hipStreamSynchronize(h->stream), "upload sync");
+}
+
+int spk_hip_download(spk_hip_lattice *h, int32_t *spin, int32_t *seed) {
+  if (!h) return set_error(SPK_HIP_EINVAL, "null handle");
+  const size_t bytes = (size_t)h->N * sizeof(int32_t);
+  int rc;
+  for (int w = 0; w < 2; w++) {
+    int32_t *dst = w == 0 ? spin : seed;
+    if (!dst) continue;
+    hipLaunchKernelGGL(k_to_natural, dim3(grid_for(h->N, 256)), dim3(256), 0, h->stream,
+                       w == 0 ? h->spin : h->seed, h->stage, h->g);
+    if ((rc = check_hip(hipGetLastError(), "layout kernel"))) return rc;
+    if ((rc = check_hip(hipMemcpyAsync(dst, h->stage, bytes, hipMemcpyDeviceToHost, h->stream), "download")))
+      return rc;
+    if ((rc = check_hip(hipStreamSynchronize(h->stream), "download sync"))) return rc;
+  }
+  return SPK_HIP_OK;
+}
+
+int spk_hip_set_range(spk_hip_lattice *h, int seed0, int lo, int hi) {
+  if (!h) return set_error(SPK_HIP_EINVAL, "null handle");
+  if (hi < lo) return set_error(SPK_HIP_EINVAL, "set range: hi < lo");
+  if (seed0 < 1 || seed0 >= kIM) return set_error(SPK_HIP_EINVAL, "set range: seed0 outside [1, 2^31-2]");
+  PowTable pw;
+  uint32_t p = kIA;
+  for (int b = 0; b < 32; b++) {
+    pw.p[b] = p;
+    p = pm_mulmod(p, p);
+  }
+  hipLaunchKernelGGL(k_set_range, dim3(grid_for(h->N, 256)), dim3(256), 0, h->stream, h->spin, h->g, seed0, lo,
+                     hi - lo + 1, pw);
+  int rc = check_hip(hipGetLastError(), "set_range launch");
+  if (rc) return rc;
+  h->energy_valid = false;
+  return check_hip(hipStreamSynchronize(h->stream), "set_range sync");
+}
+
+int spk_hip_set_value(spk_hip_lattice *h, int value) {
+  if (!h) return set_error(SPK_HIP_EINVAL, "null handle");
+  hipLaunchKernelGGL(k_fill, dim3(grid_for(h->N, 256)), dim3(256), 0, h->stream, h->spin, h->N, value);
+  int rc = check_hip(hipGetLastError(), "set_value launch");
+  if (rc) return rc;
+  h->energy_valid = false;
+  return check_hip(hipStreamSynchronize(h->stream), "set_value sync");
+}
+
+int spk_hip_init_seeds(spk_hip_lattice *h, double rseed) {
+  if (!h) return set_error(SPK_HIP_EINVAL, "null handle");
+  if (!(rseed >= 0.0 && rseed < 1.0)) return set_error(SPK_HIP_EINVAL, "init_seeds: rseed outside [0,1)");
+  double xbase = rseed * (double)kIM;
+  uint32_t a100 = 1;
+  for (int w = 0; w < 100; w++) a100 = pm_mulmod(a100, kIA);
+  hipLaunchKernelGGL(k_init_seeds, dim3(grid_for(h->N, 256)), dim3(256), 0, h->stream, h->seed, h->g, xbase,
+                     a100);
+  int rc = check_hip(hipGetLastError(), "init_seeds launch");
+  if (rc) return rc;
+  return check_hip(hipStreamSynchronize(h->stream), "init_seeds sync");
+}
+
+int spk_hip_sweeps(spk_hip_lattice *h, int nsweeps, int64_t *naccept_total, int64_t *naccept_per_sweep,
+                   double *energy_per_sweep) {
+  if (!h) return set_error(SPK_HIP_EINVAL, "null handle");
+  if (nsweeps < 0) return set_error(SPK_HIP_EINVAL, "nsweeps < 0");
+  if (nsweeps == 0) return SPK_HIP_OK;
+  int rc;
+  int64_t e0 = 0;
+  if (energy_per_sweep) {
+    if (h->energy_valid) e0 = h->energy;
+    else if ((rc = compute_energy(h, &e0))) return rc;
+  }
+  if ((rc = ensure_acc(h, nsweeps))) return rc;
+  const size_t row = kAccSlots * 2 * sizeof(unsigned long long);
+  if ((rc = check_hip(hipMemsetAsync(h->acc, 0, row * nsweeps, h->stream), "memset"))) return rc;
+  if ((rc = check_hip(hipEventRecord(h->ev0, h->stream), "event"))) return rc;
+  for (int s = 0; s < nsweeps; s++)
+    for (int c = 0; c < h->g.K; c++)
+      if ((rc = launch_sweep(h, c, h->acc + (size_t)s * kAccSlots * 2))) return rc;
+  if ((rc = check_hip(hipEventRecord(h->ev1, h->stream), "event"))) return rc;
+  if ((rc = check_hip(hipMemcpyAsync(h->hacc.data(), h->acc, row * nsweeps, hipMemcpyDeviceToHost, h->stream),
+                      "accumulator readback")))
+    return rc;
+  if ((rc = check_hip(hipStreamSynchronize(h->stream), "sweeps sync"))) return rc;
+  float ms = 0.f;
+  if ((rc = check_hip(hipEventElapsedTime(&ms, h->ev0, h->ev1), "event time"))) return rc;
+  h->last_ms = ms;
+  h->last_nlaunch = (int64_t)nsweeps * h->g.K;
+
+  int64_t nacc = 0, eng = e0;
+  const bool track = h->energy_valid || energy_per_sweep;
+  for (int s = 0; s < nsweeps; s++) {
+    int64_t a, de;
+    row_sum(h, s, &a, &de);
+    nacc += a;
+    eng += 2 * de;   // a flip changes site i's and its neighbours' counts
+    if (naccept_per_sweep) naccept_per_sweep[s] = nacc;
+    if (energy_per_sweep) energy_per_sweep[s] = (double)eng;
+  }
+  if (naccept_total) *naccept_total += nacc;
+  if (track) h->energy = eng;
+  h->energy_valid = track;
+  return SPK_HIP_OK;
+}
+
+int spk_hip_energy(spk_hip_lattice *h, double *energy) {
+  if (!h || !energy) return set_error(SPK_HIP_EINVAL, "null argument");
+  int64_t e;
+  int rc = compute_energy(h, &e);
+  if (rc) return rc;
+  *energy = (double)e;
+  return SPK_HIP_OK;
+}
+
+int spk_hip_sync(spk_hip_lattice *h) {
+  if (!h) return set_error(SPK_HIP_EINVAL, "null handle");
+  return check_hip(hipStreamSynchronize(h->stream), "sync");
+}
+
+int spk_hip_set_launch(spk_hip_lattice *h, int block) {
+  if (!h) return set_error(SPK_HIP_EINVAL, "null handle");
+  if (block < 64 || block > 1024 || block % 64)
+    return set_error(SPK_HIP_EINVAL, "block must be a multiple of 64 in [64, 1024]");
+  h->block = block;
+  return SPK_HIP_OK;
+}
+
+int spk_hip_last_timing(spk_hip_lattice *h, double *ms, int64_t *nlaunch) {
+  if (!h) return set_error(SPK_HIP_EINVAL, "null handle");
+  if (ms) *ms = h->last_ms;
+  if (nlaunch) *nlaunch = h->last_nlaunch;
+  return SPK_HIP_OK;
+}
+
+}  // extern "C"

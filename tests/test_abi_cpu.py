@@ -1,0 +1,80 @@
+"""libspk_hip.so without a GPU: it loads, exports every symbol its header
+declares, and its host-only entry points agree with the oracle."""
+import ctypes as C
+import math
+import os
+import re
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def header_symbols():
+    text = open(os.path.join(ROOT, "include", "spk_hip.h")).read()
+    return sorted(set(re.findall(r"\b(spk_hip_[a-z_0-9]+)\s*\(", text)))
+
+
+def test_header_matches_binding_list():
+    from spparks_amd import hip
+    assert sorted(hip.ABI_SYMBOLS) == header_symbols()
+
+
+def test_library_exports_every_declared_symbol():
+    from spparks_amd import hip
+    L = hip.load_library()
+    for name in header_symbols():
+        assert hasattr(L, name), name
+
+
+def test_no_fallback_when_library_missing(tmp_path):
+    from spparks_amd import hip
+    with pytest.raises(OSError):
+        hip.load_library(str(tmp_path / "missing.so"))
+
+
+def test_master_draws_match_oracle(orc):
+    import spparks_amd as S
+    for seed in [1, 56789, 12345, 24680, 900000001, 2**31 - 1]:
+        m = orc.RanMars(seed)
+        ref = [m.uniform() for _ in range(7)]
+        assert list(S.master_draws(seed, 7)) == ref
+
+
+@pytest.mark.parametrize("kt", [0.5, 2.269, 0.1, 1.0, 3.5, 1e-3, 100.0])
+def test_metropolis_thresholds(kt):
+    """thr[d] is the exact boundary: AM*(thr-1) <= exp(-d/kT) < AM*thr."""
+    import spparks_amd as S
+    am = 1.0 / 2147483647
+    tab, thr = S.metropolis_table(kt, 26)
+    for d in range(1, 27):
+        assert tab[d] == math.exp((-d) * (1.0 / kt))
+        t = int(thr[d])
+        assert 1 <= t <= 2147483647
+        if t < 2147483647:
+            assert am * t > tab[d]
+        if t > 1:
+            assert not (am * (t - 1) > tab[d])
+    t = int(thr[0])
+    assert am * t >= 0.5 and am * (t - 1) < 0.5
+
+
+def test_ncolors_and_validation():
+    import spparks_amd as S
+    assert S.ncolors("potts", "sq/8n", 128, 128) == 4
+    assert S.ncolors("ising", "sq/4n", 64, 48) == 2
+    assert S.ncolors("potts", "sc/26n", 10, 8, 6) == 8
+    assert S.ncolors("potts", "sc/6n", 4, 4, 4) == 2
+    with pytest.raises(S.HipError, match="incommensurate"):
+        S.ncolors("potts", "sq/8n", 7, 8)
+    with pytest.raises(S.HipError):
+        S.ncolors("potts", "sc/26n", 8, 8, 3)
+
+
+def test_table_rejects_bad_temperature():
+    import spparks_amd as S
+    with pytest.raises(S.HipError):
+        S.metropolis_table(0.0, 8)
+    with pytest.raises(S.HipError):
+        S.metropolis_table(-1.0, 8)

@@ -1,0 +1,58 @@
+"""The kernels' own index math and site rule (spk_device.h), run on the host by
+tests/emul/spk_emul under AddressSanitizer, against the oracle.  Catches
+layout, wrap and neighbour-set mistakes before anything reaches a GPU."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+EMUL = os.path.join(HERE, "emul", "spk_emul")
+
+CASES = [
+    ("potts", "sq/8n", (16, 12, 1), 100, 0.5, 56789, 5),
+    ("potts", "sq/8n", (2, 2, 1), 3, 0.8, 11, 4),
+    ("potts", "sq/8n", (2, 6, 1), 3, 0.8, 12, 4),
+    ("potts", "sq/4n", (18, 10, 1), 5, 1.1, 5, 4),
+    ("potts", "sq/4n", (2, 4, 1), 5, 1.1, 6, 4),
+    ("ising", "sq/4n", (64, 48, 1), 2, 2.269, 777, 10),
+    ("ising", "sq/4n", (2, 2, 1), 2, 2.0, 3, 4),
+    ("potts", "sc/26n", (10, 8, 6), 10000, 0.5, 4242, 4),
+    ("potts", "sc/26n", (2, 2, 2), 4, 0.6, 8, 4),
+    ("potts", "sc/26n", (2, 4, 6), 4, 0.6, 9, 4),
+    ("potts", "sc/26n", (6, 4, 2), 7, 0.0, 91, 4),
+    ("ising", "sc/26n", (8, 6, 4), 2, 3.0, 17, 4),
+    ("potts", "sc/6n", (8, 6, 4), 9, 0.9, 23, 4),
+    ("potts", "sc/6n", (2, 2, 2), 9, 0.9, 24, 4),
+    ("ising", "sc/6n", (4, 4, 2), 2, 1.0, 29, 4),
+    ("ising", "sq/8n", (12, 12, 1), 2, 0.0, 37, 4),
+]
+
+
+@pytest.mark.parametrize("case", CASES, ids=lambda c: "%s-%s-%s" % (c[0], c[1], "x".join(map(str, c[2]))))
+def test_emulated_kernel_matches_oracle(orc, tmp_path, case):
+    if not os.path.exists(EMUL):
+        pytest.skip("tests/emul/spk_emul not built (run __graft_entry__.build())")
+    app, lat, shape, q, kt, seed, n = case
+    d = orc.make_desc(app, lat, *shape, q, kt)
+    spin, seeds = orc.script_init(d, seed, 1, q)
+    fin, fout = tmp_path / "in.bin", tmp_path / "out.bin"
+    with open(fin, "wb") as f:
+        f.write(spin.tobytes())
+        f.write(seeds.tobytes())
+    nacc, eng = orc.sweeps(d, spin, seeds, n)
+    env = dict(os.environ, ASAN_OPTIONS="detect_leaks=0")
+    r = subprocess.run([EMUL, str(orc.APP[app]), str(orc.LATTICE[lat]), *map(str, shape), str(q),
+                        repr(kt), str(n), str(fin), str(fout)], env=env, capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr[-2000:]
+    N = spin.size
+    raw = open(fout, "rb").read()
+    es = np.frombuffer(raw, np.int32, N, 0)
+    ek = np.frombuffer(raw, np.int32, N, 4 * N)
+    en = np.frombuffer(raw, np.int64, n, 8 * N)
+    ee = np.frombuffer(raw, np.int64, n, 8 * N + 8 * n)
+    assert np.array_equal(es, spin)
+    assert np.array_equal(ek, seeds)
+    assert np.array_equal(en, nacc)
+    assert np.array_equal(ee.astype(np.float64), eng)

@@ -1,0 +1,192 @@
+"""GPU parity of the single-lattice path (libspk_hip.so) against the oracle:
+bit-exact spins, seeds, cumulative Naccept and energy after every sweep."""
+import json
+import os
+import sys
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+GOLD = os.path.join(os.path.dirname(__file__), "golden")
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def gpu_init(S, app, lat, shape, q, kt, seed, lo=1, hi=None):
+    hi = q if hi is None else hi
+    L = S.Lattice(app, lat, *shape, nspins=q, temperature=kt)
+    s0, rseed = S.strict_init_params(seed)
+    L.set_range(s0, lo, hi)
+    L.init_seeds(rseed)
+    return L
+
+
+def oracle_run(orc, app, lat, shape, q, kt, seed, nsweeps, lo=1, hi=None):
+    hi = q if hi is None else hi
+    d = orc.make_desc(app, lat, *shape, q, kt)
+    spin, seeds = orc.script_init(d, seed, lo, hi)
+    spin0, seeds0 = spin.copy(), seeds.copy()
+    nacc, eng = orc.sweeps(d, spin, seeds, nsweeps)
+    return spin0, seeds0, spin, seeds, nacc, eng
+
+
+CASES = [
+    # app, lattice, shape, Q, kT, seed, sweeps
+    ("potts", "sq/8n", (16, 12, 1), 100, 0.5, 56789, 7),
+    ("potts", "sq/8n", (2, 2, 1), 3, 0.8, 11, 5),
+    ("potts", "sq/4n", (18, 10, 1), 5, 1.1, 5, 6),
+    ("ising", "sq/4n", (64, 48, 1), 2, 2.269, 777, 40),
+    ("ising", "sq/4n", (2, 2, 1), 2, 2.0, 3, 5),
+    ("potts", "sc/26n", (10, 8, 6), 10000, 0.5, 4242, 6),
+    ("potts", "sc/26n", (2, 2, 2), 4, 0.6, 8, 5),
+    ("potts", "sc/26n", (6, 4, 2), 7, 0.0, 91, 4),
+    ("ising", "sc/26n", (8, 6, 4), 2, 3.0, 17, 5),
+    ("potts", "sc/6n", (8, 6, 4), 9, 0.9, 23, 5),
+    ("ising", "sc/6n", (4, 4, 2), 2, 1.0, 29, 5),
+    ("potts", "sq/8n", (34, 66, 1), 1, 0.5, 31, 3),
+    ("ising", "sq/8n", (12, 12, 1), 2, 0.0, 37, 4),
+    ("potts", "sq/4n", (130, 6, 1), 1000000, 0.15, 41, 3),
+]
+
+
+@pytest.mark.parametrize("case", CASES, ids=lambda c: "%s-%s-%s" % (c[0], c[1], "x".join(map(str, c[2]))))
+def test_small_lattice_bit_exact(orc, case):
+    import spparks_amd as S
+    app, lat, shape, q, kt, seed, n = case
+    say("case", case)
+    spin0, seeds0, spin1, seeds1, nacc, eng = oracle_run(orc, app, lat, shape, q, kt, seed, n)
+    with gpu_init(S, app, lat, shape, q, kt, seed) as L:
+        gs0, gk0 = L.download()
+        assert np.array_equal(gs0, spin0), "set site range on device"
+        assert np.array_equal(gk0, seeds0), "tagreset seeds on device"
+        assert L.energy() == orc.energy(orc.make_desc(app, lat, *shape, q, kt), spin0)
+        tot, gn, ge = L.sweeps(n)
+        gs1, gk1 = L.download()
+    assert np.array_equal(gs1, spin1)
+    assert np.array_equal(gk1, seeds1)
+    assert np.array_equal(gn, nacc)
+    assert np.array_equal(ge, eng)
+    assert tot == nacc[-1]
+
+
+def test_cfg1_known_answers_on_gpu():
+    import spparks_amd as S
+    g = json.load(open(os.path.join(GOLD, "cfg1_known_answers.json")))
+    with gpu_init(S, g["app"], g["lattice"], (g["nx"], g["ny"], g["nz"]), g["nspins"],
+                  g["temperature"], g["seed"]) as L:
+        assert L.energy() == g["initial_energy"]
+        tot, nacc, eng = L.sweeps(g["nsweeps"])
+    for sweep, na, e in g["rows"]:
+        assert nacc[sweep - 1] == na
+        assert eng[sweep - 1] == e
+
+
+def test_upload_download_roundtrip_and_sweeps_from_upload(orc):
+    import spparks_amd as S
+    d = orc.make_desc("potts", "sc/26n", 12, 10, 8, 300, 0.4)
+    spin, seeds = orc.script_init(d, 1234, 1, 300)
+    with S.Lattice("potts", "sc/26n", 12, 10, 8, nspins=300, temperature=0.4) as L:
+        L.upload(spin, seeds)
+        s, k = L.download()
+        assert np.array_equal(s, spin) and np.array_equal(k, seeds)
+        _, gn, ge = L.sweeps(4)
+        s, k = L.download()
+    nacc, eng = orc.sweeps(d, spin, seeds, 4)
+    assert np.array_equal(s, spin) and np.array_equal(k, seeds)
+    assert np.array_equal(gn, nacc) and np.array_equal(ge, eng)
+
+
+def test_invalid_values_rejected():
+    import spparks_amd as S
+    with S.Lattice("potts", "sq/8n", 8, 8, nspins=5, temperature=1.0) as L:
+        bad = np.full(64, 3, np.int32)
+        bad[17] = 6
+        with pytest.raises(S.HipError, match="invalid values"):
+            L.upload(bad, None)
+        bad[17] = 0
+        with pytest.raises(S.HipError, match="invalid values"):
+            L.upload(bad, None)
+        with pytest.raises(S.HipError):
+            L.upload(None, np.zeros(64, np.int32))
+
+
+@pytest.mark.parametrize("block", [64, 128, 512, 1024])
+def test_launch_shape_independence(block):
+    """colour/strict is decomposition independent: any workgroup size gives
+    the same trajectory."""
+    import spparks_amd as S
+    ref = None
+    with gpu_init(S, "potts", "sc/26n", (16, 12, 10), 50, 0.7, 99) as L:
+        L.set_launch(block)
+        _, nacc, eng = L.sweeps(5)
+        spin, seeds = L.download()
+    with gpu_init(S, "potts", "sc/26n", (16, 12, 10), 50, 0.7, 99) as L:
+        _, nacc2, eng2 = L.sweeps(5)
+        spin2, seeds2 = L.download()
+    assert np.array_equal(spin, spin2) and np.array_equal(seeds, seeds2)
+    assert np.array_equal(nacc, nacc2) and np.array_equal(eng, eng2)
+
+
+def test_split_calls_equal_one_call():
+    """Consecutive runs continue exactly (seeds persist between runs)."""
+    import spparks_amd as S
+    with gpu_init(S, "ising", "sq/4n", (32, 32, 1), 2, 2.269, 5) as L:
+        _, n1, e1 = L.sweeps(3)
+        _, n2, e2 = L.sweeps(4)
+        a = L.download()
+    with gpu_init(S, "ising", "sq/4n", (32, 32, 1), 2, 2.269, 5) as L:
+        _, n, e = L.sweeps(7)
+        b = L.download()
+    assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
+    assert np.array_equal(np.concatenate([n1, n2 + n1[-1]]), n)
+    assert np.array_equal(np.concatenate([e1, e2]), e)
+
+
+def test_temperature_change_between_runs(orc):
+    import spparks_amd as S
+    d = orc.make_desc("potts", "sq/8n", 20, 20, 1, 8, 1.5)
+    spin, seeds = orc.script_init(d, 6, 1, 8)
+    with S.Lattice("potts", "sq/8n", 20, 20, nspins=8, temperature=1.5) as L:
+        L.upload(spin, seeds)
+        L.sweeps(2)
+        L.set_temperature(0.0)
+        L.sweeps(2)
+        L.set_temperature(0.25)
+        _, gn, ge = L.sweeps(2)
+        s, k = L.download()
+    orc.sweeps(d, spin, seeds, 2)
+    for t in (0.0, 0.25):
+        d = orc.make_desc("potts", "sq/8n", 20, 20, 1, 8, t)
+        nacc, eng = orc.sweeps(d, spin, seeds, 2)
+    assert np.array_equal(s, spin) and np.array_equal(k, seeds)
+    assert np.array_equal(ge, eng)
+
+
+# ------------------------------------------------------- full-size shapes
+
+FULL = [
+    ("ising", "sq/4n", (4096, 4096, 1), 2, 2.269, 12345, 2),
+    ("potts", "sc/26n", (256, 256, 256), 10000, 0.5, 24680, 2),
+]
+
+
+@pytest.mark.parametrize("case", FULL, ids=["cfg2-ising-4096sq", "cfg3-potts-256cube"])
+def test_full_size_first_sweeps_bit_exact(orc, case):
+    import spparks_amd as S
+    app, lat, shape, q, kt, seed, n = case
+    say("full-size case", case)
+    spin0, seeds0, spin1, seeds1, nacc, eng = oracle_run(orc, app, lat, shape, q, kt, seed, n)
+    with gpu_init(S, app, lat, shape, q, kt, seed) as L:
+        gs0, gk0 = L.download()
+        assert np.array_equal(gs0, spin0)
+        assert np.array_equal(gk0, seeds0)
+        _, gn, ge = L.sweeps(n)
+        gs1, gk1 = L.download()
+        assert L.energy() == eng[-1]
+    assert np.array_equal(gn, nacc) and np.array_equal(ge, eng)
+    assert np.array_equal(gs1, spin1) and np.array_equal(gk1, seeds1)
