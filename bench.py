@@ -1,0 +1,180 @@
+#!/usr/bin/env python3
+"""bench.py -- strict colour sweep throughput on one MI355X.
+
+    python bench.py [--steps K] [--warmup W] [--workload cfg2]
+
+A step is one strict sweep (every colour once) over the whole lattice, i.e.
+nx*ny*nz site updates.  The default workload is the 4096x4096 sq/4n Ising
+lattice at kT=2.269 (1000 sweeps); `--workload` selects another single-GPU
+configuration.  Inputs are generated on the device by the `set site range`
+and strict-seed initialisation kernels before the timed region, so the timed
+region contains only the sweeps.  Prints one JSON line.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+# name -> (app, lattice, shape, Q, kT, seed, sweeps, description)
+WORKLOADS = {
+    "cfg1": ("potts", "sq/8n", (128, 128, 1), 100, 0.5, 56789, 100,
+             "Potts Q=100 sq/8n 128x128 periodic, kT=0.5, seed 56789"),
+    "cfg2": ("ising", "sq/4n", (4096, 4096, 1), 2, 2.269, 12345, 1000,
+             "Ising sq/4n 4096x4096 periodic, kT=2.269, seed 12345"),
+    "cfg3": ("potts", "sc/26n", (256, 256, 256), 10000, 0.5, 24680, 200,
+             "Potts Q=10000 sc/26n 256^3 periodic, kT=0.5, mask yes, seed 24680"),
+}
+
+NCOLORS = {"sq/4n": 2, "sq/8n": 4, "sc/6n": 2, "sc/26n": 8}
+
+
+def bytes_per_update(lattice):
+    """Algorithmic HBM bytes per site update of the per-colour design: each
+    colour pass reads the whole int32 spin array (its sites' neighbourhoods
+    cover it) and reads+writes the int32 seed and writes the int32 spin of
+    its own N/K sites: 4K + 12 bytes per update."""
+    return 4 * NCOLORS[lattice] + 12
+
+
+def load_traffic(workload):
+    p = os.path.join(ROOT, "profiles", "traffic.json")
+    try:
+        with open(p) as f:
+            t = json.load(f)
+        return t.get(workload)
+    except (OSError, ValueError):
+        return None
+
+
+def cpu_baseline(app, lattice, shape, q, kt, seed, budget_s, threads):
+    """The oracle (C restatement, OpenMP within a colour) on this host's cores:
+    whole-lattice sweeps of the same workload until budget_s has elapsed."""
+    from oracle import oracle as O
+    d = O.make_desc(app, lattice, *shape, q, kt)
+    spin, seeds = O.script_init(d, seed, 1, q)
+    n = shape[0] * shape[1] * shape[2]
+    done, t0 = 0, time.perf_counter()
+    while True:
+        O.sweeps(d, spin, seeds, 1, nthreads=threads, energies=False)
+        done += 1
+        el = time.perf_counter() - t0
+        if el >= budget_s or done >= 1000:
+            break
+    return {"value": round(n * done / el / 1e9, 6), "unit": "G site-updates/s",
+            "cores": threads, "kind": "port",
+            "sample": "%d whole-lattice strict sweeps of the same workload from the same initial "
+                      "state, oracle/spk_oracle.c (OpenMP over the sites of one colour) in %.1f s"
+                      % (done, el)}
+
+
+def copy_calibration(mib):
+    """A plain elementwise kernel of known traffic (torch.neg on int32: reads
+    and writes mib MiB each), for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE."""
+    import torch
+    n = mib * 2**20 // 4
+    a = torch.arange(n, dtype=torch.int32, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(3):
+        torch.neg(a, out=b)
+    torch.cuda.synchronize()
+    return {"copy_calibration": {"kernel": "torch.neg int32", "launches": 3,
+                                 "bytes_read_per_launch": n * 4, "bytes_written_per_launch": n * 4}}
+
+
+def run(args):
+    import spparks_amd as S
+
+    app, lattice, shape, q, kt, seed, nsw, desc = WORKLOADS[args.workload]
+    n = shape[0] * shape[1] * shape[2]
+    steps = args.steps if args.steps is not None else nsw
+    K = NCOLORS[lattice]
+    with S.Lattice(app, lattice, *shape, nspins=q, temperature=kt) as L:
+        s0, rseed = S.strict_init_params(seed)
+        L.set_range(s0, 1, q)
+        L.init_seeds(rseed)
+        if args.block:
+            L.set_launch(args.block)
+        if args.warmup:
+            L.sweeps(args.warmup, per_sweep=False)
+        L.sync()
+        t0 = time.perf_counter()
+        nacc, _, _ = L.sweeps(steps, per_sweep=False)
+        L.sync()
+        wall = time.perf_counter() - t0
+        dev_ms, nlaunch = L.last_timing()
+        energy = L.energy()
+
+    updates = n * steps
+    value = updates / wall / 1e9
+    per_launch_s = dev_ms / 1e3 / nlaunch
+    B = bytes_per_update(lattice)
+    bytes_per_launch = B * n // K
+    achieved = bytes_per_launch / per_launch_s / 1e9
+    out = {
+        "metric": "G site-updates/s & % HBM peak; batched vs plain speedup; p99 solve latency",
+        "value": round(value, 4),
+        "unit": "G site-updates/s",
+        "n_gpus": 1,
+        "steps": steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(wall * 1e3 / steps, 5),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "int32",
+        "data": "synthetic: set site range 1 Q from the master seed, strict seeds by tagreset",
+        "config": {"workload": "%s: %s, %d strict sweeps per step count" % (args.workload, desc, nsw),
+                   "app": app, "lattice": lattice, "shape": list(shape), "nspins": q, "kT": kt,
+                   "seed": seed, "sites": n, "colors": K, "sweep": "color/strict",
+                   "naccept": int(nacc), "final_energy": energy},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": load_traffic(args.workload),
+                     "kernel": "k_sweep_color<%s>" % lattice,
+                     "bytes_per_launch": bytes_per_launch,
+                     "avg_launch_us": round(per_launch_s * 1e6, 3),
+                     "bytes_per_update": B},
+        "pct_hbm_peak": round(100 * value * 1e9 * B / (HBM_PEAK_GBS * 1e9), 2),
+    }
+    if not args.no_cpu_baseline:
+        threads = args.cpu_threads or min(os.cpu_count() or 1, 16)
+        out["cpu_baseline"] = cpu_baseline(app, lattice, shape, q, kt, seed, args.cpu_seconds, threads)
+    else:
+        out["cpu_baseline"] = None
+    return out
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=None, help="timed sweeps (default: the workload's count)")
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--workload", default="cfg2", choices=sorted(WORKLOADS))
+    ap.add_argument("--block", type=int, default=0, help="sweep-kernel workgroup size (0 = default)")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="run only a plain copy-like kernel of MIB MiB (rocprof counter scaling)")
+    args = ap.parse_args()
+    if args.copy_calibration:
+        print(json.dumps(copy_calibration(args.copy_calibration)), flush=True)
+        return
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path "
+              "(a single lattice would shard by slabs with a per-colour halo; replicas by "
+              "replica).  Run with --gpus 1.", file=sys.stderr)
+        sys.exit(2)
+    print(json.dumps(run(args)), flush=True)
+
+
+if __name__ == "__main__":
+    main()

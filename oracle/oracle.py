@@ -126,12 +126,14 @@ def script_init(desc, master_seed, lo, hi):
     return spin, seeds
 
 
-def sweeps(desc, spin, seeds, nsweeps, nthreads=0):
-    """Run nsweeps in place; returns (cumulative naccept[nsweeps], energy[nsweeps])."""
+def sweeps(desc, spin, seeds, nsweeps, nthreads=0, energies=True):
+    """Run nsweeps in place; returns (cumulative naccept[nsweeps], energy[nsweeps]).
+    energies=False skips the per-sweep energy pass (energy comes back zero)."""
     nacc = np.zeros(max(nsweeps, 1), np.int64)
     eng = np.zeros(max(nsweeps, 1), np.float64)
     rc = lib().orc_sweeps(C.byref(desc), spin, seeds, int(nsweeps),
-                          nacc.ctypes.data, eng.ctypes.data, int(nthreads))
+                          nacc.ctypes.data, eng.ctypes.data if energies else None,
+                          int(nthreads))
     if rc:
         raise ValueError("orc_sweeps failed: %d" % rc)
     return nacc[:nsweeps], eng[:nsweeps]

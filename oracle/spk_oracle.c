@@ -170,6 +170,17 @@ int orc_color(const orc_desc *d, int64_t gid)
   return 0;
 }
 
+/* the same colour from coordinates (no division) */
+static inline int orc_color_ijk(int lattice, int64_t i, int64_t j, int64_t k)
+{
+  switch (lattice) {
+    case ORC_SQ_4N: return (int)((i + j) & 1) + 1;
+    case ORC_SQ_8N: return (int)(2 * (j & 1) + (i & 1)) + 1;
+    case ORC_SC_6N: return (int)((i + j + k) & 1) + 1;
+    default: return (int)(4 * (k & 1) + 2 * (j & 1) + (i & 1)) + 1;
+  }
+}
+
 /* Neighbour stencil by the reference's distance criterion
    (src/create_sites.cpp:1303-1371): offsets in {-1,0,1}^dim whose length r
    satisfies cutlo-eps < r < cuthi+eps, cutlo = 1, cuthi = 1 (4n/6n),
@@ -211,9 +222,12 @@ void orc_neighbors(const orc_desc *d, int *neigh)
     int64_t i = g % nx, j = (g / nx) % ny, k = g / (nx * ny);
     for (int m = 0; m < ns; m++) {
       int64_t a = i + offs[3 * m], b = j + offs[3 * m + 1], c = k + offs[3 * m + 2];
-      if (a < 0) a += nx; if (a >= nx) a -= nx;
-      if (b < 0) b += ny; if (b >= ny) b -= ny;
-      if (c < 0) c += nz; if (c >= nz) c -= nz;
+      if (a < 0) a += nx;
+      if (a >= nx) a -= nx;
+      if (b < 0) b += ny;
+      if (b >= ny) b -= ny;
+      if (c < 0) c += nz;
+      if (c >= nz) c -= nz;
       neigh[g * ns + m] = (int)(a + nx * (b + ny * c));
     }
   }
@@ -274,37 +288,39 @@ static void orc_geom_init(const orc_desc *d, orc_geom *G)
   G->nx = d->nx; G->ny = d->ny; G->nz = d->nz;
 }
 
-static inline int64_t orc_nb(const orc_geom *G, int64_t g, int m)
+/* neighbour ids of site (i,j,k) with periodic wrap, in stencil order:
+   the wrapped coordinates i-1,i,i+1 (likewise j, k) are formed once and the
+   stencil entries index them */
+static inline void orc_site_nbrs(const orc_geom *G, int64_t i, int64_t j, int64_t k,
+                                 int64_t *nb)
 {
-  int64_t nx = G->nx, ny = G->ny, nz = G->nz;
-  int64_t i = g % nx, j = (g / nx) % ny, k = g / (nx * ny);
-  int64_t a = i + G->offs[3 * m], b = j + G->offs[3 * m + 1], c = k + G->offs[3 * m + 2];
-  if (a < 0) a += nx; else if (a >= nx) a -= nx;
-  if (b < 0) b += ny; else if (b >= ny) b -= ny;
-  if (c < 0) c += nz; else if (c >= nz) c -= nz;
-  return a + nx * (b + ny * c);
+  int64_t xs[3] = {i == 0 ? G->nx - 1 : i - 1, i, i == G->nx - 1 ? 0 : i + 1};
+  int64_t ys[3] = {j == 0 ? G->ny - 1 : j - 1, j, j == G->ny - 1 ? 0 : j + 1};
+  int64_t zs[3] = {k == 0 ? G->nz - 1 : k - 1, k, k == G->nz - 1 ? 0 : k + 1};
+  for (int m = 0; m < G->ns; m++)
+    nb[m] = xs[G->offs[3 * m] + 1] +
+            G->nx * (ys[G->offs[3 * m + 1] + 1] + G->ny * zs[G->offs[3 * m + 2] + 1]);
 }
 
 /* site_energy: number of unlike neighbours, duplicates counted with
    multiplicity (src/app_potts.cpp:97-104, src/app_ising.cpp:88-95) */
-static double orc_site_energy(const orc_geom *G, const int *spin, int64_t g)
+static inline double orc_site_energy_nb(int ns, const int64_t *nb, const int *spin, int me)
 {
-  int me = spin[g];
   int e = 0;
-  for (int m = 0; m < G->ns; m++)
-    if (spin[orc_nb(G, g, m)] != me) e++;
+  for (int m = 0; m < ns; m++)
+    if (spin[nb[m]] != me) e++;
   return (double)e;
 }
 
 /* AppPotts::site_event_rejection (src/app_potts.cpp:112-146) and
    AppIsing::site_event_rejection (src/app_ising.cpp:104-136); mask
    bookkeeping omitted (it never skips a site in color/strict mode).
-   Returns 1 when the spin changed. */
-static int orc_event(const orc_desc *d, const orc_geom *G, int *spin,
-                     int64_t g, int *seed, double t_inverse)
+   nb holds the site's neighbour ids.  Returns 1 when the spin changed. */
+static int orc_event_nb(const orc_desc *d, int ns, const int64_t *nb, int *spin,
+                        int64_t g, int *seed, double t_inverse)
 {
   int old = spin[g];
-  double einitial = orc_site_energy(G, spin, g);
+  double einitial = orc_site_energy_nb(ns, nb, spin, old);
   if (d->app == ORC_APP_POTTS) {
     int q = d->nspins;
     int iran = (int)(q * orc_pm_uniform(seed)) + 1;
@@ -313,7 +329,7 @@ static int orc_event(const orc_desc *d, const orc_geom *G, int *spin,
   } else {
     spin[g] = orc_pm_uniform(seed) < 0.5 ? 1 : 2;
   }
-  double efinal = orc_site_energy(G, spin, g);
+  double efinal = orc_site_energy_nb(ns, nb, spin, spin[g]);
   if (efinal <= einitial) {
     /* downhill or level: accept */
   } else if (d->temperature == 0.0) {
@@ -322,6 +338,14 @@ static int orc_event(const orc_desc *d, const orc_geom *G, int *spin,
     spin[g] = old;
   }
   return spin[g] != old;
+}
+
+static int orc_event(const orc_desc *d, const orc_geom *G, int *spin,
+                     int64_t g, int *seed, double t_inverse)
+{
+  int64_t nb[26];
+  orc_site_nbrs(G, g % G->nx, (g / G->nx) % G->ny, g / (G->nx * G->ny), nb);
+  return orc_event_nb(d, G->ns, nb, spin, g, seed, t_inverse);
 }
 
 int orc_site_event(const orc_desc *d, int *spin, int64_t g, int *seed)
@@ -342,8 +366,16 @@ double orc_energy(const orc_desc *d, const int *spin)
      reference's left-to-right double sum because every partial sum is an
      integer below 2^53 */
   int64_t tot = 0;
-#pragma omp parallel for reduction(+:tot) schedule(static)
-  for (int64_t g = 0; g < n; g++) tot += (int64_t)orc_site_energy(&G, spin, g);
+  (void)n;
+#pragma omp parallel for collapse(2) reduction(+:tot) schedule(static)
+  for (int64_t k = 0; k < G.nz; k++)
+    for (int64_t j = 0; j < G.ny; j++)
+      for (int64_t i = 0; i < G.nx; i++) {
+        int64_t nb[26];
+        int64_t g = i + G.nx * (j + G.ny * k);
+        orc_site_nbrs(&G, i, j, k, nb);
+        tot += (int64_t)orc_site_energy_nb(G.ns, nb, spin, spin[g]);
+      }
   e = (double)tot;
   return e;
 }
@@ -366,29 +398,29 @@ int orc_sweeps(const orc_desc *d, int *spin, int *seeds, int nsweeps,
 #else
   (void)nthreads;
 #endif
-  /* colour of each site, computed once */
-  unsigned char *col = (unsigned char *)malloc((size_t)n);
-  if (!col) return -2;
-  for (int64_t g = 0; g < n; g++) col[g] = (unsigned char)orc_color(d, g + 1);
-
   int64_t naccept = 0;
   for (int sw = 0; sw < nsweeps; sw++) {
     for (int c = 1; c <= K; c++) {
       int64_t acc = 0;
       /* sites of one colour share no neighbour, so their order is free */
-#pragma omp parallel for reduction(+:acc) schedule(static)
-      for (int64_t g = 0; g < n; g++) {
-        if (col[g] != c) continue;
-        int s = seeds[g];
-        acc += orc_event(d, &G, spin, g, &s, t_inverse);
-        seeds[g] = s;
-      }
+#pragma omp parallel for collapse(2) reduction(+:acc) schedule(static)
+      for (int64_t k = 0; k < G.nz; k++)
+        for (int64_t j = 0; j < G.ny; j++)
+          for (int64_t i = 0; i < G.nx; i++) {
+            int64_t g = i + G.nx * (j + G.ny * k);
+            if (orc_color_ijk(d->lattice, i, j, k) != c) continue;
+            int64_t nb[26];
+            orc_site_nbrs(&G, i, j, k, nb);
+            int s = seeds[g];
+            acc += orc_event_nb(d, G.ns, nb, spin, g, &s, t_inverse);
+            seeds[g] = s;
+          }
       naccept += acc;
     }
     if (naccept_out) naccept_out[sw] = naccept;
     if (energy_out) energy_out[sw] = orc_energy(d, spin);
   }
-  free(col);
+  (void)n;
   return 0;
 }
 
