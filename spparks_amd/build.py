@@ -21,7 +21,7 @@ HIP_SOURCES = ["spk_common.hip", "spk_plain.hip"]
 FRONT_SOURCES = []          # C++ front end, filled in as it lands
 
 HIPCC_FLAGS = [
-    "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC",
+    "--offload-arch=gfx950", "-O3", "-std=c++20", "-fPIC",
     # bit-exact doubles: no a*b+c contraction anywhere (tagreset's rseed*IM+id)
     "-ffp-contract=off",
     # no packed FP32 instructions
@@ -72,7 +72,7 @@ def build_emulator():
     if not os.path.exists(src):
         return None
     out = os.path.join(emul, "spk_emul")
-    run([hipcc(), "-x", "hip", "--offload-host-only", "-O1", "-g", "-std=c++17", "-ffp-contract=off",
+    run([hipcc(), "-x", "hip", "--offload-host-only", "-O1", "-g", "-std=c++20", "-ffp-contract=off",
          "-Xarch_host", "-fsanitize=address", "-I" + INCLUDE, "-I" + CSRC, "-o", out, src,
          "-L" + LIBDIR, "-lspk_hip", "-Wl,-rpath," + LIBDIR])
     return out

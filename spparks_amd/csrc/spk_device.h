@@ -1,8 +1,9 @@
 // spk_device.h -- device-side building blocks of the strict colour sweep:
 // Park-Miller arithmetic, the colour-blocked lattice geometry, neighbour
-// gathers for each lattice style and the Potts / Ising rejection rule.
+// access for each lattice style and the Potts / Ising rejection rule.
 //
-// Included by the .hip translation units of libspk_hip.so only.
+// Everything here that touches lattice memory is __host__ __device__ so the
+// same code can be run on the CPU (tests/emul) and checked there.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -39,11 +40,14 @@ __host__ __device__ __forceinline__ uint32_t pm_mulmod(uint32_t a, uint32_t b) {
 // ------------------------------------------------------------ geometry
 // Colour-blocked layout.  Colour c (0-based; the reference's colour c+1,
 // src/lattice.cpp:234-283) occupies [c*M, (c+1)*M).  Within a colour the
-// sites form a dense (hx, hy, hz) sub-grid, x fastest:
-//   sq/4n : hx=nx/2 hy=ny   hz=1    site (2h + ((j+c)&1), j)
-//   sc/6n : hx=nx/2 hy=ny   hz=nz   site (2h + ((j+k+c)&1), j, k)
-//   sq/8n : hx=nx/2 hy=ny/2 hz=1    c = 2(j&1)+(i&1),          site (2a+pi, 2b+pj)
-//   sc/26n: hx=nx/2 hy=ny/2 hz=nz/2 c = 4(k&1)+2(j&1)+(i&1),   site (2a+pi, 2b+pj, 2e+pk)
+// sites form a dense (hx, hy, hz) sub-grid, x fastest, local index
+// a + hx*(b + hy*e):
+//   sq/4n : hx=nx/2 hy=ny   hz=1    site (2a + ((b+c)&1), b)
+//   sc/6n : hx=nx/2 hy=ny   hz=nz   site (2a + ((b+e+c)&1), b, e)
+//   sq/8n : hx=nx/2 hy=ny/2 hz=1    c = 2(j&1)+(i&1),        site (2a+pi, 2b+pj)
+//   sc/26n: hx=nx/2 hy=ny/2 hz=nz/2 c = 4(k&1)+2(j&1)+(i&1), site (2a+pi, 2b+pj, 2e+pk)
+// Spins are stored as uint8 / uint16 / int32 (the narrowest type holding
+// 1..Q); seeds as int32.
 struct Geom {
   int lat;
   int nx, ny, nz;
@@ -84,16 +88,20 @@ __host__ __device__ __forceinline__ int64_t blocked_index(const Geom &g, int i, 
 __host__ __device__ __forceinline__ int wrap_dec(int a, int n) { return a == 0 ? n - 1 : a - 1; }
 __host__ __device__ __forceinline__ int wrap_inc(int a, int n) { return a == n - 1 ? 0 : a + 1; }
 
+template <int LAT> struct NeighCount { static constexpr int value = 26; };
+template <> struct NeighCount<SPK_HIP_SQ_4N> { static constexpr int value = 4; };
+template <> struct NeighCount<SPK_HIP_SC_6N> { static constexpr int value = 6; };
+template <> struct NeighCount<SPK_HIP_SQ_8N> { static constexpr int value = 8; };
+
 // Gather the neighbour spins of sub-grid site (a,b,e) of colour c into nb[].
-// Returns the neighbour count.  The neighbour multiset equals the
-// reference's periodic Moore / von Neumann shell
-// (src/create_sites.cpp:589-640, :1303-1371), duplicates included on edges
-// of length 2.
-template <int LAT>
-__host__ __device__ __forceinline__ void gather(const int32_t *__restrict__ spin, const Geom &g,
-                                       int c, int a, int b, int e, int32_t *nb) {
+// The neighbour multiset equals the reference's periodic Moore / von Neumann
+// shell (src/create_sites.cpp:589-640, :1303-1371), duplicates included on
+// edges of length 2.  Used by the energy reduction.
+template <int LAT, typename T>
+__host__ __device__ __forceinline__ void gather(const T *__restrict__ spin, const Geom &g,
+                                                int c, int a, int b, int e, int32_t *nb) {
   if constexpr (LAT == SPK_HIP_SQ_4N) {
-    const int32_t *o = spin + (int64_t)(1 - c) * g.M;
+    const T *o = spin + (int64_t)(1 - c) * g.M;
     int p = (b + c) & 1;
     int xo = p ? wrap_inc(a, g.hx) : wrap_dec(a, g.hx);
     int64_t row = (int64_t)g.hx * b;
@@ -102,7 +110,7 @@ __host__ __device__ __forceinline__ void gather(const int32_t *__restrict__ spin
     nb[2] = o[a + (int64_t)g.hx * wrap_dec(b, g.hy)];
     nb[3] = o[a + (int64_t)g.hx * wrap_inc(b, g.hy)];
   } else if constexpr (LAT == SPK_HIP_SC_6N) {
-    const int32_t *o = spin + (int64_t)(1 - c) * g.M;
+    const T *o = spin + (int64_t)(1 - c) * g.M;
     int p = (b + e + c) & 1;
     int xo = p ? wrap_inc(a, g.hx) : wrap_dec(a, g.hx);
     int64_t hxy = (int64_t)g.hx * g.hy;
@@ -147,11 +155,6 @@ __host__ __device__ __forceinline__ void gather(const int32_t *__restrict__ spin
   }
 }
 
-template <int LAT> struct NeighCount { static constexpr int value = 26; };
-template <> struct NeighCount<SPK_HIP_SQ_4N> { static constexpr int value = 4; };
-template <> struct NeighCount<SPK_HIP_SC_6N> { static constexpr int value = 6; };
-template <> struct NeighCount<SPK_HIP_SQ_8N> { static constexpr int value = 8; };
-
 // ------------------------------------------------------------ the rule
 // Metropolis parameters of one lattice.  thr[d] is the smallest seed s with
 // AM*s > exp(-d/kT) (the reference's `u2 > exp((ei-ef)*t_inverse)` rejection,
@@ -165,6 +168,30 @@ struct Rule {
   int thr[28];      // index 1..maxneigh
 };
 
+// Proposed spin from the first draw s (already advanced):
+// Potts `(int)(nspins*uniform())+1` clamped to Q (src/app_potts.cpp:119-120),
+// Ising `uniform() < 0.5 ? 1 : 2` (src/app_ising.cpp:112-113).
+__host__ __device__ __forceinline__ int32_t propose(const Rule &r, int32_t s) {
+  if (r.app == SPK_HIP_POTTS) {
+    double u = kAM * (double)s;
+    int32_t p = (int32_t)((double)r.nspins * u) + 1;
+    return p > r.nspins ? r.nspins : p;
+  }
+  return s < r.half ? 1 : 2;
+}
+
+// The accept/reject tail of site_event_rejection (src/app_potts.cpp:125-133):
+// ef <= ei accepts; T == 0 rejects an uphill move without a second draw;
+// otherwise a second draw s2 rejects when AM*s2 > exp(-(ef-ei)/kT), i.e.
+// when s2 >= thr[ef-ei].  Advances s in place; returns accept.
+__host__ __device__ __forceinline__ bool metropolis(const Rule &r, const int *thr, int ei, int ef,
+                                                    int32_t &s) {
+  if (ef <= ei) return true;
+  if (r.zero_t) return false;
+  s = pm_step(s);
+  return s < thr[ef - ei];
+}
+
 struct SiteOutcome {
   int32_t spin;     // new value (== old when rejected)
   int32_t seed;     // advanced generator state
@@ -172,37 +199,23 @@ struct SiteOutcome {
   int dE;           // ef - ei for a change, else 0
 };
 
-// One strict site event: AppPotts/AppIsing::site_event_rejection
-// (src/app_potts.cpp:112-146, src/app_ising.cpp:104-136) on gathered
-// neighbours; thr points at the Metropolis thresholds (LDS copy).
+// One strict site event on gathered neighbours
+// (AppPotts/AppIsing::site_event_rejection, src/app_potts.cpp:112-146,
+// src/app_ising.cpp:104-136).  The mask bookkeeping (:140-145) is omitted:
+// in color/strict mode the mask never skips a site.
 template <int NN>
 __host__ __device__ __forceinline__ SiteOutcome site_event(const Rule &r, const int *thr,
-                                                  int32_t old, int32_t seed,
-                                                  const int32_t *nb) {
-  int ei = 0;
-#pragma unroll
-  for (int m = 0; m < NN; m++) ei += (nb[m] != old);
+                                                           int32_t old, int32_t seed,
+                                                           const int32_t *nb) {
   int32_t s = pm_step(seed);
-  int32_t prop;
-  if (r.app == SPK_HIP_POTTS) {
-    double u = kAM * (double)s;
-    prop = (int32_t)((double)r.nspins * u) + 1;
-    if (prop > r.nspins) prop = r.nspins;
-  } else {
-    prop = s < r.half ? 1 : 2;
-  }
-  int ef = 0;
+  const int32_t prop = propose(r, s);
+  int ei = 0, ef = 0;
 #pragma unroll
-  for (int m = 0; m < NN; m++) ef += (nb[m] != prop);
-  bool accept = true;
-  if (ef > ei) {
-    if (r.zero_t) {
-      accept = false;
-    } else {
-      s = pm_step(s);
-      accept = s < thr[ef - ei];
-    }
+  for (int m = 0; m < NN; m++) {
+    ei += (nb[m] != old);
+    ef += (nb[m] != prop);
   }
+  const bool accept = metropolis(r, thr, ei, ef, s);
   SiteOutcome o;
   o.changed = accept && prop != old;
   o.spin = o.changed ? prop : old;
@@ -211,8 +224,142 @@ __host__ __device__ __forceinline__ SiteOutcome site_event(const Rule &r, const 
   return o;
 }
 
+// ------------------------------------------------ vectorised colour pass
+// V consecutive sites of one sub-grid row per thread.  Requires hx % V == 0
+// (so packs never straddle a row and stay V-aligned).
+template <typename T, int V>
+struct alignas(sizeof(T) * V) Pack {
+  T v[V];
+};
+
+template <typename T, int V>
+__host__ __device__ __forceinline__ void load_pack(const T *p, int32_t (&out)[V]) {
+  const Pack<T, V> pk = *reinterpret_cast<const Pack<T, V> *>(p);
+#pragma unroll
+  for (int v = 0; v < V; v++) out[v] = (int32_t)pk.v[v];
+}
+
+// Row segment of V+1 values of the other x-parity: positions
+// a0+off-1 .. a0+off+V-1 (off = 0 or 1) with periodic wrap.  Site v of the
+// thread has its two x-neighbours at seg[v] and seg[v+1].
+template <typename T, int V>
+__host__ __device__ __forceinline__ void load_segment(const T *row, int a0, int hx, int off,
+                                                      int32_t (&seg)[V + 1]) {
+  int32_t pk[V];
+  load_pack<T, V>(row + a0, pk);
+  const int xe = off ? (a0 + V == hx ? 0 : a0 + V) : (a0 == 0 ? hx - 1 : a0 - 1);
+  const int32_t extra = (int32_t)row[xe];
+#pragma unroll
+  for (int v = 0; v <= V; v++) {
+    const int32_t lo = v == 0 ? extra : pk[v - 1];     // off == 0
+    const int32_t hi = v == V ? extra : pk[v];         // off == 1
+    seg[v] = off ? hi : lo;
+  }
+}
+
+struct Counts {
+  int nacc;   // accepted flips
+  int de;     // sum of (ef - ei) over accepted flips
+};
+
+// Strict events for sites a0..a0+V-1 of sub-grid row (b, e) of colour c.
+// The first draw and the proposal come first, so the neighbour values are
+// streamed into two running counts (unlike the old / the proposed spin).
+template <int LAT, typename T, int V>
+__host__ __device__ __forceinline__ Counts sweep_sites(T *__restrict__ spin, int32_t *__restrict__ seed,
+                                                       const Geom &g, const Rule &r, const int *thr,
+                                                       int c, int a0, int b, int e) {
+  const int64_t self = (int64_t)c * g.M + a0 + (int64_t)g.hx * (b + (int64_t)g.hy * e);
+  int32_t old[V], s[V], prop[V], ei[V], ef[V];
+  load_pack<T, V>(spin + self, old);
+  {
+    const Pack<int32_t, V> sp = *reinterpret_cast<const Pack<int32_t, V> *>(seed + self);
+#pragma unroll
+    for (int v = 0; v < V; v++) {
+      s[v] = pm_step(sp.v[v]);
+      prop[v] = propose(r, s[v]);
+      ei[v] = 0;
+      ef[v] = 0;
+    }
+  }
+  auto count1 = [&](int v, int32_t x) {
+    ei[v] += (x != old[v]);
+    ef[v] += (x != prop[v]);
+  };
+  auto count_pack = [&](const T *row) {
+    int32_t x[V];
+    load_pack<T, V>(row + a0, x);
+#pragma unroll
+    for (int v = 0; v < V; v++) count1(v, x[v]);
+  };
+  auto count_segment = [&](const T *row, int off) {
+    int32_t sg[V + 1];
+    load_segment<T, V>(row, a0, g.hx, off, sg);
+#pragma unroll
+    for (int v = 0; v < V; v++) {
+      count1(v, sg[v]);
+      count1(v, sg[v + 1]);
+    }
+  };
+  const int64_t hx = g.hx, hxy = (int64_t)g.hx * g.hy;
+
+  if constexpr (LAT == SPK_HIP_SQ_4N) {
+    const T *o = spin + (int64_t)(1 - c) * g.M;
+    count_segment(o + hx * b, (b + c) & 1);
+    count_pack(o + hx * wrap_dec(b, g.hy));
+    count_pack(o + hx * wrap_inc(b, g.hy));
+  } else if constexpr (LAT == SPK_HIP_SC_6N) {
+    const T *o = spin + (int64_t)(1 - c) * g.M;
+    count_segment(o + hx * b + hxy * e, (b + e + c) & 1);
+    count_pack(o + hx * wrap_dec(b, g.hy) + hxy * e);
+    count_pack(o + hx * wrap_inc(b, g.hy) + hxy * e);
+    count_pack(o + hx * b + hxy * wrap_dec(e, g.hz));
+    count_pack(o + hx * b + hxy * wrap_inc(e, g.hz));
+  } else if constexpr (LAT == SPK_HIP_SQ_8N) {
+    const int pi = c & 1, pj = c >> 1;
+    const int ys[3] = {pj ? b : wrap_dec(b, g.hy), b, pj ? wrap_inc(b, g.hy) : b};
+#pragma unroll
+    for (int dy = 0; dy < 3; dy++) {
+      const int qy = pj ^ (dy != 1);
+      const int64_t row = hx * ys[dy];
+      if (dy != 1) count_pack(spin + (int64_t)(pi + 2 * qy) * g.M + row);
+      count_segment(spin + (int64_t)((1 - pi) + 2 * qy) * g.M + row, pi);
+    }
+  } else {
+    const int pi = c & 1, pj = (c >> 1) & 1, pk = c >> 2;
+    const int ys[3] = {pj ? b : wrap_dec(b, g.hy), b, pj ? wrap_inc(b, g.hy) : b};
+    const int zs[3] = {pk ? e : wrap_dec(e, g.hz), e, pk ? wrap_inc(e, g.hz) : e};
+#pragma unroll
+    for (int dz = 0; dz < 3; dz++)
+#pragma unroll
+      for (int dy = 0; dy < 3; dy++) {
+        const int qyz = 2 * (pj ^ (dy != 1)) + 4 * (pk ^ (dz != 1));
+        const int64_t row = hx * ys[dy] + hxy * zs[dz];
+        if (dy != 1 || dz != 1) count_pack(spin + (int64_t)(pi + qyz) * g.M + row);
+        count_segment(spin + (int64_t)((1 - pi) + qyz) * g.M + row, pi);
+      }
+  }
+
+  Counts out{0, 0};
+  Pack<T, V> nsp;
+  Pack<int32_t, V> nsd;
+  bool any = false;
+#pragma unroll
+  for (int v = 0; v < V; v++) {
+    const bool accept = metropolis(r, thr, ei[v], ef[v], s[v]);
+    const bool changed = accept && prop[v] != old[v];
+    nsp.v[v] = (T)(changed ? prop[v] : old[v]);
+    nsd.v[v] = s[v];
+    any |= changed;
+    out.nacc += changed;
+    out.de += changed ? ef[v] - ei[v] : 0;
+  }
+  *reinterpret_cast<Pack<int32_t, V> *>(seed + self) = nsd;
+  if (any) *reinterpret_cast<Pack<T, V> *>(spin + self) = nsp;
+  return out;
+}
+
 // --------------------------------------------------------- reductions
-// Wavefront (64-lane) sum, then one value per workgroup via LDS.
 __device__ __forceinline__ int wave_sum(int v) {
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
@@ -224,7 +371,8 @@ __device__ __forceinline__ long long wave_sum64(long long v) {
   return v;
 }
 
-// Adds (x, y) of every thread of the block into acc[2*slot], acc[2*slot+1].
+// Adds (x, y) of every thread of the block into acc[2*slot], acc[2*slot+1]:
+// a wavefront shuffle sum, one LDS slot per wave, one atomic per block.
 // All threads of the block must call it.  Needs blockDim.x % 64 == 0.
 __device__ __forceinline__ void block_accumulate(long long x, long long y,
                                                  unsigned long long *acc, int slot) {
@@ -244,5 +392,13 @@ __device__ __forceinline__ void block_accumulate(long long x, long long y,
 }
 
 constexpr int kAccSlots = 64;   // spread of per-sweep accumulator slots
+constexpr int kNumXcd = 8;      // MI355X: workgroups are dealt round-robin to 8 XCDs
+
+// Logical tile of hardware workgroup `bid` in a grid of `nb` workgroups
+// (nb a multiple of 8): consecutive logical tiles land on the same XCD, so
+// tiles that share halo rows share that XCD's L2.
+__host__ __device__ __forceinline__ int xcd_tile(int bid, int nb) {
+  return (bid % kNumXcd) * (nb / kNumXcd) + bid / kNumXcd;
+}
 
 }  // namespace spk

@@ -19,42 +19,33 @@ using namespace spk;
 // ==================================================================== kernels
 
 // One colour pass: every site of colour `color` performs one strict event.
-// Thread t -> sub-grid site (a,b,e) of that colour (x fastest), so the
-// wavefront's own spin/seed loads and stores are unit-stride.
-template <int LAT>
-__global__ __launch_bounds__(1024) void k_sweep_color(int32_t *__restrict__ spin,
-                                                      int32_t *__restrict__ seed,
+// Thread t owns V consecutive sites of one sub-grid row (x fastest), so its
+// own spins/seeds are one aligned pack and the wavefront's accesses are
+// unit-stride.  Workgroups are remapped so neighbouring tiles share an XCD.
+template <int LAT, typename T, int V>
+__global__ __launch_bounds__(1024) void k_sweep_color(T *__restrict__ spin, int32_t *__restrict__ seed,
                                                       Geom g, Rule r, int color,
                                                       unsigned long long *__restrict__ acc) {
-  constexpr int NN = NeighCount<LAT>::value;
   __shared__ int s_thr[32];
   if (threadIdx.x < 28) s_thr[threadIdx.x] = r.thr[threadIdx.x];
   __syncthreads();
 
-  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  int changed = 0, dE = 0;
-  if (t < g.M) {
+  const int tile = xcd_tile(blockIdx.x, gridDim.x);
+  const int64_t t = (int64_t)tile * blockDim.x + threadIdx.x;
+  const int per_row = g.hx / V;
+  Counts cnt{0, 0};
+  if (t < (int64_t)(g.M / V)) {
     const int ti = (int)t;
-    const int a = ti % g.hx;
-    const int rest = ti / g.hx;
-    const int b = rest % g.hy;
-    const int e = rest / g.hy;
-    int32_t nb[NN];
-    gather<LAT>(spin, g, color, a, b, e, nb);
-    const int64_t self = (int64_t)color * g.M + t;
-    const int32_t old = spin[self];
-    SiteOutcome o = site_event<NN>(r, s_thr, old, seed[self], nb);
-    seed[self] = o.seed;
-    if (o.changed) spin[self] = o.spin;
-    changed = o.changed;
-    dE = o.dE;
+    const int a0 = (ti % per_row) * V;
+    const int rest = ti / per_row;
+    cnt = sweep_sites<LAT, T, V>(spin, seed, g, r, s_thr, color, a0, rest % g.hy, rest / g.hy);
   }
-  block_accumulate(changed, dE, acc, blockIdx.x & (kAccSlots - 1));
+  block_accumulate(cnt.nacc, cnt.de, acc, blockIdx.x & (kAccSlots - 1));
 }
 
 // Total energy: sum over all sites of #unlike neighbours.
-template <int LAT>
-__global__ __launch_bounds__(256) void k_energy(const int32_t *__restrict__ spin, Geom g,
+template <int LAT, typename T>
+__global__ __launch_bounds__(256) void k_energy(const T *__restrict__ spin, Geom g,
                                                 unsigned long long *__restrict__ acc) {
   constexpr int NN = NeighCount<LAT>::value;
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -64,10 +55,8 @@ __global__ __launch_bounds__(256) void k_energy(const int32_t *__restrict__ spin
     const int ti = (int)(t - (int64_t)c * g.M);
     const int a = ti % g.hx;
     const int rest = ti / g.hx;
-    const int b = rest % g.hy;
-    const int e = rest / g.hy;
     int32_t nb[NN];
-    gather<LAT>(spin, g, c, a, b, e, nb);
+    gather<LAT, T>(spin, g, c, a, rest % g.hy, rest / g.hy, nb);
     const int32_t me = spin[t];
 #pragma unroll
     for (int m = 0; m < NN; m++) eng += (nb[m] != me);
@@ -75,27 +64,30 @@ __global__ __launch_bounds__(256) void k_energy(const int32_t *__restrict__ spin
   block_accumulate(eng, 0, acc, blockIdx.x & (kAccSlots - 1));
 }
 
-// natural order <-> colour-blocked order
-__global__ void k_to_blocked(const int32_t *__restrict__ nat, int32_t *__restrict__ blk, Geom g) {
-  const int64_t n = (int64_t)g.M * g.K;
-  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= n) return;
-  const int i = (int)(t % g.nx);
+__device__ __forceinline__ void natural_ijk(const Geom &g, int64_t t, int &i, int &j, int &k) {
+  i = (int)(t % g.nx);
   const int64_t r = t / g.nx;
-  const int j = (int)(r % g.ny);
-  const int k = (int)(r / g.ny);
-  blk[blocked_index(g, i, j, k)] = nat[t];
+  j = (int)(r % g.ny);
+  k = (int)(r / g.ny);
 }
 
-__global__ void k_to_natural(const int32_t *__restrict__ blk, int32_t *__restrict__ nat, Geom g) {
-  const int64_t n = (int64_t)g.M * g.K;
+// natural order <-> colour-blocked order (with the storage type change)
+template <typename T>
+__global__ void k_to_blocked(const int32_t *__restrict__ nat, T *__restrict__ blk, Geom g) {
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= n) return;
-  const int i = (int)(t % g.nx);
-  const int64_t r = t / g.nx;
-  const int j = (int)(r % g.ny);
-  const int k = (int)(r / g.ny);
-  nat[t] = blk[blocked_index(g, i, j, k)];
+  if (t >= (int64_t)g.M * g.K) return;
+  int i, j, k;
+  natural_ijk(g, t, i, j, k);
+  blk[blocked_index(g, i, j, k)] = (T)nat[t];
+}
+
+template <typename T>
+__global__ void k_to_natural(const T *__restrict__ blk, int32_t *__restrict__ nat, Geom g) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (int64_t)g.M * g.K) return;
+  int i, j, k;
+  natural_ijk(g, t, i, j, k);
+  nat[t] = (int32_t)blk[blocked_index(g, i, j, k)];
 }
 
 // counts values outside [lo, hi]
@@ -113,29 +105,27 @@ __global__ void k_count_outside(const int32_t *__restrict__ v, int64_t n, int lo
 // (src/set.cpp:598-627); 16807^gid mod m from the binary powers in pw.
 struct PowTable { uint32_t p[32]; };
 
-__global__ void k_set_range(int32_t *__restrict__ spin, Geom g, int seed0, int lo, int range,
-                            PowTable pw) {
-  const int64_t n = (int64_t)g.M * g.K;
+template <typename T>
+__global__ void k_set_range(T *__restrict__ spin, Geom g, int seed0, int lo, int range, PowTable pw) {
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= n) return;
+  if (t >= (int64_t)g.M * g.K) return;
   const uint64_t gid = (uint64_t)t + 1;
   uint32_t s = (uint32_t)seed0;
 #pragma unroll
   for (int bit = 0; bit < 32; bit++)
     if ((gid >> bit) & 1u) s = pm_mulmod(s, pw.p[bit]);
-  double u = kAM * (double)(int32_t)s;            // RandomPark::uniform
+  const double u = kAM * (double)(int32_t)s;      // RandomPark::uniform
   int v = (int)(u * (double)range) + 1;           // irandom(range)
   if (v > range) v = range;
-  const int i = (int)(t % g.nx);
-  const int64_t r = t / g.nx;
-  const int j = (int)(r % g.ny);
-  const int k = (int)(r / g.ny);
-  spin[blocked_index(g, i, j, k)] = v - 1 + lo;
+  int i, j, k;
+  natural_ijk(g, t, i, j, k);
+  spin[blocked_index(g, i, j, k)] = (T)(v - 1 + lo);
 }
 
-__global__ void k_fill(int32_t *__restrict__ v, int64_t n, int32_t value) {
+template <typename T>
+__global__ void k_fill(T *__restrict__ v, int64_t n, int32_t value) {
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t < n) v[t] = value;
+  if (t < n) v[t] = (T)value;
 }
 
 // siteseeds = tagreset(rseed, id, 100) (src/random_park.cpp:67-73).
@@ -145,23 +135,22 @@ __global__ void k_fill(int32_t *__restrict__ v, int64_t n, int32_t value) {
 // difference x-IM (Sterbenz).  The 100 warm-up steps are one multiply by
 // 16807^100 mod m.
 __global__ void k_init_seeds(int32_t *__restrict__ seed, Geom g, double xbase, uint32_t a100) {
-  const int64_t n = (int64_t)g.M * g.K;
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= n) return;
+  if (t >= (int64_t)g.M * g.K) return;
   const double im = (double)kIM;
   double x = xbase + (double)(t + 1);
   while (x >= im) x -= im;
   int32_t s = (int32_t)x;
   if (s == 0) s = 1;
   s = (int32_t)pm_mulmod((uint32_t)s, a100);
-  const int i = (int)(t % g.nx);
-  const int64_t r = t / g.nx;
-  const int j = (int)(r % g.ny);
-  const int k = (int)(r / g.ny);
+  int i, j, k;
+  natural_ijk(g, t, i, j, k);
   seed[blocked_index(g, i, j, k)] = s;
 }
 
 // ===================================================================== host
+
+enum SpinType { kU8 = 0, kU16 = 1, kI32 = 2 };
 
 struct spk_hip_lattice {
   spk_hip_desc d;
@@ -169,8 +158,12 @@ struct spk_hip_lattice {
   Rule rule;
   int64_t N = 0;
   int device = 0;
+  int stype = kI32;                    // storage type of spins on the device
+  int vec = 1;                         // sites per thread in the sweep kernel
   hipStream_t stream = nullptr;
-  int32_t *spin = nullptr, *seed = nullptr, *stage = nullptr;
+  void *spin = nullptr;                // N spins, colour-blocked, stype
+  int32_t *seed = nullptr;             // N seeds, colour-blocked
+  int32_t *stage = nullptr;            // N int32, natural order (transfers)
   unsigned long long *acc = nullptr;   // [cap][kAccSlots][2]
   int acc_cap = 0;
   int block = 256;
@@ -184,60 +177,85 @@ struct spk_hip_lattice {
 
 namespace {
 
+constexpr int kInitialAccRows = 1024;
+
+unsigned grid_for(int64_t n, int bs) { return (unsigned)((n + bs - 1) / bs); }
+
+int spin_type_for(int app, int nspins) {
+  const int qmax = app == SPK_HIP_POTTS ? nspins : 2;
+  return qmax <= 255 ? kU8 : qmax <= 65535 ? kU16 : kI32;
+}
+size_t spin_bytes(int stype) { return stype == kU8 ? 1 : stype == kU16 ? 2 : 4; }
+
+// Calls f.template operator()<T>() with the handle's spin storage type.
+template <typename F>
+void with_type(int stype, F &&f) {
+  switch (stype) {
+    case kU8: f.template operator()<uint8_t>(); break;
+    case kU16: f.template operator()<uint16_t>(); break;
+    default: f.template operator()<int32_t>(); break;
+  }
+}
+
+// Calls f.template operator()<LAT>() for the lattice style.
+template <typename F>
+void with_lattice(int lat, F &&f) {
+  switch (lat) {
+    case SPK_HIP_SQ_4N: f.template operator()<SPK_HIP_SQ_4N>(); break;
+    case SPK_HIP_SQ_8N: f.template operator()<SPK_HIP_SQ_8N>(); break;
+    case SPK_HIP_SC_6N: f.template operator()<SPK_HIP_SC_6N>(); break;
+    default: f.template operator()<SPK_HIP_SC_26N>(); break;
+  }
+}
+
 int launch_sweep(spk_hip_lattice *h, int color, unsigned long long *acc) {
   const int bs = h->block;
-  const unsigned grid = (unsigned)((h->g.M + bs - 1) / bs);
-  switch (h->g.lat) {
-    case SPK_HIP_SQ_4N:
-      hipLaunchKernelGGL(k_sweep_color<SPK_HIP_SQ_4N>, dim3(grid), dim3(bs), 0, h->stream,
-                         h->spin, h->seed, h->g, h->rule, color, acc);
-      break;
-    case SPK_HIP_SQ_8N:
-      hipLaunchKernelGGL(k_sweep_color<SPK_HIP_SQ_8N>, dim3(grid), dim3(bs), 0, h->stream,
-                         h->spin, h->seed, h->g, h->rule, color, acc);
-      break;
-    case SPK_HIP_SC_6N:
-      hipLaunchKernelGGL(k_sweep_color<SPK_HIP_SC_6N>, dim3(grid), dim3(bs), 0, h->stream,
-                         h->spin, h->seed, h->g, h->rule, color, acc);
-      break;
-    default:
-      hipLaunchKernelGGL(k_sweep_color<SPK_HIP_SC_26N>, dim3(grid), dim3(bs), 0, h->stream,
-                         h->spin, h->seed, h->g, h->rule, color, acc);
-      break;
-  }
+  const int64_t threads = h->g.M / h->vec;
+  unsigned grid = grid_for(threads, bs);
+  grid = (grid + kNumXcd - 1) / kNumXcd * kNumXcd;   // xcd_tile needs a multiple of 8
+  with_lattice(h->g.lat, [&]<int LAT>() {
+    with_type(h->stype, [&]<typename T>() {
+      T *sp = static_cast<T *>(h->spin);
+      if (h->vec == 4)
+        hipLaunchKernelGGL((k_sweep_color<LAT, T, 4>), dim3(grid), dim3(bs), 0, h->stream, sp, h->seed, h->g,
+                           h->rule, color, acc);
+      else
+        hipLaunchKernelGGL((k_sweep_color<LAT, T, 1>), dim3(grid), dim3(bs), 0, h->stream, sp, h->seed, h->g,
+                           h->rule, color, acc);
+    });
+  });
   return check_hip(hipGetLastError(), "sweep kernel launch");
 }
 
 int launch_energy(spk_hip_lattice *h, unsigned long long *acc) {
-  const int bs = 256;
-  const unsigned grid = (unsigned)((h->N + bs - 1) / bs);
-  switch (h->g.lat) {
-    case SPK_HIP_SQ_4N:
-      hipLaunchKernelGGL(k_energy<SPK_HIP_SQ_4N>, dim3(grid), dim3(bs), 0, h->stream, h->spin, h->g, acc);
-      break;
-    case SPK_HIP_SQ_8N:
-      hipLaunchKernelGGL(k_energy<SPK_HIP_SQ_8N>, dim3(grid), dim3(bs), 0, h->stream, h->spin, h->g, acc);
-      break;
-    case SPK_HIP_SC_6N:
-      hipLaunchKernelGGL(k_energy<SPK_HIP_SC_6N>, dim3(grid), dim3(bs), 0, h->stream, h->spin, h->g, acc);
-      break;
-    default:
-      hipLaunchKernelGGL(k_energy<SPK_HIP_SC_26N>, dim3(grid), dim3(bs), 0, h->stream, h->spin, h->g, acc);
-      break;
-  }
+  with_lattice(h->g.lat, [&]<int LAT>() {
+    with_type(h->stype, [&]<typename T>() {
+      hipLaunchKernelGGL((k_energy<LAT, T>), dim3(grid_for(h->N, 256)), dim3(256), 0, h->stream,
+                         static_cast<const T *>(h->spin), h->g, acc);
+    });
+  });
   return check_hip(hipGetLastError(), "energy kernel launch");
 }
 
-unsigned grid_for(int64_t n, int bs) { return (unsigned)((n + bs - 1) / bs); }
+int launch_fill_spins(spk_hip_lattice *h, int32_t value) {
+  with_type(h->stype, [&]<typename T>() {
+    hipLaunchKernelGGL((k_fill<T>), dim3(grid_for(h->N, 256)), dim3(256), 0, h->stream,
+                       static_cast<T *>(h->spin), h->N, value);
+  });
+  return check_hip(hipGetLastError(), "fill kernel launch");
+}
 
 int ensure_acc(spk_hip_lattice *h, int nsweeps) {
   if (nsweeps <= h->acc_cap) return SPK_HIP_OK;
-  int cap = h->acc_cap ? h->acc_cap : 16;
+  int cap = h->acc_cap ? h->acc_cap : kInitialAccRows;
   while (cap < nsweeps) cap *= 2;
   unsigned long long *p = nullptr;
   if (hipMalloc(&p, (size_t)cap * kAccSlots * 2 * sizeof(unsigned long long)) != hipSuccess)
     return set_error(SPK_HIP_ENOMEM, "device allocation of %d accumulator rows failed", cap);
-  if (h->acc) (void)hipFree(h->acc);
+  if (h->acc) {
+    (void)hipStreamSynchronize(h->stream);
+    (void)hipFree(h->acc);
+  }
   h->acc = p;
   h->acc_cap = cap;
   h->hacc.resize((size_t)cap * kAccSlots * 2);
@@ -248,7 +266,10 @@ int ensure_acc(spk_hip_lattice *h, int nsweeps) {
 void row_sum(const spk_hip_lattice *h, int row, int64_t *x, int64_t *y) {
   unsigned long long sx = 0, sy = 0;
   const unsigned long long *p = h->hacc.data() + (size_t)row * kAccSlots * 2;
-  for (int s = 0; s < kAccSlots; s++) { sx += p[2 * s]; sy += p[2 * s + 1]; }
+  for (int s = 0; s < kAccSlots; s++) {
+    sx += p[2 * s];
+    sy += p[2 * s + 1];
+  }
   *x = (int64_t)sx;
   *y = (int64_t)sy;
 }
@@ -311,6 +332,9 @@ int spk_hip_create(const spk_hip_desc *d, spk_hip_lattice **out) {
   h->g = g;
   h->rule = r;
   h->N = (int64_t)g.M * g.K;
+  h->stype = spin_type_for(d->app, d->nspins);
+  h->vec = g.hx % 4 == 0 ? 4 : 1;
+  h->block = g.lat == SPK_HIP_SQ_4N ? 512 : 256;   // measured best on MI355X (4096^2, 256^3)
   if ((rc = check_hip(hipGetDevice(&h->device), "hipGetDevice")) ||
       (rc = check_hip(hipStreamCreate(&h->stream), "hipStreamCreate")) ||
       (rc = check_hip(hipEventCreate(&h->ev0), "hipEventCreate")) ||
@@ -319,13 +343,16 @@ int spk_hip_create(const spk_hip_desc *d, spk_hip_lattice **out) {
     return rc;
   }
   const size_t bytes = (size_t)h->N * sizeof(int32_t);
-  if (hipMalloc(&h->spin, bytes) != hipSuccess || hipMalloc(&h->seed, bytes) != hipSuccess ||
-      hipMalloc(&h->stage, bytes) != hipSuccess) {
+  if (hipMalloc(&h->spin, (size_t)h->N * spin_bytes(h->stype)) != hipSuccess ||
+      hipMalloc(&h->seed, bytes) != hipSuccess || hipMalloc(&h->stage, bytes) != hipSuccess) {
     spk_hip_destroy(h);
     return set_error(SPK_HIP_ENOMEM, "device allocation of %lld sites failed", (long long)h->N);
   }
-  hipLaunchKernelGGL(k_fill, dim3(grid_for(h->N, 256)), dim3(256), 0, h->stream, h->spin, h->N, 1);
-  hipLaunchKernelGGL(k_fill, dim3(grid_for(h->N, 256)), dim3(256), 0, h->stream, h->seed, h->N, 1);
+  if ((rc = ensure_acc(h, kInitialAccRows)) || (rc = launch_fill_spins(h, 1))) {
+    spk_hip_destroy(h);
+    return rc;
+  }
+  hipLaunchKernelGGL(k_fill<int32_t>, dim3(grid_for(h->N, 256)), dim3(256), 0, h->stream, h->seed, h->N, 1);
   if ((rc = check_hip(hipStreamSynchronize(h->stream), "create sync"))) {
     spk_hip_destroy(h);
     return rc;
@@ -368,8 +395,10 @@ int spk_hip_upload(spk_hip_lattice *h, const int32_t *spin, const int32_t *seed)
     int64_t nbad = 0;
     if ((rc = validate_on_device(h, h->stage, 1, qmax, &nbad))) return rc;
     if (nbad) return set_error(SPK_HIP_EVALUE, "One or more sites have invalid values (%lld)", (long long)nbad);
-    hipLaunchKernelGGL(k_to_blocked, dim3(grid_for(h->N, 256)), dim3(256), 0, h->stream, h->stage, h->spin,
-                       h->g);
+    with_type(h->stype, [&]<typename T>() {
+      hipLaunchKernelGGL((k_to_blocked<T>), dim3(grid_for(h->N, 256)), dim3(256), 0, h->stream, h->stage,
+                         static_cast<T *>(h->spin), h->g);
+    });
     if ((rc = check_hip(hipGetLastError(), "layout kernel"))) return rc;
     h->energy_valid = false;
   }
@@ -379,8 +408,8 @@ int spk_hip_upload(spk_hip_lattice *h, const int32_t *spin, const int32_t *seed)
     int64_t nbad = 0;
     if ((rc = validate_on_device(h, h->stage, 1, kIM - 1, &nbad))) return rc;
     if (nbad) return set_error(SPK_HIP_EVALUE, "%lld seeds outside [1, 2^31-2]", (long long)nbad);
-    hipLaunchKernelGGL(k_to_blocked, dim3(grid_for(h->N, 256)), dim3(256), 0, h->stream, h->stage, h->seed,
-                       h->g);
+    hipLaunchKernelGGL(k_to_blocked<int32_t>, dim3(grid_for(h->N, 256)), dim3(256), 0, h->stream, h->stage,
+                       h->seed, h->g);
     if ((rc = check_hip(hipGetLastError(), "layout kernel"))) return rc;
   }
   return check_hip(hipStreamSynchronize(h->stream), "upload sync");
@@ -390,13 +419,21 @@ int spk_hip_download(spk_hip_lattice *h, int32_t *spin, int32_t *seed) {
   if (!h) return set_error(SPK_HIP_EINVAL, "null handle");
   const size_t bytes = (size_t)h->N * sizeof(int32_t);
   int rc;
-  for (int w = 0; w < 2; w++) {
-    int32_t *dst = w == 0 ? spin : seed;
-    if (!dst) continue;
-    hipLaunchKernelGGL(k_to_natural, dim3(grid_for(h->N, 256)), dim3(256), 0, h->stream,
-                       w == 0 ? h->spin : h->seed, h->stage, h->g);
+  if (spin) {
+    with_type(h->stype, [&]<typename T>() {
+      hipLaunchKernelGGL((k_to_natural<T>), dim3(grid_for(h->N, 256)), dim3(256), 0, h->stream,
+                         static_cast<const T *>(h->spin), h->stage, h->g);
+    });
     if ((rc = check_hip(hipGetLastError(), "layout kernel"))) return rc;
-    if ((rc = check_hip(hipMemcpyAsync(dst, h->stage, bytes, hipMemcpyDeviceToHost, h->stream), "download")))
+    if ((rc = check_hip(hipMemcpyAsync(spin, h->stage, bytes, hipMemcpyDeviceToHost, h->stream), "download")))
+      return rc;
+    if ((rc = check_hip(hipStreamSynchronize(h->stream), "download sync"))) return rc;
+  }
+  if (seed) {
+    hipLaunchKernelGGL(k_to_natural<int32_t>, dim3(grid_for(h->N, 256)), dim3(256), 0, h->stream, h->seed,
+                       h->stage, h->g);
+    if ((rc = check_hip(hipGetLastError(), "layout kernel"))) return rc;
+    if ((rc = check_hip(hipMemcpyAsync(seed, h->stage, bytes, hipMemcpyDeviceToHost, h->stream), "download")))
       return rc;
     if ((rc = check_hip(hipStreamSynchronize(h->stream), "download sync"))) return rc;
   }
@@ -407,14 +444,20 @@ int spk_hip_set_range(spk_hip_lattice *h, int seed0, int lo, int hi) {
   if (!h) return set_error(SPK_HIP_EINVAL, "null handle");
   if (hi < lo) return set_error(SPK_HIP_EINVAL, "set range: hi < lo");
   if (seed0 < 1 || seed0 >= kIM) return set_error(SPK_HIP_EINVAL, "set range: seed0 outside [1, 2^31-2]");
+  const int qmax = h->d.app == SPK_HIP_POTTS ? h->d.nspins : 2;
+  if (lo < 1 || hi > qmax)
+    return set_error(SPK_HIP_EVALUE, "One or more sites have invalid values (set range %d %d, Q = %d)", lo, hi,
+                     qmax);
   PowTable pw;
   uint32_t p = kIA;
   for (int b = 0; b < 32; b++) {
     pw.p[b] = p;
     p = pm_mulmod(p, p);
   }
-  hipLaunchKernelGGL(k_set_range, dim3(grid_for(h->N, 256)), dim3(256), 0, h->stream, h->spin, h->g, seed0, lo,
-                     hi - lo + 1, pw);
+  with_type(h->stype, [&]<typename T>() {
+    hipLaunchKernelGGL((k_set_range<T>), dim3(grid_for(h->N, 256)), dim3(256), 0, h->stream,
+                       static_cast<T *>(h->spin), h->g, seed0, lo, hi - lo + 1, pw);
+  });
   int rc = check_hip(hipGetLastError(), "set_range launch");
   if (rc) return rc;
   h->energy_valid = false;
@@ -423,8 +466,10 @@ int spk_hip_set_range(spk_hip_lattice *h, int seed0, int lo, int hi) {
 
 int spk_hip_set_value(spk_hip_lattice *h, int value) {
   if (!h) return set_error(SPK_HIP_EINVAL, "null handle");
-  hipLaunchKernelGGL(k_fill, dim3(grid_for(h->N, 256)), dim3(256), 0, h->stream, h->spin, h->N, value);
-  int rc = check_hip(hipGetLastError(), "set_value launch");
+  const int qmax = h->d.app == SPK_HIP_POTTS ? h->d.nspins : 2;
+  if (value < 1 || value > qmax)
+    return set_error(SPK_HIP_EVALUE, "One or more sites have invalid values (set value %d, Q = %d)", value, qmax);
+  int rc = launch_fill_spins(h, value);
   if (rc) return rc;
   h->energy_valid = false;
   return check_hip(hipStreamSynchronize(h->stream), "set_value sync");
@@ -433,7 +478,7 @@ int spk_hip_set_value(spk_hip_lattice *h, int value) {
 int spk_hip_init_seeds(spk_hip_lattice *h, double rseed) {
   if (!h) return set_error(SPK_HIP_EINVAL, "null handle");
   if (!(rseed >= 0.0 && rseed < 1.0)) return set_error(SPK_HIP_EINVAL, "init_seeds: rseed outside [0,1)");
-  double xbase = rseed * (double)kIM;
+  const double xbase = rseed * (double)kIM;
   uint32_t a100 = 1;
   for (int w = 0; w < 100; w++) a100 = pm_mulmod(a100, kIA);
   hipLaunchKernelGGL(k_init_seeds, dim3(grid_for(h->N, 256)), dim3(256), 0, h->stream, h->seed, h->g, xbase,
@@ -473,6 +518,7 @@ int spk_hip_sweeps(spk_hip_lattice *h, int nsweeps, int64_t *naccept_total, int6
 
   int64_t nacc = 0, eng = e0;
   const bool track = h->energy_valid || energy_per_sweep;
+  if (track && !energy_per_sweep) eng = h->energy;
   for (int s = 0; s < nsweeps; s++) {
     int64_t a, de;
     row_sum(h, s, &a, &de);

@@ -11,6 +11,10 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 EMUL = os.path.join(HERE, "emul", "spk_emul")
 
 CASES = [
+    ("potts", "sq/8n", (16, 8, 1), 300, 0.7, 1, 4),
+    ("potts", "sc/26n", (8, 8, 4), 70000, 0.4, 2, 3),
+    ("potts", "sq/4n", (16, 6, 1), 65535, 0.9, 3, 3),
+    ("ising", "sc/6n", (16, 4, 4), 2, 1.7, 4, 3),
     ("potts", "sq/8n", (16, 12, 1), 100, 0.5, 56789, 5),
     ("potts", "sq/8n", (2, 2, 1), 3, 0.8, 11, 4),
     ("potts", "sq/8n", (2, 6, 1), 3, 0.8, 12, 4),
@@ -30,7 +34,16 @@ CASES = [
 ]
 
 
-@pytest.mark.parametrize("case", CASES, ids=lambda c: "%s-%s-%s" % (c[0], c[1], "x".join(map(str, c[2]))))
+def storage_variants(app, q, nx):
+    """(stype, vec) pairs the library may pick for this case, plus wider
+    storage types, which must give the same trajectory."""
+    qmax = q if app == "potts" else 2
+    narrow = 0 if qmax <= 255 else 1 if qmax <= 65535 else 2
+    vecs = [1] + ([4] if (nx // 2) % 4 == 0 else [])
+    return [(t, v) for t in range(narrow, 3) for v in vecs]
+
+
+@pytest.mark.parametrize("case", CASES, ids=lambda c: "%s-%s-%s-q%d" % (c[0], c[1], "x".join(map(str, c[2])), c[3]))
 def test_emulated_kernel_matches_oracle(orc, tmp_path, case):
     if not os.path.exists(EMUL):
         pytest.skip("tests/emul/spk_emul not built (run __graft_entry__.build())")
@@ -43,16 +56,18 @@ def test_emulated_kernel_matches_oracle(orc, tmp_path, case):
         f.write(seeds.tobytes())
     nacc, eng = orc.sweeps(d, spin, seeds, n)
     env = dict(os.environ, ASAN_OPTIONS="detect_leaks=0")
-    r = subprocess.run([EMUL, str(orc.APP[app]), str(orc.LATTICE[lat]), *map(str, shape), str(q),
-                        repr(kt), str(n), str(fin), str(fout)], env=env, capture_output=True, text=True)
-    assert r.returncode == 0, r.stderr[-2000:]
-    N = spin.size
-    raw = open(fout, "rb").read()
-    es = np.frombuffer(raw, np.int32, N, 0)
-    ek = np.frombuffer(raw, np.int32, N, 4 * N)
-    en = np.frombuffer(raw, np.int64, n, 8 * N)
-    ee = np.frombuffer(raw, np.int64, n, 8 * N + 8 * n)
-    assert np.array_equal(es, spin)
-    assert np.array_equal(ek, seeds)
-    assert np.array_equal(en, nacc)
-    assert np.array_equal(ee.astype(np.float64), eng)
+    for stype, vec in storage_variants(app, q, shape[0]):
+        r = subprocess.run([EMUL, str(orc.APP[app]), str(orc.LATTICE[lat]), *map(str, shape), str(q),
+                            repr(kt), str(n), str(stype), str(vec), str(fin), str(fout)],
+                           env=env, capture_output=True, text=True)
+        assert r.returncode == 0, (stype, vec, r.stderr[-2000:])
+        N = spin.size
+        raw = open(fout, "rb").read()
+        es = np.frombuffer(raw, np.int32, N, 0)
+        ek = np.frombuffer(raw, np.int32, N, 4 * N)
+        en = np.frombuffer(raw, np.int64, n, 8 * N)
+        ee = np.frombuffer(raw, np.int64, n, 8 * N + 8 * n)
+        assert np.array_equal(es, spin), (stype, vec)
+        assert np.array_equal(ek, seeds), (stype, vec)
+        assert np.array_equal(en, nacc), (stype, vec)
+        assert np.array_equal(ee.astype(np.float64), eng), (stype, vec)

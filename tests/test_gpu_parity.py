@@ -51,6 +51,13 @@ CASES = [
     ("potts", "sq/8n", (34, 66, 1), 1, 0.5, 31, 3),
     ("ising", "sq/8n", (12, 12, 1), 2, 0.0, 37, 4),
     ("potts", "sq/4n", (130, 6, 1), 1000000, 0.15, 41, 3),
+    # storage / vector-width variants: u16 + 4 sites per thread, i32 + 4, u8 + 4
+    ("potts", "sq/8n", (16, 8, 1), 300, 0.7, 43, 4),
+    ("potts", "sq/4n", (16, 6, 1), 65535, 0.9, 47, 3),
+    ("potts", "sc/26n", (16, 8, 8), 70000, 0.4, 53, 3),
+    ("ising", "sc/6n", (16, 4, 4), 2, 1.7, 59, 3),
+    ("potts", "sc/26n", (32, 16, 8), 255, 0.6, 61, 3),
+    ("potts", "sc/26n", (32, 16, 8), 256, 0.6, 61, 3),
 ]
 
 
