@@ -18,7 +18,8 @@ LIBDIR = os.path.join(HERE, "lib")
 INCLUDE = os.path.join(ROOT, "include")
 
 HIP_SOURCES = ["spk_common.hip", "spk_plain.hip"]
-FRONT_SOURCES = []          # C++ front end, filled in as it lands
+FRONT_SOURCES = ["front/spk_front.cpp"]
+CXX_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-Wall", "-Wextra", "-ffp-contract=off"]
 
 HIPCC_FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++20", "-fPIC",
@@ -58,10 +59,29 @@ def build_hip():
     return out
 
 
+def cxx():
+    return os.environ.get("CXX") or shutil.which("g++") or "g++"
+
+
 def build_front():
-    if not FRONT_SOURCES:
-        return None
-    raise NotImplementedError
+    """The spk_hip program and libspparks_hip.so (host C++ over the C ABI)."""
+    inc = ["-I" + INCLUDE, "-I" + CSRC, "-I" + os.path.join(CSRC, "front")]
+    link = ["-L" + LIBDIR, "-lspk_hip", "-Wl,-rpath,$ORIGIN"]
+    objs = []
+    for src in FRONT_SOURCES:
+        obj = os.path.join(LIBDIR, os.path.basename(src).replace(".cpp", ".o"))
+        run([cxx(), *CXX_FLAGS, *inc, "-c", os.path.join(CSRC, src), "-o", obj])
+        objs.append(obj)
+    lib = os.path.join(LIBDIR, "libspparks_hip.so")
+    run([cxx(), *CXX_FLAGS, *inc, "-shared", "-o", lib + ".tmp", os.path.join(CSRC, "front", "spk_library.cpp"),
+         *objs, *link])
+    os.replace(lib + ".tmp", lib)
+    exe = os.path.join(LIBDIR, "spk_hip")
+    run([cxx(), *CXX_FLAGS, *inc, "-o", exe + ".tmp", os.path.join(CSRC, "front", "spk_main.cpp"), *objs, *link])
+    os.replace(exe + ".tmp", exe)
+    for o in objs:
+        os.remove(o)
+    return exe
 
 
 def build_emulator():

@@ -1,0 +1,799 @@
+// spk_front.cpp -- command execution of the spk_hip front end (see
+// spk_front.h).  Each command cites the reference routine whose behaviour it
+// reproduces; everything that touches lattice data goes through the C ABI of
+// libspk_hip.so.
+#include "spk_front.h"
+
+#include <chrono>
+#include <cmath>
+#include <cstdarg>
+#include <cstdlib>
+#include <cstring>
+
+namespace spkfront {
+
+namespace {
+
+constexpr double kEpsilon = 1.0e-6;   // src/create_sites.cpp EPSILON
+
+double wall() {
+  using namespace std::chrono;
+  return duration<double>(steady_clock::now().time_since_epoch()).count();
+}
+
+double to_double(const std::string &s, const char *what) {
+  char *end = nullptr;
+  double v = std::strtod(s.c_str(), &end);
+  if (end == s.c_str()) throw Error(std::string("Illegal ") + what + " command");
+  return v;
+}
+
+int to_int(const std::string &s, const char *what) {
+  char *end = nullptr;
+  long v = std::strtol(s.c_str(), &end, 10);
+  if (end == s.c_str()) throw Error(std::string("Illegal ") + what + " command");
+  return (int)v;
+}
+
+int lattice_id(const std::string &s) {
+  if (s == "sq/4n") return SPK_HIP_SQ_4N;
+  if (s == "sq/8n") return SPK_HIP_SQ_8N;
+  if (s == "sc/6n") return SPK_HIP_SC_6N;
+  if (s == "sc/26n") return SPK_HIP_SC_26N;
+  return -1;
+}
+
+int maxneigh(const std::string &s) {
+  return s == "sq/4n" ? 4 : s == "sq/8n" ? 8 : s == "sc/6n" ? 6 : 26;
+}
+
+// splits a command line into words; text in double quotes is one word
+// (Input::parse, src/input.cpp:289-355)
+std::vector<std::string> tokenize(const std::string &line) {
+  std::vector<std::string> out;
+  size_t i = 0, n = line.size();
+  while (i < n) {
+    while (i < n && std::strchr(" \t\n\r\f", line[i])) i++;
+    if (i >= n) break;
+    if (line[i] == '"') {
+      size_t j = line.find('"', i + 1);
+      if (j == std::string::npos) throw Error("Unbalanced quotes in input line");
+      out.push_back(line.substr(i + 1, j - i - 1));
+      i = j + 1;
+    } else {
+      size_t j = i;
+      while (j < n && !std::strchr(" \t\n\r\f", line[j])) j++;
+      out.push_back(line.substr(i, j - i));
+      i = j;
+    }
+  }
+  return out;
+}
+
+}  // namespace
+
+Spparks::Spparks(FILE *screen, FILE *logfile) : screen_(screen), logfile_(logfile) {
+  message("SPPARKS (spk_hip: strict colour sweep on MI355X)\n");
+}
+
+Spparks::~Spparks() {
+  for (auto &d : dumps_)
+    if (d.fp) std::fclose(d.fp);
+  if (h_) spk_hip_destroy(h_);
+}
+
+void Spparks::message(const char *fmt, ...) {
+  if (quiet_) return;
+  va_list ap;
+  if (screen_) {
+    va_start(ap, fmt);
+    std::vfprintf(screen_, fmt, ap);
+    va_end(ap);
+    std::fflush(screen_);
+  }
+  if (logfile_) {
+    va_start(ap, fmt);
+    std::vfprintf(logfile_, fmt, ap);
+    va_end(ap);
+    std::fflush(logfile_);
+  }
+}
+
+void Spparks::hip(int rc) {
+  if (rc != SPK_HIP_OK) throw Error(spk_hip_last_error());
+}
+
+// ------------------------------------------------------------ input
+
+void Spparks::file(const char *path) {
+  FILE *fp = std::fopen(path, "r");
+  if (!fp) throw Error(std::string("Cannot open input script ") + path);
+  try {
+    file(fp);
+  } catch (...) {
+    std::fclose(fp);
+    throw;
+  }
+  std::fclose(fp);
+}
+
+// Input::file (src/input.cpp:121-200): lines ending in '&' continue
+void Spparks::file(FILE *fp) {
+  std::string line;
+  char buf[4096];
+  while (std::fgets(buf, sizeof(buf), fp)) {
+    line += buf;
+    if (!line.empty() && line.back() != '\n' && !std::feof(fp)) continue;   // long line
+    size_t e = line.find_last_not_of(" \t\r\n");
+    if (e != std::string::npos && line[e] == '&') {
+      line.erase(e);
+      line += ' ';
+      continue;
+    }
+    one(line);
+    line.clear();
+  }
+  if (!line.empty()) one(line);
+}
+
+std::string Spparks::substitute(const std::string &in) const {
+  // $x and ${name} replaced from the variable table (Input::substitute)
+  std::string out;
+  for (size_t i = 0; i < in.size(); i++) {
+    if (in[i] != '$') {
+      out += in[i];
+      continue;
+    }
+    std::string name;
+    if (i + 1 < in.size() && in[i + 1] == '{') {
+      size_t j = in.find('}', i + 2);
+      if (j == std::string::npos) throw Error("Invalid variable name");
+      name = in.substr(i + 2, j - i - 2);
+      i = j;
+    } else if (i + 1 < in.size()) {
+      name = in.substr(i + 1, 1);
+      i += 1;
+    } else {
+      throw Error("Invalid variable name");
+    }
+    auto it = vars_.find(name);
+    if (it == vars_.end()) throw Error("Substitution for illegal variable " + name);
+    out += it->second;
+  }
+  return out;
+}
+
+std::string Spparks::one(const std::string &raw) {
+  // strip # comment outside quotes
+  std::string line;
+  bool inq = false;
+  for (char ch : raw) {
+    if (ch == '"') inq = !inq;
+    if (ch == '#' && !inq) break;
+    line += ch;
+  }
+  line = substitute(line);
+  std::vector<std::string> w = tokenize(line);
+  if (w.empty()) return "";
+  const std::string cmd = w[0];
+  std::vector<std::string> a(w.begin() + 1, w.end());
+  if (cmd == "seed") cmd_seed(a);
+  else if (cmd == "app_style") cmd_app_style(a);
+  else if (cmd == "dimension") cmd_dimension(a);
+  else if (cmd == "lattice") cmd_lattice(a);
+  else if (cmd == "region") cmd_region(a);
+  else if (cmd == "boundary") cmd_boundary(a);
+  else if (cmd == "create_box") cmd_create_box(a);
+  else if (cmd == "create_sites") cmd_create_sites(a);
+  else if (cmd == "set") cmd_set(a);
+  else if (cmd == "diag_style") cmd_diag_style(a);
+  else if (cmd == "stats") cmd_stats(a);
+  else if (cmd == "dump") cmd_dump(a);
+  else if (cmd == "undump") cmd_undump(a);
+  else if (cmd == "run") cmd_run(a);
+  else if (cmd == "variable") cmd_variable(a);
+  else if (cmd == "print") cmd_print(a);
+  else if (cmd == "log") cmd_log(a);
+  else if (cmd == "echo" || cmd == "processors") {
+    // accepted: echo is cosmetic; one GPU has no processor grid
+  } else if (cmd == "sweep" || cmd == "temperature") {
+    // app-specific commands fall through to the app (src/input.cpp:512-514)
+    need_app(cmd.c_str());
+    if (cmd == "sweep") cmd_sweep(a);
+    else cmd_temperature(a);
+  } else if (cmd == "sector" || cmd == "solve_style") {
+    throw Error("Command " + cmd + " is outside this build (strict colour sweep only)");
+  } else {
+    throw Error("Unknown command: " + line);
+  }
+  return cmd;
+}
+
+void Spparks::set_variable(const std::string &name, const std::string &value) { vars_[name] = value; }
+
+void Spparks::need_app(const char *cmd) const {
+  if (app_.empty()) throw Error(std::string("Command ") + cmd + " before app_style set");
+}
+
+// ------------------------------------------------------------ commands
+
+// Input::seed (src/input.cpp:942-951)
+void Spparks::cmd_seed(const std::vector<std::string> &a) {
+  if (a.size() != 1) throw Error("Illegal seed command");
+  int s = to_int(a[0], "seed");
+  if (s <= 0) throw Error("Illegal seed command");
+  ranmaster_ = std::make_unique<spk::MasterRng>(s);
+}
+
+// Input::app_style (src/input.cpp:727-747), AppPotts/AppIsing constructors
+// (src/app_potts.cpp:30-54, src/app_ising.cpp:30-47)
+void Spparks::cmd_app_style(const std::vector<std::string> &a) {
+  if (a.empty()) throw Error("Illegal app_style command");
+  if (a[0] == "potts") {
+    if (a.size() != 2) throw Error("Illegal app_style command");
+    nspins_ = to_int(a[1], "app_style");
+    if (nspins_ <= 0) throw Error("Illegal app_style command");
+    dt_sweep_ = 1.0 / nspins_;
+  } else if (a[0] == "ising") {
+    if (a.size() != 1) throw Error("Illegal app_style command");
+    nspins_ = 2;
+    dt_sweep_ = 1.0 / 2.0;
+  } else {
+    throw Error("Unrecognized app style " + a[0] + " (this build runs potts and ising)");
+  }
+  app_ = a[0];
+  if (h_) {
+    spk_hip_destroy(h_);
+    h_ = nullptr;
+  }
+  nsites_ = 0;
+  values_set_ = false;
+  sweepflag_ = 0;
+  mask_ = false;
+  temperature_ = 0.0;
+  ranapp_made_ = ranstrict_made_ = false;
+  time_ = 0.0;
+  naccept_ = nattempt_ = 0;
+  nsweeps_ = 0;
+  first_run_ = true;
+}
+
+void Spparks::cmd_dimension(const std::vector<std::string> &a) {
+  if (a.size() != 1) throw Error("Illegal dimension command");
+  if (box_exist_) throw Error("Dimension command after simulation box is defined");
+  int d = to_int(a[0], "dimension");
+  if (d != 2 && d != 3) throw Error("Illegal simulation dimension (this build runs 2d and 3d lattices)");
+  dimension_ = d;
+}
+
+// Lattice constructor (src/lattice.cpp:37-160)
+void Spparks::cmd_lattice(const std::vector<std::string> &a) {
+  if (a.empty()) throw Error("Illegal lattice command");
+  if (lattice_id(a[0]) < 0)
+    throw Error("Lattice style " + a[0] + " is outside this build (sq/4n, sq/8n, sc/6n, sc/26n)");
+  if (a.size() != 2) throw Error("Illegal lattice command");
+  const bool is3 = a[0] == "sc/6n" || a[0] == "sc/26n";
+  if ((is3 && dimension_ != 3) || (!is3 && dimension_ != 2))
+    throw Error("Lattice style does not match dimension");
+  lattice_ = a[0];
+  latconst_ = to_double(a[1], "lattice");
+  if (latconst_ <= 0.0) throw Error("Illegal lattice command");
+}
+
+// RegBlock, scaled by the lattice spacing (src/region.cpp:71-78)
+void Spparks::cmd_region(const std::vector<std::string> &a) {
+  if (a.size() < 2) throw Error("Illegal region command");
+  if (a[1] != "block") throw Error("Region style " + a[1] + " is outside this build (block)");
+  if (a.size() < 8) throw Error("Illegal region command");
+  if (lattice_.empty()) throw Error("Use of region with undefined lattice");
+  Region r;
+  for (int d = 0; d < 3; d++) {
+    r.lo[d] = to_double(a[2 + 2 * d], "region") * latconst_;
+    r.hi[d] = to_double(a[3 + 2 * d], "region") * latconst_;
+  }
+  for (size_t i = 8; i < a.size(); i += 2)
+    if (a[i] != "side" || i + 1 >= a.size() || a[i + 1] != "in") throw Error("Illegal region command");
+  regions_[a[0]] = r;
+}
+
+void Spparks::cmd_boundary(const std::vector<std::string> &a) {
+  if (a.size() != 3) throw Error("Illegal boundary command");
+  for (int d = 0; d < 3; d++) {
+    if (a[d] == "p") periodic_[d] = true;
+    else if (a[d] == "n") periodic_[d] = false;
+    else throw Error("Illegal boundary command");
+  }
+}
+
+// CreateBox (src/create_box.cpp)
+void Spparks::cmd_create_box(const std::vector<std::string> &a) {
+  if (a.size() != 1) throw Error("Illegal create_box command");
+  if (box_exist_) throw Error("Cannot create_box after simulation box is defined");
+  need_app("create_box");
+  auto it = regions_.find(a[0]);
+  if (it == regions_.end()) throw Error("Create_box region ID does not exist");
+  for (int d = 0; d < 3; d++) {
+    boxlo_[d] = it->second.lo[d];
+    boxhi_[d] = it->second.hi[d];
+  }
+  box_exist_ = true;
+  message("Created box = (%g %g %g) to (%g %g %g)\n", boxlo_[0], boxlo_[1], boxlo_[2], boxhi_[0], boxhi_[1],
+          boxhi_[2]);
+  message("  1 by 1 by 1 processor grid\n");
+}
+
+// CreateSites::structured_lattice for a fully periodic box
+// (src/create_sites.cpp:198-409)
+void Spparks::cmd_create_sites(const std::vector<std::string> &a) {
+  if (a.size() != 1 || a[0] != "box")
+    throw Error("create_sites style is outside this build (create_sites box)");
+  if (!box_exist_) throw Error("Cannot create sites before simulation box is defined");
+  if (lattice_.empty()) throw Error("Cannot create sites with undefined lattice");
+  if (h_) throw Error("Cannot create sites after sites already exist");
+  for (int d = 0; d < dimension_; d++)
+    if (!periodic_[d]) throw Error("Non-periodic boundaries are outside this build");
+  message("Creating sites ...\n");
+  const double xprd = boxhi_[0] - boxlo_[0], yprd = boxhi_[1] - boxlo_[1], zprd = boxhi_[2] - boxlo_[2];
+  nx_ = (int)(xprd / latconst_);
+  ny_ = (int)(yprd / latconst_);
+  nz_ = dimension_ == 3 ? (int)(zprd / latconst_) : 1;
+  if (std::fabs(nx_ * latconst_ - xprd) > kEpsilon * latconst_ ||
+      std::fabs(ny_ * latconst_ - yprd) > kEpsilon * latconst_ ||
+      (dimension_ == 3 && std::fabs(nz_ * latconst_ - zprd) > kEpsilon * latconst_))
+    throw Error("Periodic box is not a multiple of lattice spacing");
+  if (nx_ < 1 || ny_ < 1 || nz_ < 1) throw Error("Simulation box holds no lattice sites");
+  nsites_ = (int64_t)nx_ * ny_ * nz_;
+  if (nsites_ > 2147483647LL) throw Error("Site IDs may exceed max ID value");
+  // the device lattice needs even edges; odd ones fail at the first run,
+  // where the reference checks colourability (src/lattice.cpp:199-220)
+  if (nx_ % 2 || ny_ % 2 || (dimension_ == 3 && nz_ % 2)) {
+    message("  %lld sites\n", (long long)nsites_);
+    h_ = nullptr;
+    return;
+  }
+  spk_hip_desc d;
+  d.app = app_ == "potts" ? SPK_HIP_POTTS : SPK_HIP_ISING;
+  d.lattice = lattice_id(lattice_);
+  d.nx = nx_;
+  d.ny = ny_;
+  d.nz = nz_;
+  d.nspins = nspins_;
+  d.temperature = 0.0;
+  hip(spk_hip_create(&d, &h_));
+  message("  %lld sites\n", (long long)nsites_);
+  message("  %lld sites have %d neighbors\n", (long long)nsites_, maxneigh(lattice_));
+}
+
+// Set::command with site range / value, loop all (src/set.cpp:100-300,
+// :317-345, :582-627): one master draw per set command.
+void Spparks::cmd_set(const std::vector<std::string> &a) {
+  if (a.size() < 2) throw Error("Illegal set command");
+  if (nsites_ == 0) throw Error("Set command before sites exist");
+  if (a[0] != "site") throw Error("set " + a[0] + " is outside this build (set site ...)");
+  if (!ranmaster_) throw Error("Seed command has not been used");
+  message("Setting site values ...\n");
+  size_t iarg;
+  int lo = 0, hi = 0;
+  bool range;
+  if (a[1] == "range") {
+    if (a.size() < 4) throw Error("Illegal set command");
+    lo = to_int(a[2], "set");
+    hi = to_int(a[3], "set");
+    range = true;
+    iarg = 4;
+  } else if (a[1] == "value") {
+    if (a.size() < 3) throw Error("Illegal set command");
+    lo = hi = to_int(a[2], "set");
+    range = false;
+    iarg = 3;
+  } else {
+    throw Error("set site " + a[1] + " is outside this build (range, value)");
+  }
+  for (; iarg < a.size(); iarg += 2) {
+    if (a[iarg] == "loop" && iarg + 1 < a.size() && a[iarg + 1] == "all") continue;
+    throw Error("set option " + a[iarg] + " is outside this build");
+  }
+  const int seed0 = spk::park_seed_from_draw(ranmaster_->uniform());
+  if (!h_) throw Error("Color stencil is incommensurate with lattice size");
+  // the reference stores any int; a value outside 1..Q fails at the run's
+  // init_app check, which the device enforces here
+  if (range) hip(spk_hip_set_range(h_, seed0, lo, hi));
+  else hip(spk_hip_set_value(h_, lo));
+  values_set_ = true;
+  message("  %lld settings made for site\n", (long long)nsites_);
+}
+
+// AppLattice::set_sweep (src/app_lattice.cpp:867-889)
+void Spparks::cmd_sweep(const std::vector<std::string> &a) {
+  if (a.empty()) throw Error("Illegal sweep command");
+  if (a[0] == "color/strict" || a[0] == "color/strict/hip") sweepflag_ = 1;
+  else if (a[0] == "none") sweepflag_ = 0;
+  else if (a[0] == "random" || a[0] == "raster" || a[0] == "color")
+    throw Error("sweep " + a[0] + " is outside this build: it runs sweep color/strict on the GPU");
+  else throw Error("Illegal sweep command");
+  mask_ = false;
+  for (size_t i = 1; i < a.size(); i += 2) {
+    if (a[i] != "mask" || i + 1 >= a.size()) throw Error("Illegal sweep command");
+    if (a[i + 1] == "yes") mask_ = true;
+    else if (a[i + 1] == "no") mask_ = false;
+    else throw Error("Illegal sweep command");
+  }
+}
+
+// AppLattice::set_temperature (src/app_lattice.cpp:893-898)
+void Spparks::cmd_temperature(const std::vector<std::string> &a) {
+  if (a.size() != 1) throw Error("Illegal temperature command");
+  temperature_ = to_double(a[0], "temperature");
+}
+
+// diag_style energy [stats yes|no] (src/diag_energy.cpp, src/diag.cpp)
+void Spparks::cmd_diag_style(const std::vector<std::string> &a) {
+  if (a.empty()) throw Error("Illegal diag_style command");
+  if (a[0] != "energy") throw Error("diag_style " + a[0] + " is outside this build (energy)");
+  need_app("diag_style");
+  diag_energy_ = true;
+  diag_stats_ = true;
+  for (size_t i = 1; i < a.size(); i += 2) {
+    if (i + 1 >= a.size()) throw Error("Illegal diag_style command");
+    if (a[i] == "stats") diag_stats_ = a[i + 1] == "yes";
+    else throw Error("diag_style option " + a[i] + " is outside this build");
+  }
+}
+
+// Output::set_stats (src/output.cpp:206-246)
+void Spparks::cmd_stats(const std::vector<std::string> &a) {
+  if (a.empty()) throw Error("Illegal stats command");
+  stats_delta_ = to_double(a[0], "stats");
+  if (stats_delta_ < 0.0) throw Error("Illegal stats command");
+  if (stats_delta_ == 0.0 && a.size() > 1) throw Error("Illegal stats command");
+  stats_delay_ = 0.0;
+  stats_logfreq_ = 0;
+  for (size_t i = 1; i < a.size();) {
+    if (a[i] == "delay" && i + 1 < a.size()) {
+      stats_delay_ = to_double(a[i + 1], "stats");
+      i += 2;
+    } else if ((a[i] == "logfreq" || a[i] == "loglinfreq") && i + 2 < a.size()) {
+      stats_nrepeat_ = to_int(a[i + 1], "stats");
+      stats_scale_ = to_double(a[i + 2], "stats");
+      if (stats_nrepeat_ < 0 || (a[i] == "logfreq" && stats_scale_ <= 0)) throw Error("Illegal stats command");
+      stats_logfreq_ = stats_nrepeat_ == 0 ? 0 : (a[i] == "logfreq" ? 1 : 2);
+      i += 3;
+    } else if (a[i] == "tol" && i + 1 < a.size()) {
+      stats_tol_ = to_double(a[i + 1], "stats");
+      if (stats_tol_ < 0.0) throw Error("Illegal stats command");
+      i += 2;
+    } else {
+      throw Error("Illegal stats command");
+    }
+  }
+}
+
+// dump ID text delta file [fields] (src/dump.cpp:41-130, src/dump_text.cpp)
+void Spparks::cmd_dump(const std::vector<std::string> &a) {
+  if (a.size() < 4) throw Error("Illegal dump command");
+  if (a[1] != "text") throw Error("dump style " + a[1] + " is outside this build (text)");
+  if (!box_exist_) throw Error("Dump command before simulation box is defined");
+  for (auto &d : dumps_)
+    if (d.id == a[0]) throw Error("Reuse of dump ID");
+  Dump d;
+  d.id = a[0];
+  d.delta = to_double(a[2], "dump");
+  if (d.delta <= 0.0) throw Error("Illegal dump command");
+  d.file = a[3];
+  if (a.size() == 4) d.fields = {"id", "site", "x", "y", "z"};
+  for (size_t i = 4; i < a.size(); i++) {
+    if (a[i] != "id" && a[i] != "site" && a[i] != "x" && a[i] != "y" && a[i] != "z" && a[i] != "energy")
+      throw Error("Invalid attribute " + a[i] + " in dump text command");
+    d.fields.push_back(a[i]);
+  }
+  d.fp = std::fopen(d.file.c_str(), "w");
+  if (!d.fp) throw Error("Cannot open dump file");
+  dumps_.push_back(std::move(d));
+}
+
+void Spparks::cmd_undump(const std::vector<std::string> &a) {
+  if (a.size() != 1) throw Error("Illegal undump command");
+  for (size_t i = 0; i < dumps_.size(); i++)
+    if (dumps_[i].id == a[0]) {
+      if (dumps_[i].fp) std::fclose(dumps_[i].fp);
+      dumps_.erase(dumps_.begin() + i);
+      return;
+    }
+  throw Error("Could not find undump ID");
+}
+
+void Spparks::cmd_variable(const std::vector<std::string> &a) {
+  if (a.size() < 3) throw Error("Illegal variable command");
+  if (a[1] != "index" && a[1] != "string" && a[1] != "equal")
+    throw Error("variable style " + a[1] + " is outside this build (index, string, equal numbers)");
+  if (a[1] == "equal") {
+    char *end = nullptr;
+    std::strtod(a[2].c_str(), &end);
+    if (*end) throw Error("equal-style formulas are outside this build; give a number");
+  }
+  if (a[1] == "index" && vars_.count(a[0])) return;   // -var on the command line wins
+  vars_[a[0]] = a[2];
+}
+
+void Spparks::cmd_print(const std::vector<std::string> &a) {
+  std::string s;
+  for (size_t i = 0; i < a.size(); i++) s += (i ? " " : "") + a[i];
+  message("%s\n", s.c_str());
+}
+
+void Spparks::cmd_log(const std::vector<std::string> &a) {
+  if (a.size() != 1) throw Error("Illegal log command");
+  if (logfile_ && logfile_ != stdout && logfile_ != stderr) std::fclose(logfile_);
+  logfile_ = a[0] == "none" ? nullptr : std::fopen(a[0].c_str(), "w");
+  if (a[0] != "none" && !logfile_) throw Error("Cannot open logfile");
+}
+
+// ------------------------------------------------------------ run
+
+// AppLattice::init (src/app_lattice.cpp:133-328) for sweep color/strict
+void Spparks::init_run() {
+  if (sweepflag_ == 0) throw Error("App needs a KMC or rejection KMC solver");
+  if (lattice_.empty()) throw Error("Cannot color without a lattice definition of sites");
+  if (!h_) throw Error("Color stencil is incommensurate with lattice size");
+  if (!ranmaster_) throw Error("Seed command has not been used");
+  if (!ranapp_made_) {          // ranapp: RandomPark(uniform()) then reset(uniform(),me,100)
+    ranmaster_->uniform();
+    ranmaster_->uniform();
+    ranapp_made_ = true;
+  }
+  if (!ranstrict_made_) {       // ranstrict: RandomPark(uniform()); seed = uniform()
+    ranmaster_->uniform();
+    const double rseed = ranmaster_->uniform();
+    hip(spk_hip_init_seeds(h_, rseed));
+    ranstrict_made_ = true;
+  }
+  // AppPotts/AppIsing::init_app site-value check
+  if (!values_set_) throw Error("One or more sites have invalid values");
+  hip(spk_hip_set_temperature(h_, temperature_));
+  if (dt_sweep_ == 0.0) throw Error("App did not set dt_sweep");
+}
+
+double Spparks::next_time(double tcurrent, int logfreq, double delta, int nrepeat, double scale,
+                          double delay) const {
+  // Output::next_time (src/output.cpp:414-437)
+  double tnew = 0.0;
+  if (logfreq == 0) {
+    tnew = std::ceil(tcurrent / delta) * delta;
+    if (tnew == tcurrent) tnew = tcurrent + delta;
+  } else if (logfreq == 1) {
+    while (tcurrent >= delta * scale) delta *= scale;
+    double ktmp = std::pow(scale, 1.0 / nrepeat);
+    tnew = delta;
+    while (tcurrent >= tnew) tnew *= ktmp;
+  } else {
+    double start = delta;
+    while (tcurrent >= start * scale) start *= scale;
+    tnew = std::ceil(tcurrent / start) * start;
+    if (tnew == tcurrent) tnew = tcurrent + start;
+    if (static_cast<int>(tnew / start) > nrepeat) tnew = start * scale;
+  }
+  return tnew > delay ? tnew : delay;
+}
+
+void Spparks::stats_header() {
+  char line[256];
+  int n = std::snprintf(line, sizeof line, "%10s %10s %10s %10s %10s", "Time", "Naccept", "Nreject", "Nsweeps",
+                        "CPU");
+  if (diag_energy_ && diag_stats_) std::snprintf(line + n, sizeof line - n, " %10s", "Energy");
+  message("%s\n", line);
+}
+
+// Output::stats (src/output.cpp:346-379) with AppLattice::stats
+// (src/app_lattice.cpp:917-934) and DiagEnergy::stats
+void Spparks::stats_line(bool timeflag) {
+  char line[256];
+  int n = std::snprintf(line, sizeof line, "%10g %10lld %10lld %10d %10.3g", time_, (long long)naccept_,
+                        (long long)(nattempt_ - naccept_), nsweeps_, timeflag ? wall() - loop_t0_ : 0.0);
+  if (diag_energy_ && diag_stats_) std::snprintf(line + n, sizeof line - n, " %10g", energy_);
+  message("%s\n", line);
+}
+
+void Spparks::download() {
+  host_spin_.resize(nsites_);
+  host_seed_.resize(nsites_);
+  hip(spk_hip_download(h_, host_spin_.data(), host_seed_.data()));
+}
+
+// DumpText::write (src/dump_text.cpp:384-392 header; one line per site)
+void Spparks::write_dump(Dump &d) {
+  download();
+  std::vector<double> eng;
+  std::fprintf(d.fp, "ITEM: TIMESTEP\n%d %10g\n", d.idump, time_);
+  std::fprintf(d.fp, "ITEM: NUMBER OF ATOMS\n%lld\n", (long long)nsites_);
+  std::fprintf(d.fp, "ITEM: BOX BOUNDS\n%g %g\n%g %g\n%g %g\n", boxlo_[0], boxhi_[0], boxlo_[1], boxhi_[1],
+               boxlo_[2], boxhi_[2]);
+  std::string cols;
+  for (size_t i = 0; i < d.fields.size(); i++) cols += (i ? " " : "") + (d.fields[i] == "site" ? std::string("type") : d.fields[i]);
+  std::fprintf(d.fp, "ITEM: ATOMS %s\n", cols.c_str());
+  for (int64_t g = 0; g < nsites_; g++) {
+    const int64_t i = g % nx_, j = (g / nx_) % ny_, k = g / ((int64_t)nx_ * ny_);
+    for (const auto &f : d.fields) {
+      if (f == "id") std::fprintf(d.fp, "%lld ", (long long)(g + 1));
+      else if (f == "site") std::fprintf(d.fp, "%d ", host_spin_[g]);
+      else if (f == "x") std::fprintf(d.fp, "%g ", i * latconst_ + boxlo_[0]);
+      else if (f == "y") std::fprintf(d.fp, "%g ", j * latconst_ + boxlo_[1]);
+      else if (f == "z") std::fprintf(d.fp, "%g ", dimension_ == 3 ? k * latconst_ + boxlo_[2] : 0.5 * (boxlo_[2] + boxhi_[2]));
+      else if (f == "energy") {
+        // site_energy: #unlike neighbours of the periodic shell
+        int e = 0;
+        for (int dk = -1; dk <= 1; dk++) {
+          if (dimension_ == 2 && dk) continue;
+          for (int dj = -1; dj <= 1; dj++)
+            for (int di = -1; di <= 1; di++) {
+              const int nb = std::abs(di) + std::abs(dj) + std::abs(dk);
+              if (nb == 0) continue;
+              if ((lattice_ == "sq/4n" || lattice_ == "sc/6n") && nb != 1) continue;
+              const int64_t a = (i + di + nx_) % nx_, b = (j + dj + ny_) % ny_, c = (k + dk + nz_) % nz_;
+              e += host_spin_[a + (int64_t)nx_ * (b + (int64_t)ny_ * c)] != host_spin_[g];
+            }
+        }
+        std::fprintf(d.fp, "%g ", (double)e);
+      }
+    }
+    std::fprintf(d.fp, "\n");
+  }
+  std::fflush(d.fp);
+  d.idump++;
+}
+
+// Output::setup (src/output.cpp:82-137)
+double Spparks::output_setup() {
+  double dump_time = stoptime_;
+  for (auto &d : dumps_) {
+    if (d.idump == 0) write_dump(d);
+    d.next_time = next_time(time_, 0, d.delta, 0, 0.0, 0.0);
+    dump_time = std::min(dump_time, d.next_time);
+  }
+  if (diag_energy_ && diag_stats_) hip(spk_hip_energy(h_, &energy_));
+  if (first_run_) {
+    message("Running with 32-bit site IDs\n");
+    message("Memory usage per processor = %g Mbytes\n", nsites_ * 8.0 / 1048576.0);
+  }
+  stats_header();
+  stats_line(false);
+  stats_time_ = stoptime_;
+  if (stats_delta_ > 0.0) {
+    stats_time_ =
+        next_time(time_ + stats_tol_, stats_logfreq_, stats_delta_, stats_nrepeat_, stats_scale_, stats_delay_);
+    stats_time_ -= stats_tol_;
+  }
+  return std::min(std::min(stoptime_, dump_time), stats_time_);
+}
+
+// Output::compute (src/output.cpp:145-202)
+double Spparks::output_compute(bool done) {
+  const bool sflag = time_ >= stats_time_ || done;
+  if (sflag && diag_energy_ && diag_stats_) hip(spk_hip_energy(h_, &energy_));
+  if (sflag) {
+    stats_line(true);
+    stats_time_ = stoptime_;
+    if (stats_delta_ != 0.0) {
+      stats_time_ =
+          next_time(time_ + stats_tol_, stats_logfreq_, stats_delta_, stats_nrepeat_, stats_scale_, stats_delay_);
+      stats_time_ -= stats_tol_;
+    }
+  }
+  double dump_time = stoptime_;
+  for (auto &d : dumps_) {
+    if (time_ >= d.next_time) {
+      write_dump(d);
+      d.next_time = next_time(time_, 0, d.delta, 0, 0.0, 0.0);
+    }
+    dump_time = std::min(dump_time, d.next_time);
+  }
+  return std::min(std::min(stoptime_, dump_time), stats_time_);
+}
+
+// App::run (src/app.cpp:86-148) -> AppLattice::setup / iterate_rejection
+// (src/app_lattice.cpp:332-440, :632-711) -> Finish
+void Spparks::cmd_run(const std::vector<std::string> &a) {
+  if (app_.empty()) throw Error("Run command before app_style set");
+  if (!box_exist_) throw Error("Cannot run application until simulation box is defined");
+  if (a.empty()) throw Error("Illegal run command");
+  const double delta = to_double(a[0], "run");
+  stoptime_ = time_ + delta;
+  if (stoptime_ < time_) throw Error("Illegal run command");
+  bool upto = false, pre = true, post = true;
+  for (size_t i = 1; i < a.size();) {
+    if (a[i] == "upto") {
+      upto = true;
+      i += 1;
+    } else if ((a[i] == "pre" || a[i] == "post") && i + 1 < a.size()) {
+      bool v;
+      if (a[i + 1] == "yes") v = true;
+      else if (a[i + 1] == "no") v = false;
+      else throw Error("Illegal run command");
+      (a[i] == "pre" ? pre : post) = v;
+      i += 2;
+    } else {
+      throw Error("Illegal run command");
+    }
+  }
+  if (upto) stoptime_ -= time_;
+  if (upto && stoptime_ < time_) throw Error("Run upto value is before current time");
+
+  if (pre || first_run_) init_run();
+  message("Setting up run ...\n");
+  t_solve_ = t_output_ = t_loop_ = 0.0;
+
+  // setup: one colour sweep per dt_rkmc = dt_sweep, cut at the stop time
+  double dt_rkmc = dt_sweep_;
+  if (stoptime_ - time_ < dt_rkmc) dt_rkmc = stoptime_ - time_;
+  double nextoutput = output_setup();
+  first_run_ = false;
+
+  loop_t0_ = wall();
+  if (stoptime_ > time_) {
+    bool done = false;
+    while (!done) {
+      // the sweeps up to the next output or the stop time run as one
+      // device call; time advances exactly as one sweep at a time would
+      int n = 0;
+      double t = time_;
+      for (;;) {
+        n++;
+        t += dt_rkmc;
+        if (t >= stoptime_) {
+          done = true;
+          break;
+        }
+        if (t >= nextoutput) break;
+        if (n == (1 << 30)) break;
+      }
+      const double t0 = wall();
+      int64_t nacc = 0;
+      hip(spk_hip_sweeps(h_, n, &nacc, nullptr, nullptr));
+      t_solve_ += wall() - t0;
+      naccept_ += nacc;
+      nattempt_ += (int64_t)n * nsites_;
+      nsweeps_ += n;
+      time_ = t;
+      if (done || time_ >= nextoutput) {
+        const double t1 = wall();
+        nextoutput = output_compute(done);
+        t_output_ += wall() - t1;
+      }
+    }
+  }
+  t_loop_ = wall() - loop_t0_;
+  // Finish (src/finish.cpp:30-120)
+  message("Loop time of %g on %d procs\n", t_loop_, 1);
+  if (!post) return;
+  const double tl = t_loop_ == 0.0 ? 1.0 : t_loop_;
+  message("\n");
+  message("Solve time (%%) = %g (%g)\n", t_solve_, t_solve_ / tl * 100.0);
+  message("Update time (%%) = %g (%g)\n", 0.0, 0.0);
+  message("Comm  time (%%) = %g (%g)\n", 0.0, 0.0);
+  message("Outpt time (%%) = %g (%g)\n", t_output_, t_output_ / tl * 100.0);
+  message("App   time (%%) = %g (%g)\n", 0.0, 0.0);
+  const double other = t_loop_ - t_solve_ - t_output_;
+  message("Other time (%%) = %g (%g)\n", other, other / tl * 100.0);
+}
+
+// ------------------------------------------------------------ library
+
+double Spparks::energy() {
+  if (!h_) throw Error("No sites on the device");
+  double e = 0.0;
+  hip(spk_hip_energy(h_, &e));
+  return e;
+}
+
+const int32_t *Spparks::site_values() {
+  if (!h_) return nullptr;
+  download();
+  return host_spin_.data();
+}
+
+const int32_t *Spparks::site_seeds() {
+  if (!h_) return nullptr;
+  download();
+  return host_seed_.data();
+}
+
+}  // namespace spkfront

@@ -5,6 +5,7 @@
 #include <hip/hip_runtime.h>
 
 #include "spk_device.h"
+#include "spk_master_rng.h"
 
 namespace spk {
 
@@ -16,17 +17,5 @@ int check_hip(hipError_t e, const char *what);
 int make_geom(const spk_hip_desc *d, Geom *g);
 // Metropolis thresholds and proposal parameters for one temperature
 int make_rule(int app, int nspins, int lattice, double temperature, Rule *r);
-
-// the `seed` command's generator (src/random_mars.cpp:43-105)
-class MasterRng {
- public:
-  explicit MasterRng(int seed);
-  double uniform();
-
- private:
-  double u_[98];
-  double c_, cd_, cm_;
-  int i97_, j97_;
-};
 
 }  // namespace spk

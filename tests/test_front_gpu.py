@@ -1,0 +1,163 @@
+"""The drop-in boundary end to end on the GPU: unchanged SPPARKS input scripts
+through the spk_hip program and through libspparks_hip.so (the reference's
+library API), checked against the reference run's known answers and the
+oracle."""
+import json
+import os
+import subprocess
+import sys
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+EXE = os.path.join(ROOT, "spparks_amd", "lib", "spk_hip")
+SCRIPTS = os.path.join(ROOT, "tests", "scripts")
+GOLD = os.path.join(ROOT, "tests", "golden")
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def run_exe(tmp_path, script, *args):
+    say("spk_hip", script, args)
+    r = subprocess.run([EXE, "-in", script, "-log", "none", *args], capture_output=True, text=True,
+                       cwd=tmp_path, timeout=110)
+    assert r.returncode == 0, r.stdout[-3000:] + r.stderr[-2000:]
+    return r.stdout
+
+
+def stats_rows(out):
+    """(time, naccept, nreject, nsweeps, energy) rows of the stats output."""
+    rows, on = [], False
+    for line in out.splitlines():
+        w = line.split()
+        if w[:4] == ["Time", "Naccept", "Nreject", "Nsweeps"]:
+            on = True
+            continue
+        if on and len(w) >= 6:
+            try:
+                rows.append((float(w[0]), int(w[1]), int(w[2]), int(w[3]), float(w[5])))
+            except ValueError:
+                on = False
+        elif on:
+            on = False
+    return rows
+
+
+def test_cfg1_script_known_answers(tmp_path):
+    """in.cfg1 unchanged: the stats lines carry the reference run's numbers."""
+    g = json.load(open(os.path.join(GOLD, "cfg1_known_answers.json")))
+    out = run_exe(tmp_path, os.path.join(SCRIPTS, "in.cfg1"), "-sf", "hip")
+    rows = stats_rows(out)
+    assert rows[0] == (0.0, 0, 0, 0, float(g["initial_energy"]))
+    assert rows[-1][3] == 100
+    by_sweep = {r[3]: r for r in rows}
+    for sweep, na, e in g["rows"]:
+        t, nacc, nrej, nsw, eng = by_sweep[sweep]
+        assert nacc == na and eng == e
+        assert nacc + nrej == sweep * 128 * 128
+    assert "Loop time of" in out
+
+
+def test_library_api_matches_oracle(orc, tmp_path, monkeypatch):
+    from spparks_amd.spparks import spparks
+    monkeypatch.chdir(tmp_path)
+    spk = spparks(cmdargs=["-screen", "none", "-log", "none"])
+    lines = open(os.path.join(SCRIPTS, "in.cfg1")).read().replace("run             1.0", "run 0.1")
+    for line in lines.splitlines():
+        spk.command(line)
+    # time accumulates 0.01 per sweep in floating point: `run 0.1` is 11 sweeps
+    n1, t1 = orc.run_nsweeps(0.0, 0.1, 0.01)
+    n2, t2 = orc.run_nsweeps(t1, 0.05, 0.01)
+    d = orc.make_desc("potts", "sq/8n", 128, 128, 1, 100, 0.5)
+    spin, seeds = orc.script_init(d, 56789, 1, 100)
+    orc.sweeps(d, spin, seeds, n1)
+    n = spk.extract("nglobal", 0)
+    assert n == 128 * 128
+    site = np.ctypeslib.as_array(spk.extract("site", 1), shape=(n,)).copy()
+    seed = np.ctypeslib.as_array(spk.extract("seeds", 1), shape=(n,)).copy()
+    assert np.array_equal(site, spin) and np.array_equal(seed, seeds)
+    assert spk.energy() == orc.energy(d, spin)
+    # a second run at another temperature continues exactly
+    spk.command("temperature 0.3")
+    spk.command("run 0.05")
+    d2 = orc.make_desc("potts", "sq/8n", 128, 128, 1, 100, 0.3)
+    orc.sweeps(d2, spin, seeds, n2)
+    site = np.ctypeslib.as_array(spk.extract("site", 1), shape=(n,)).copy()
+    assert np.array_equal(site, spin)
+    assert spk.extract("time", 3) == t2
+    assert spk.extract("dimension", 0) == 2
+    ids = np.ctypeslib.as_array(spk.extract("id", 1), shape=(n,))
+    assert ids[0] == 1 and ids[-1] == n
+    spk.close()
+
+
+def test_sweep_count_from_time_accumulation(tmp_path):
+    """run 0.02 with Q=1e4 makes 201 sweeps, run 0.01995 makes 200."""
+    base = ("seed 7\napp_style potts 10000\ndimension 3\nlattice sc/26n 1.0\n"
+            "region box block 0 4 0 4 0 4\ncreate_box box\ncreate_sites box\n"
+            "set site range 1 10000\nsweep color/strict\ntemperature 0.5\ndiag_style energy\n")
+    for runtime, n in (("0.02", 201), ("0.01995", 200)):
+        p = tmp_path / ("in.t" + runtime)
+        p.write_text(base + "run %s\n" % runtime)
+        rows = stats_rows(run_exe(tmp_path, str(p)))
+        assert rows[-1][3] == n
+        assert rows[-1][1] + rows[-1][2] == n * 64
+
+
+def test_full_size_scripts_first_sweeps(orc, tmp_path):
+    """in.cfg2 / in.cfg3 unchanged except the run length (-var T): the first
+    two sweeps of the 4096^2 Ising and 256^3 Potts runs against the oracle."""
+    cases = [("in.cfg2", "ising", "sq/4n", (4096, 4096, 1), 2, 2.269, 12345, "1.0", "0.5"),
+             ("in.cfg3", "potts", "sc/26n", (256, 256, 256), 10000, 0.5, 24680, "0.00015", "1.0e-4")]
+    for script, app, lat, shape, q, kt, seed, T, DT in cases:
+        out = run_exe(tmp_path, os.path.join(SCRIPTS, script), "-var", "T", T, "-var", "DT", DT)
+        rows = stats_rows(out)
+        d = orc.make_desc(app, lat, *shape, q, kt)
+        spin, seeds = orc.script_init(d, seed, 1, q)
+        e0 = orc.energy(d, spin)
+        nacc, eng = orc.sweeps(d, spin, seeds, 2)
+        assert rows[0][4] == float("%10g" % e0)
+        assert [r[1] for r in rows[1:]] == list(nacc)
+        assert [r[4] for r in rows[1:]] == [float("%10g" % e) for e in eng]
+
+
+def test_dump_text_snapshots(orc, tmp_path):
+    p = tmp_path / "in.dump"
+    p.write_text("seed 99\napp_style ising\ndimension 2\nlattice sq/4n 1.0\n"
+                 "region box block 0 16 0 12 -0.5 0.5\ncreate_box box\ncreate_sites box\n"
+                 "set site range 1 2\nsweep color/strict\ntemperature 1.5\n"
+                 "dump 1 text 1.0 dump.ising id site x y energy\nrun 2.0\n")
+    run_exe(tmp_path, str(p))
+    text = open(tmp_path / "dump.ising").read().split("ITEM: TIMESTEP\n")[1:]
+    assert len(text) == 3   # t = 0, 1, 2
+    d = orc.make_desc("ising", "sq/4n", 16, 12, 1, 2, 1.5)
+    spin, seeds = orc.script_init(d, 99, 1, 2)
+    nb = orc.neighbors(d)
+    for snap, nsw in zip(text, (0, 2, 2)):
+        if nsw:
+            orc.sweeps(d, spin, seeds, nsw)
+        lines = snap.splitlines()
+        assert lines[1:3] == ["ITEM: NUMBER OF ATOMS", "192"]
+        assert lines[7] == "ITEM: ATOMS id type x y energy"
+        vals = np.array([[float(x) for x in ln.split()] for ln in lines[8:]])
+        assert np.array_equal(vals[:, 0], np.arange(1, 193))
+        assert np.array_equal(vals[:, 1], spin)
+        assert np.array_equal(vals[:, 2], np.arange(192) % 16)
+        assert np.array_equal(vals[:, 4], (spin[nb] != spin[:, None]).sum(1))
+
+
+def test_invalid_values_error(tmp_path):
+    for body, msg in (("", "invalid values"), ("set site value 11\n", "invalid values")):
+        p = tmp_path / "in.bad"
+        p.write_text("seed 5\napp_style potts 10\ndimension 2\nlattice sq/8n 1.0\n"
+                     "region box block 0 8 0 8 -0.5 0.5\ncreate_box box\ncreate_sites box\n" + body +
+                     "sweep color/strict\ntemperature 1.0\nrun 1.0\n")
+        r = subprocess.run([EXE, "-in", str(p), "-log", "none"], capture_output=True, text=True, cwd=tmp_path,
+                           timeout=60)
+        assert r.returncode != 0 and msg in r.stdout + r.stderr
