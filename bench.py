@@ -36,12 +36,19 @@ WORKLOADS = {
 NCOLORS = {"sq/4n": 2, "sq/8n": 4, "sc/6n": 2, "sc/26n": 8}
 
 
-def bytes_per_update(lattice):
+def spin_bytes(app, q):
+    """Device storage of one spin: the narrowest of uint8/uint16/int32
+    holding 1..Q."""
+    qmax = q if app == "potts" else 2
+    return 1 if qmax <= 255 else 2 if qmax <= 65535 else 4
+
+
+def bytes_per_update(lattice, sbytes):
     """Algorithmic HBM bytes per site update of the per-colour design: each
-    colour pass reads the whole int32 spin array (its sites' neighbourhoods
-    cover it) and reads+writes the int32 seed and writes the int32 spin of
-    its own N/K sites: 4K + 12 bytes per update."""
-    return 4 * NCOLORS[lattice] + 12
+    colour pass reads the whole spin array (its sites' neighbourhoods cover
+    it), reads and writes the int32 seed and writes the spin of its own N/K
+    sites: K*s + 8 + s bytes for s-byte spins (4K + 12 for int32)."""
+    return NCOLORS[lattice] * sbytes + 8 + sbytes
 
 
 def load_traffic(workload):
@@ -115,7 +122,8 @@ def run(args):
     updates = n * steps
     value = updates / wall / 1e9
     per_launch_s = dev_ms / 1e3 / nlaunch
-    B = bytes_per_update(lattice)
+    B = bytes_per_update(lattice, spin_bytes(app, q))
+    B_survey = bytes_per_update(lattice, 4)
     bytes_per_launch = B * n // K
     achieved = bytes_per_launch / per_launch_s / 1e9
     out = {
@@ -138,11 +146,13 @@ def run(args):
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": load_traffic(args.workload),
-                     "kernel": "k_sweep_color<%s>" % lattice,
+                     "kernel": "k_sweep_color<%s, %s, 4>" % (lattice, {1: "u8", 2: "u16", 4: "i32"}[spin_bytes(app, q)]),
                      "bytes_per_launch": bytes_per_launch,
                      "avg_launch_us": round(per_launch_s * 1e6, 3),
                      "bytes_per_update": B},
         "pct_hbm_peak": round(100 * value * 1e9 * B / (HBM_PEAK_GBS * 1e9), 2),
+        "survey_model_pct_hbm": round(100 * value * 1e9 * B_survey / (HBM_PEAK_GBS * 1e9), 2),
+        "spin_bytes": spin_bytes(app, q),
     }
     if not args.no_cpu_baseline:
         threads = args.cpu_threads or min(os.cpu_count() or 1, 16)
