@@ -141,6 +141,73 @@ int spk_hip_set_launch(spk_hip_lattice *h, int block);
    and the number of sweep-kernel launches it made. */
 int spk_hip_last_timing(spk_hip_lattice *h, double *ms, int64_t *nlaunch);
 
+/* ======================================================================
+   Batched replicas: many independent lattices (each its own script: app,
+   lattice, size, Q, kT, seed, sweep count) resident in one device pool and
+   advanced together.  Each colour step of a block is one persistent kernel
+   over a tile work-list spanning every replica that is due a colour pass
+   at that step; a replica's K*n passes of an n-sweep block are spread evenly
+   over the block's steps, always in colour order 1..K.  Every replica's
+   trajectory is identical to running it alone (the strict sweep is
+   decomposition independent), which the tests check.  Blocks after the
+   first are replayed as HIP graphs.  No counterpart in the reference, which
+   runs one lattice per process (its closest relative is `-partition`,
+   src/spparks.cpp:58-65).
+   ====================================================================== */
+
+typedef struct spk_hip_batch spk_hip_batch;   /* opaque handle */
+
+typedef struct spk_hip_replica_desc {
+  spk_hip_desc lattice;   /* app, lattice style, edges, Q, kT               */
+  int seed;               /* the replica script's `seed S`                  */
+  int lo, hi;             /* `set site range lo hi`; lo == hi: `set site value lo` */
+  int nsweeps;            /* strict sweeps to run (`run` length)            */
+} spk_hip_replica_desc;
+
+/* Device pool of pool_bytes for up to max_replicas replicas. */
+int spk_hip_batch_create(int64_t pool_bytes, int max_replicas, spk_hip_batch **out);
+void spk_hip_batch_destroy(spk_hip_batch *b);
+
+/* Bytes a replica of this description occupies in a pool (admission). */
+int spk_hip_replica_bytes(const spk_hip_desc *d, int64_t *bytes);
+
+/* Admit a replica: allocate it in the pool and build its initial state on
+   the device (set range / value, strict seeds) exactly as its script would
+   (seed, set, first run's draws).  SPK_HIP_ENOMEM when the pool has no room
+   or all replica slots are taken; the handle is unchanged then. */
+int spk_hip_batch_add(spk_hip_batch *b, const spk_hip_replica_desc *r, int *id);
+/* Retire a replica and return its pool space. */
+int spk_hip_batch_remove(spk_hip_batch *b, int id);
+
+/* Spins / seeds of one replica in the reference's site order (NULL skips). */
+int spk_hip_batch_upload(spk_hip_batch *b, int id, const int32_t *spin, const int32_t *seed);
+int spk_hip_batch_download(spk_hip_batch *b, int id, int32_t *spin, int32_t *seed);
+
+/* Advance every replica with sweeps left by min(left, max_sweeps) sweeps.
+   use_graph != 0 captures the block's launches into a HIP graph (cached by
+   block shape) and replays it. */
+int spk_hip_batch_run(spk_hip_batch *b, int max_sweeps, int use_graph);
+
+/* Per replica: sweeps done and left, accepted flips and energy now. */
+int spk_hip_batch_status(spk_hip_batch *b, int id, int *done, int *left, int64_t *naccept,
+                         double *energy);
+/* Cumulative Naccept and energy after each of the first n sweeps. */
+int spk_hip_batch_trace(spk_hip_batch *b, int id, int n, int64_t *naccept, double *energy);
+
+/* Launch shape of the step kernel: block = threads per tile (64..1024,
+   multiple of 64), grid = persistent workgroups.  0, 0 = autotune: the next
+   run times candidate shapes on its own steps (results do not depend on the
+   shape) and keeps the fastest. */
+int spk_hip_batch_set_launch(spk_hip_batch *b, int block, int grid);
+
+/* Last run: device milliseconds between events around its steps, site
+   updates it made, step kernels it launched, and the launch shape used. */
+int spk_hip_batch_last_timing(spk_hip_batch *b, double *ms, int64_t *updates, int64_t *nlaunch,
+                              int *block, int *grid);
+
+/* Pool occupancy: bytes in use, largest free extent, live replicas. */
+int spk_hip_batch_pool(spk_hip_batch *b, int64_t *used, int64_t *largest_free, int *live);
+
 #ifdef __cplusplus
 }
 #endif

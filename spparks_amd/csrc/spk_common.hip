@@ -39,17 +39,8 @@ int make_geom(const spk_hip_desc *d, Geom *g) {
     return set_error(SPK_HIP_EINVAL, "Color stencil is incommensurate with lattice size");
   const long long n = (long long)d->nx * d->ny * d->nz;
   if (n > 2147483647LL) return set_error(SPK_HIP_EINVAL, "lattice exceeds 2^31-1 sites");
-  g->lat = lat;
-  g->nx = d->nx;
-  g->ny = d->ny;
-  g->nz = d->nz;
-  switch (lat) {
-    case SPK_HIP_SQ_4N: g->K = 2; g->hx = d->nx / 2; g->hy = d->ny; g->hz = 1; break;
-    case SPK_HIP_SC_6N: g->K = 2; g->hx = d->nx / 2; g->hy = d->ny; g->hz = d->nz; break;
-    case SPK_HIP_SQ_8N: g->K = 4; g->hx = d->nx / 2; g->hy = d->ny / 2; g->hz = 1; break;
-    default: g->K = 8; g->hx = d->nx / 2; g->hy = d->ny / 2; g->hz = d->nz / 2; break;
-  }
-  g->M = (int)(n / g->K);
+  geom_init(*g, lat, d->nx, d->ny, d->nz);
+  if ((long long)g->Mp * g->K > 2147483647LL) return set_error(SPK_HIP_EINVAL, "lattice exceeds 2^31-1 slots");
   return SPK_HIP_OK;
 }
 

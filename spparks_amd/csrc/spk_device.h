@@ -39,9 +39,11 @@ __host__ __device__ __forceinline__ uint32_t pm_mulmod(uint32_t a, uint32_t b) {
 
 // ------------------------------------------------------------ geometry
 // Colour-blocked layout.  Colour c (0-based; the reference's colour c+1,
-// src/lattice.cpp:234-283) occupies [c*M, (c+1)*M).  Within a colour the
-// sites form a dense (hx, hy, hz) sub-grid, x fastest, local index
-// a + hx*(b + hy*e):
+// src/lattice.cpp:234-283) occupies [c*Mp, (c+1)*Mp).  Within a colour the
+// sites form an (hx, hy, hz) sub-grid, x fastest, stored with a row pitch px
+// (hx rounded up to a multiple of 4, so 4-site packs never straddle rows):
+// local index a + px*(b + hy*e), Mp = px*hy*hz; the px-hx pad slots of a row
+// hold no site and are never read as a neighbour.
 //   sq/4n : hx=nx/2 hy=ny   hz=1    site (2a + ((b+c)&1), b)
 //   sc/6n : hx=nx/2 hy=ny   hz=nz   site (2a + ((b+e+c)&1), b, e)
 //   sq/8n : hx=nx/2 hy=ny/2 hz=1    c = 2(j&1)+(i&1),        site (2a+pi, 2b+pj)
@@ -53,8 +55,27 @@ struct Geom {
   int nx, ny, nz;
   int hx, hy, hz;
   int K;       // colours
-  int M;       // sites per colour
+  int M;       // sites per colour (hx*hy*hz)
+  int px;      // row pitch: hx rounded up to a multiple of 4
+  int Mp;      // stored slots per colour (px*hy*hz)
 };
+
+// Fills the colour sub-grid of a lattice style; edges must be even.
+__host__ __device__ inline void geom_init(Geom &g, int lat, int nx, int ny, int nz) {
+  g.lat = lat;
+  g.nx = nx;
+  g.ny = ny;
+  g.nz = nz;
+  switch (lat) {
+    case SPK_HIP_SQ_4N: g.K = 2; g.hx = nx / 2; g.hy = ny; g.hz = 1; break;
+    case SPK_HIP_SC_6N: g.K = 2; g.hx = nx / 2; g.hy = ny; g.hz = nz; break;
+    case SPK_HIP_SQ_8N: g.K = 4; g.hx = nx / 2; g.hy = ny / 2; g.hz = 1; break;
+    default: g.K = 8; g.hx = nx / 2; g.hy = ny / 2; g.hz = nz / 2; break;
+  }
+  g.M = g.hx * g.hy * g.hz;
+  g.px = (g.hx + 3) & ~3;
+  g.Mp = g.px * g.hy * g.hz;
+}
 
 __host__ __device__ inline int maxneigh_of(int lat) {
   return lat == SPK_HIP_SQ_4N ? 4 : lat == SPK_HIP_SQ_8N ? 8 : lat == SPK_HIP_SC_6N ? 6 : 26;
@@ -67,22 +88,22 @@ __host__ __device__ __forceinline__ int64_t blocked_index(const Geom &g, int i, 
   switch (g.lat) {
     case SPK_HIP_SQ_4N:
       c = (i + j) & 1;
-      local = (i >> 1) + (int64_t)g.hx * j;
+      local = (i >> 1) + (int64_t)g.px * j;
       break;
     case SPK_HIP_SC_6N:
       c = (i + j + k) & 1;
-      local = (i >> 1) + (int64_t)g.hx * (j + (int64_t)g.ny * k);
+      local = (i >> 1) + (int64_t)g.px * (j + (int64_t)g.ny * k);
       break;
     case SPK_HIP_SQ_8N:
       c = 2 * (j & 1) + (i & 1);
-      local = (i >> 1) + (int64_t)g.hx * (j >> 1);
+      local = (i >> 1) + (int64_t)g.px * (j >> 1);
       break;
     default:
       c = 4 * (k & 1) + 2 * (j & 1) + (i & 1);
-      local = (i >> 1) + (int64_t)g.hx * ((j >> 1) + (int64_t)g.hy * (k >> 1));
+      local = (i >> 1) + (int64_t)g.px * ((j >> 1) + (int64_t)g.hy * (k >> 1));
       break;
   }
-  return (int64_t)c * g.M + local;
+  return (int64_t)c * g.Mp + local;
 }
 
 __host__ __device__ __forceinline__ int wrap_dec(int a, int n) { return a == 0 ? n - 1 : a - 1; }
@@ -101,26 +122,26 @@ template <int LAT, typename T>
 __host__ __device__ __forceinline__ void gather(const T *__restrict__ spin, const Geom &g,
                                                 int c, int a, int b, int e, int32_t *nb) {
   if constexpr (LAT == SPK_HIP_SQ_4N) {
-    const T *o = spin + (int64_t)(1 - c) * g.M;
+    const T *o = spin + (int64_t)(1 - c) * g.Mp;
     int p = (b + c) & 1;
     int xo = p ? wrap_inc(a, g.hx) : wrap_dec(a, g.hx);
-    int64_t row = (int64_t)g.hx * b;
+    int64_t row = (int64_t)g.px * b;
     nb[0] = o[a + row];
     nb[1] = o[xo + row];
-    nb[2] = o[a + (int64_t)g.hx * wrap_dec(b, g.hy)];
-    nb[3] = o[a + (int64_t)g.hx * wrap_inc(b, g.hy)];
+    nb[2] = o[a + (int64_t)g.px * wrap_dec(b, g.hy)];
+    nb[3] = o[a + (int64_t)g.px * wrap_inc(b, g.hy)];
   } else if constexpr (LAT == SPK_HIP_SC_6N) {
-    const T *o = spin + (int64_t)(1 - c) * g.M;
+    const T *o = spin + (int64_t)(1 - c) * g.Mp;
     int p = (b + e + c) & 1;
     int xo = p ? wrap_inc(a, g.hx) : wrap_dec(a, g.hx);
-    int64_t hxy = (int64_t)g.hx * g.hy;
-    int64_t row = (int64_t)g.hx * b + hxy * e;
+    int64_t hxy = (int64_t)g.px * g.hy;
+    int64_t row = (int64_t)g.px * b + hxy * e;
     nb[0] = o[a + row];
     nb[1] = o[xo + row];
-    nb[2] = o[a + (int64_t)g.hx * wrap_dec(b, g.hy) + hxy * e];
-    nb[3] = o[a + (int64_t)g.hx * wrap_inc(b, g.hy) + hxy * e];
-    nb[4] = o[a + (int64_t)g.hx * b + hxy * wrap_dec(e, g.hz)];
-    nb[5] = o[a + (int64_t)g.hx * b + hxy * wrap_inc(e, g.hz)];
+    nb[2] = o[a + (int64_t)g.px * wrap_dec(b, g.hy) + hxy * e];
+    nb[3] = o[a + (int64_t)g.px * wrap_inc(b, g.hy) + hxy * e];
+    nb[4] = o[a + (int64_t)g.px * b + hxy * wrap_dec(e, g.hz)];
+    nb[5] = o[a + (int64_t)g.px * b + hxy * wrap_inc(e, g.hz)];
   } else if constexpr (LAT == SPK_HIP_SQ_8N) {
     const int pi = c & 1, pj = c >> 1;
     // x index per offset: dx=0 -> a, dx=-1 -> a+pi-1, dx=+1 -> a+pi
@@ -133,7 +154,7 @@ __host__ __device__ __forceinline__ void gather(const T *__restrict__ spin, cons
       for (int dx = 0; dx < 3; dx++) {
         if (dx == 1 && dy == 1) continue;
         int q = (pi ^ (dx != 1)) + 2 * (pj ^ (dy != 1));
-        nb[n++] = spin[(int64_t)q * g.M + xs[dx] + (int64_t)g.hx * ys[dy]];
+        nb[n++] = spin[(int64_t)q * g.Mp + xs[dx] + (int64_t)g.px * ys[dy]];
       }
   } else {
     const int pi = c & 1, pj = (c >> 1) & 1, pk = c >> 2;
@@ -149,8 +170,8 @@ __host__ __device__ __forceinline__ void gather(const T *__restrict__ spin, cons
         for (int dx = 0; dx < 3; dx++) {
           if (dx == 1 && dy == 1 && dz == 1) continue;
           int q = (pi ^ (dx != 1)) + 2 * (pj ^ (dy != 1)) + 4 * (pk ^ (dz != 1));
-          nb[n++] = spin[(int64_t)q * g.M + xs[dx] +
-                         (int64_t)g.hx * (ys[dy] + (int64_t)g.hy * zs[dz])];
+          nb[n++] = spin[(int64_t)q * g.Mp + xs[dx] +
+                         (int64_t)g.px * (ys[dy] + (int64_t)g.hy * zs[dz])];
         }
   }
 }
@@ -225,8 +246,10 @@ __host__ __device__ __forceinline__ SiteOutcome site_event(const Rule &r, const 
 }
 
 // ------------------------------------------------ vectorised colour pass
-// V consecutive sites of one sub-grid row per thread.  Requires hx % V == 0
-// (so packs never straddle a row and stay V-aligned).
+// V consecutive sites of one sub-grid row per thread; px % V == 0, so packs
+// never straddle a row and stay V-aligned.  The last pack of a row may run
+// into the row's pad slots: those lanes are computed on pad data and
+// discarded.
 template <typename T, int V>
 struct alignas(sizeof(T) * V) Pack {
   T v[V];
@@ -240,19 +263,22 @@ __host__ __device__ __forceinline__ void load_pack(const T *p, int32_t (&out)[V]
 }
 
 // Row segment of V+1 values of the other x-parity: positions
-// a0+off-1 .. a0+off+V-1 (off = 0 or 1) with periodic wrap.  Site v of the
-// thread has its two x-neighbours at seg[v] and seg[v+1].
+// a0+off-1 .. a0+off+V-1 (off = 0 or 1) with periodic wrap at hx.  Site v of
+// the thread (a0+v < hx) has its two x-neighbours at seg[v] and seg[v+1].
 template <typename T, int V>
 __host__ __device__ __forceinline__ void load_segment(const T *row, int a0, int hx, int off,
                                                       int32_t (&seg)[V + 1]) {
   int32_t pk[V];
   load_pack<T, V>(row + a0, pk);
-  const int xe = off ? (a0 + V == hx ? 0 : a0 + V) : (a0 == 0 ? hx - 1 : a0 - 1);
+  // the one element outside the pack: a0-1 (off 0) or a0+V (off 1)
+  const int xe = off ? (a0 + V >= hx ? 0 : a0 + V) : (a0 == 0 ? hx - 1 : a0 - 1);
   const int32_t extra = (int32_t)row[xe];
 #pragma unroll
   for (int v = 0; v <= V; v++) {
     const int32_t lo = v == 0 ? extra : pk[v - 1];     // off == 0
-    const int32_t hi = v == V ? extra : pk[v];         // off == 1
+    // off == 1: position a0+v, which wraps to 0 at a0+v == hx (the last
+    // site's right neighbour); pad positions beyond hx feed only pad lanes
+    const int32_t hi = (v == V || a0 + v == hx) ? extra : pk[v];
     seg[v] = off ? hi : lo;
   }
 }
@@ -269,7 +295,7 @@ template <int LAT, typename T, int V>
 __host__ __device__ __forceinline__ Counts sweep_sites(T *__restrict__ spin, int32_t *__restrict__ seed,
                                                        const Geom &g, const Rule &r, const int *thr,
                                                        int c, int a0, int b, int e) {
-  const int64_t self = (int64_t)c * g.M + a0 + (int64_t)g.hx * (b + (int64_t)g.hy * e);
+  const int64_t self = (int64_t)c * g.Mp + a0 + (int64_t)g.px * (b + (int64_t)g.hy * e);
   int32_t old[V], s[V], prop[V], ei[V], ef[V];
   load_pack<T, V>(spin + self, old);
   {
@@ -301,29 +327,29 @@ __host__ __device__ __forceinline__ Counts sweep_sites(T *__restrict__ spin, int
       count1(v, sg[v + 1]);
     }
   };
-  const int64_t hx = g.hx, hxy = (int64_t)g.hx * g.hy;
+  const int64_t px = g.px, pxy = (int64_t)g.px * g.hy;
 
   if constexpr (LAT == SPK_HIP_SQ_4N) {
-    const T *o = spin + (int64_t)(1 - c) * g.M;
-    count_segment(o + hx * b, (b + c) & 1);
-    count_pack(o + hx * wrap_dec(b, g.hy));
-    count_pack(o + hx * wrap_inc(b, g.hy));
+    const T *o = spin + (int64_t)(1 - c) * g.Mp;
+    count_segment(o + px * b, (b + c) & 1);
+    count_pack(o + px * wrap_dec(b, g.hy));
+    count_pack(o + px * wrap_inc(b, g.hy));
   } else if constexpr (LAT == SPK_HIP_SC_6N) {
-    const T *o = spin + (int64_t)(1 - c) * g.M;
-    count_segment(o + hx * b + hxy * e, (b + e + c) & 1);
-    count_pack(o + hx * wrap_dec(b, g.hy) + hxy * e);
-    count_pack(o + hx * wrap_inc(b, g.hy) + hxy * e);
-    count_pack(o + hx * b + hxy * wrap_dec(e, g.hz));
-    count_pack(o + hx * b + hxy * wrap_inc(e, g.hz));
+    const T *o = spin + (int64_t)(1 - c) * g.Mp;
+    count_segment(o + px * b + pxy * e, (b + e + c) & 1);
+    count_pack(o + px * wrap_dec(b, g.hy) + pxy * e);
+    count_pack(o + px * wrap_inc(b, g.hy) + pxy * e);
+    count_pack(o + px * b + pxy * wrap_dec(e, g.hz));
+    count_pack(o + px * b + pxy * wrap_inc(e, g.hz));
   } else if constexpr (LAT == SPK_HIP_SQ_8N) {
     const int pi = c & 1, pj = c >> 1;
     const int ys[3] = {pj ? b : wrap_dec(b, g.hy), b, pj ? wrap_inc(b, g.hy) : b};
 #pragma unroll
     for (int dy = 0; dy < 3; dy++) {
       const int qy = pj ^ (dy != 1);
-      const int64_t row = hx * ys[dy];
-      if (dy != 1) count_pack(spin + (int64_t)(pi + 2 * qy) * g.M + row);
-      count_segment(spin + (int64_t)((1 - pi) + 2 * qy) * g.M + row, pi);
+      const int64_t row = px * ys[dy];
+      if (dy != 1) count_pack(spin + (int64_t)(pi + 2 * qy) * g.Mp + row);
+      count_segment(spin + (int64_t)((1 - pi) + 2 * qy) * g.Mp + row, pi);
     }
   } else {
     const int pi = c & 1, pj = (c >> 1) & 1, pk = c >> 2;
@@ -334,9 +360,9 @@ __host__ __device__ __forceinline__ Counts sweep_sites(T *__restrict__ spin, int
 #pragma unroll
       for (int dy = 0; dy < 3; dy++) {
         const int qyz = 2 * (pj ^ (dy != 1)) + 4 * (pk ^ (dz != 1));
-        const int64_t row = hx * ys[dy] + hxy * zs[dz];
-        if (dy != 1 || dz != 1) count_pack(spin + (int64_t)(pi + qyz) * g.M + row);
-        count_segment(spin + (int64_t)((1 - pi) + qyz) * g.M + row, pi);
+        const int64_t row = px * ys[dy] + pxy * zs[dz];
+        if (dy != 1 || dz != 1) count_pack(spin + (int64_t)(pi + qyz) * g.Mp + row);
+        count_segment(spin + (int64_t)((1 - pi) + qyz) * g.Mp + row, pi);
       }
   }
 
@@ -347,7 +373,8 @@ __host__ __device__ __forceinline__ Counts sweep_sites(T *__restrict__ spin, int
 #pragma unroll
   for (int v = 0; v < V; v++) {
     const bool accept = metropolis(r, thr, ei[v], ef[v], s[v]);
-    const bool changed = accept && prop[v] != old[v];
+    const bool site = a0 + v < g.hx;            // pad lanes count nothing
+    const bool changed = site && accept && prop[v] != old[v];
     nsp.v[v] = (T)(changed ? prop[v] : old[v]);
     nsd.v[v] = s[v];
     any |= changed;

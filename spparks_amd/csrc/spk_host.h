@@ -18,4 +18,52 @@ int make_geom(const spk_hip_desc *d, Geom *g);
 // Metropolis thresholds and proposal parameters for one temperature
 int make_rule(int app, int nspins, int lattice, double temperature, Rule *r);
 
+enum SpinType { kU8 = 0, kU16 = 1, kI32 = 2 };
+
+// narrowest storage holding 1..Q
+inline int spin_type_for(int app, int nspins) {
+  const int qmax = app == SPK_HIP_POTTS ? nspins : 2;
+  return qmax <= 255 ? kU8 : qmax <= 65535 ? kU16 : kI32;
+}
+inline size_t spin_bytes(int stype) { return stype == kU8 ? 1 : stype == kU16 ? 2 : 4; }
+
+// Calls f.template operator()<T>() with the spin storage type.
+template <typename F>
+void with_type(int stype, F &&f) {
+  switch (stype) {
+    case kU8: f.template operator()<uint8_t>(); break;
+    case kU16: f.template operator()<uint16_t>(); break;
+    default: f.template operator()<int32_t>(); break;
+  }
+}
+
+// Calls f.template operator()<LAT>() for the lattice style.
+template <typename F>
+void with_lattice(int lat, F &&f) {
+  switch (lat) {
+    case SPK_HIP_SQ_4N: f.template operator()<SPK_HIP_SQ_4N>(); break;
+    case SPK_HIP_SQ_8N: f.template operator()<SPK_HIP_SQ_8N>(); break;
+    case SPK_HIP_SC_6N: f.template operator()<SPK_HIP_SC_6N>(); break;
+    default: f.template operator()<SPK_HIP_SC_26N>(); break;
+  }
+}
+
+// 16807^(2^b) mod m, b = 0..31 (jump-ahead for `set site range`)
+struct PowTable { uint32_t p[32]; };
+inline PowTable park_powers() {
+  PowTable pw;
+  uint32_t p = kIA;
+  for (int b = 0; b < 32; b++) {
+    pw.p[b] = p;
+    p = pm_mulmod(p, p);
+  }
+  return pw;
+}
+// 16807^100 mod m: tagreset's 100 warm-up steps
+inline uint32_t park_warmup100() {
+  uint32_t a = 1;
+  for (int w = 0; w < 100; w++) a = pm_mulmod(a, kIA);
+  return a;
+}
+
 }  // namespace spk

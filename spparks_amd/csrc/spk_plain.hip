@@ -13,6 +13,7 @@
 
 #include "spk_device.h"
 #include "spk_host.h"
+#include "spk_kernels.h"
 
 using namespace spk;
 
@@ -32,9 +33,9 @@ __global__ __launch_bounds__(1024) void k_sweep_color(T *__restrict__ spin, int3
 
   const int tile = xcd_tile(blockIdx.x, gridDim.x);
   const int64_t t = (int64_t)tile * blockDim.x + threadIdx.x;
-  const int per_row = g.hx / V;
+  const int per_row = g.px / V;
   Counts cnt{0, 0};
-  if (t < (int64_t)(g.M / V)) {
+  if (t < (int64_t)(g.Mp / V)) {
     const int ti = (int)t;
     const int a0 = (ti % per_row) * V;
     const int rest = ti / per_row;
@@ -43,114 +44,7 @@ __global__ __launch_bounds__(1024) void k_sweep_color(T *__restrict__ spin, int3
   block_accumulate(cnt.nacc, cnt.de, acc, blockIdx.x & (kAccSlots - 1));
 }
 
-// Total energy: sum over all sites of #unlike neighbours.
-template <int LAT, typename T>
-__global__ __launch_bounds__(256) void k_energy(const T *__restrict__ spin, Geom g,
-                                                unsigned long long *__restrict__ acc) {
-  constexpr int NN = NeighCount<LAT>::value;
-  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  int eng = 0;
-  if (t < (int64_t)g.M * g.K) {
-    const int c = (int)(t / g.M);
-    const int ti = (int)(t - (int64_t)c * g.M);
-    const int a = ti % g.hx;
-    const int rest = ti / g.hx;
-    int32_t nb[NN];
-    gather<LAT, T>(spin, g, c, a, rest % g.hy, rest / g.hy, nb);
-    const int32_t me = spin[t];
-#pragma unroll
-    for (int m = 0; m < NN; m++) eng += (nb[m] != me);
-  }
-  block_accumulate(eng, 0, acc, blockIdx.x & (kAccSlots - 1));
-}
-
-__device__ __forceinline__ void natural_ijk(const Geom &g, int64_t t, int &i, int &j, int &k) {
-  i = (int)(t % g.nx);
-  const int64_t r = t / g.nx;
-  j = (int)(r % g.ny);
-  k = (int)(r / g.ny);
-}
-
-// natural order <-> colour-blocked order (with the storage type change)
-template <typename T>
-__global__ void k_to_blocked(const int32_t *__restrict__ nat, T *__restrict__ blk, Geom g) {
-  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= (int64_t)g.M * g.K) return;
-  int i, j, k;
-  natural_ijk(g, t, i, j, k);
-  blk[blocked_index(g, i, j, k)] = (T)nat[t];
-}
-
-template <typename T>
-__global__ void k_to_natural(const T *__restrict__ blk, int32_t *__restrict__ nat, Geom g) {
-  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= (int64_t)g.M * g.K) return;
-  int i, j, k;
-  natural_ijk(g, t, i, j, k);
-  nat[t] = (int32_t)blk[blocked_index(g, i, j, k)];
-}
-
-// counts values outside [lo, hi]
-__global__ void k_count_outside(const int32_t *__restrict__ v, int64_t n, int lo, int hi,
-                                unsigned long long *__restrict__ bad) {
-  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  int out = 0;
-  if (t < n) out = (v[t] < lo || v[t] > hi);
-  out = wave_sum(out);
-  if ((threadIdx.x & 63) == 0 && out) atomicAdd(bad, (unsigned long long)out);
-}
-
-// `set site range lo hi`, loop all: site id gid = t+1 draws
-// irandom(range) from a generator advanced gid times from seed0
-// (src/set.cpp:598-627); 16807^gid mod m from the binary powers in pw.
-struct PowTable { uint32_t p[32]; };
-
-template <typename T>
-__global__ void k_set_range(T *__restrict__ spin, Geom g, int seed0, int lo, int range, PowTable pw) {
-  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= (int64_t)g.M * g.K) return;
-  const uint64_t gid = (uint64_t)t + 1;
-  uint32_t s = (uint32_t)seed0;
-#pragma unroll
-  for (int bit = 0; bit < 32; bit++)
-    if ((gid >> bit) & 1u) s = pm_mulmod(s, pw.p[bit]);
-  const double u = kAM * (double)(int32_t)s;      // RandomPark::uniform
-  int v = (int)(u * (double)range) + 1;           // irandom(range)
-  if (v > range) v = range;
-  int i, j, k;
-  natural_ijk(g, t, i, j, k);
-  spin[blocked_index(g, i, j, k)] = (T)(v - 1 + lo);
-}
-
-template <typename T>
-__global__ void k_fill(T *__restrict__ v, int64_t n, int32_t value) {
-  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t < n) v[t] = (T)value;
-}
-
-// siteseeds = tagreset(rseed, id, 100) (src/random_park.cpp:67-73).
-// xbase = rseed*IM is formed on the host; x = xbase + id is one rounded
-// addition as in the reference (no contraction: built with
-// -ffp-contract=off).  For 0 <= x < 2*IM, fmod(x, IM) is x or the exact
-// difference x-IM (Sterbenz).  The 100 warm-up steps are one multiply by
-// 16807^100 mod m.
-__global__ void k_init_seeds(int32_t *__restrict__ seed, Geom g, double xbase, uint32_t a100) {
-  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= (int64_t)g.M * g.K) return;
-  const double im = (double)kIM;
-  double x = xbase + (double)(t + 1);
-  while (x >= im) x -= im;
-  int32_t s = (int32_t)x;
-  if (s == 0) s = 1;
-  s = (int32_t)pm_mulmod((uint32_t)s, a100);
-  int i, j, k;
-  natural_ijk(g, t, i, j, k);
-  seed[blocked_index(g, i, j, k)] = s;
-}
-
 // ===================================================================== host
-
-enum SpinType { kU8 = 0, kU16 = 1, kI32 = 2 };
 
 struct spk_hip_lattice {
   spk_hip_desc d;
@@ -159,10 +53,10 @@ struct spk_hip_lattice {
   int64_t N = 0;
   int device = 0;
   int stype = kI32;                    // storage type of spins on the device
-  int vec = 1;                         // sites per thread in the sweep kernel
+  int64_t slots = 0;                   // K*Mp stored values (sites + row pads)
   hipStream_t stream = nullptr;
-  void *spin = nullptr;                // N spins, colour-blocked, stype
-  int32_t *seed = nullptr;             // N seeds, colour-blocked
+  void *spin = nullptr;                // slots spins, colour-blocked, stype
+  int32_t *seed = nullptr;             // slots seeds, colour-blocked
   int32_t *stage = nullptr;            // N int32, natural order (transfers)
   unsigned long long *acc = nullptr;   // [cap][kAccSlots][2]
   int acc_cap = 0;
@@ -178,50 +72,19 @@ struct spk_hip_lattice {
 namespace {
 
 constexpr int kInitialAccRows = 1024;
+constexpr int kVec = 4;   // sites per thread in the sweep kernel
 
 unsigned grid_for(int64_t n, int bs) { return (unsigned)((n + bs - 1) / bs); }
 
-int spin_type_for(int app, int nspins) {
-  const int qmax = app == SPK_HIP_POTTS ? nspins : 2;
-  return qmax <= 255 ? kU8 : qmax <= 65535 ? kU16 : kI32;
-}
-size_t spin_bytes(int stype) { return stype == kU8 ? 1 : stype == kU16 ? 2 : 4; }
-
-// Calls f.template operator()<T>() with the handle's spin storage type.
-template <typename F>
-void with_type(int stype, F &&f) {
-  switch (stype) {
-    case kU8: f.template operator()<uint8_t>(); break;
-    case kU16: f.template operator()<uint16_t>(); break;
-    default: f.template operator()<int32_t>(); break;
-  }
-}
-
-// Calls f.template operator()<LAT>() for the lattice style.
-template <typename F>
-void with_lattice(int lat, F &&f) {
-  switch (lat) {
-    case SPK_HIP_SQ_4N: f.template operator()<SPK_HIP_SQ_4N>(); break;
-    case SPK_HIP_SQ_8N: f.template operator()<SPK_HIP_SQ_8N>(); break;
-    case SPK_HIP_SC_6N: f.template operator()<SPK_HIP_SC_6N>(); break;
-    default: f.template operator()<SPK_HIP_SC_26N>(); break;
-  }
-}
-
 int launch_sweep(spk_hip_lattice *h, int color, unsigned long long *acc) {
   const int bs = h->block;
-  const int64_t threads = h->g.M / h->vec;
+  const int64_t threads = h->g.Mp / kVec;
   unsigned grid = grid_for(threads, bs);
   grid = (grid + kNumXcd - 1) / kNumXcd * kNumXcd;   // xcd_tile needs a multiple of 8
   with_lattice(h->g.lat, [&]<int LAT>() {
     with_type(h->stype, [&]<typename T>() {
-      T *sp = static_cast<T *>(h->spin);
-      if (h->vec == 4)
-        hipLaunchKernelGGL((k_sweep_color<LAT, T, 4>), dim3(grid), dim3(bs), 0, h->stream, sp, h->seed, h->g,
-                           h->rule, color, acc);
-      else
-        hipLaunchKernelGGL((k_sweep_color<LAT, T, 1>), dim3(grid), dim3(bs), 0, h->stream, sp, h->seed, h->g,
-                           h->rule, color, acc);
+      hipLaunchKernelGGL((k_sweep_color<LAT, T, kVec>), dim3(grid), dim3(bs), 0, h->stream,
+                         static_cast<T *>(h->spin), h->seed, h->g, h->rule, color, acc);
     });
   });
   return check_hip(hipGetLastError(), "sweep kernel launch");
@@ -239,8 +102,8 @@ int launch_energy(spk_hip_lattice *h, unsigned long long *acc) {
 
 int launch_fill_spins(spk_hip_lattice *h, int32_t value) {
   with_type(h->stype, [&]<typename T>() {
-    hipLaunchKernelGGL((k_fill<T>), dim3(grid_for(h->N, 256)), dim3(256), 0, h->stream,
-                       static_cast<T *>(h->spin), h->N, value);
+    hipLaunchKernelGGL((k_fill<T>), dim3(grid_for(h->slots, 256)), dim3(256), 0, h->stream,
+                       static_cast<T *>(h->spin), h->slots, value);
   });
   return check_hip(hipGetLastError(), "fill kernel launch");
 }
@@ -332,8 +195,8 @@ int spk_hip_create(const spk_hip_desc *d, spk_hip_lattice **out) {
   h->g = g;
   h->rule = r;
   h->N = (int64_t)g.M * g.K;
+  h->slots = (int64_t)g.Mp * g.K;
   h->stype = spin_type_for(d->app, d->nspins);
-  h->vec = g.hx % 4 == 0 ? 4 : 1;
   h->block = g.lat == SPK_HIP_SQ_4N ? 512 : 256;   // measured best on MI355X (4096^2, 256^3)
   if ((rc = check_hip(hipGetDevice(&h->device), "hipGetDevice")) ||
       (rc = check_hip(hipStreamCreate(&h->stream), "hipStreamCreate")) ||
@@ -342,9 +205,9 @@ int spk_hip_create(const spk_hip_desc *d, spk_hip_lattice **out) {
     spk_hip_destroy(h);
     return rc;
   }
-  const size_t bytes = (size_t)h->N * sizeof(int32_t);
-  if (hipMalloc(&h->spin, (size_t)h->N * spin_bytes(h->stype)) != hipSuccess ||
-      hipMalloc(&h->seed, bytes) != hipSuccess || hipMalloc(&h->stage, bytes) != hipSuccess) {
+  if (hipMalloc(&h->spin, (size_t)h->slots * spin_bytes(h->stype)) != hipSuccess ||
+      hipMalloc(&h->seed, (size_t)h->slots * sizeof(int32_t)) != hipSuccess ||
+      hipMalloc(&h->stage, (size_t)h->N * sizeof(int32_t)) != hipSuccess) {
     spk_hip_destroy(h);
     return set_error(SPK_HIP_ENOMEM, "device allocation of %lld sites failed", (long long)h->N);
   }
@@ -352,7 +215,8 @@ int spk_hip_create(const spk_hip_desc *d, spk_hip_lattice **out) {
     spk_hip_destroy(h);
     return rc;
   }
-  hipLaunchKernelGGL(k_fill<int32_t>, dim3(grid_for(h->N, 256)), dim3(256), 0, h->stream, h->seed, h->N, 1);
+  hipLaunchKernelGGL(k_fill<int32_t>, dim3(grid_for(h->slots, 256)), dim3(256), 0, h->stream, h->seed, h->slots,
+                     1);
   if ((rc = check_hip(hipStreamSynchronize(h->stream), "create sync"))) {
     spk_hip_destroy(h);
     return rc;
@@ -448,12 +312,7 @@ int spk_hip_set_range(spk_hip_lattice *h, int seed0, int lo, int hi) {
   if (lo < 1 || hi > qmax)
     return set_error(SPK_HIP_EVALUE, "One or more sites have invalid values (set range %d %d, Q = %d)", lo, hi,
                      qmax);
-  PowTable pw;
-  uint32_t p = kIA;
-  for (int b = 0; b < 32; b++) {
-    pw.p[b] = p;
-    p = pm_mulmod(p, p);
-  }
+  const PowTable pw = park_powers();
   with_type(h->stype, [&]<typename T>() {
     hipLaunchKernelGGL((k_set_range<T>), dim3(grid_for(h->N, 256)), dim3(256), 0, h->stream,
                        static_cast<T *>(h->spin), h->g, seed0, lo, hi - lo + 1, pw);
@@ -479,8 +338,7 @@ int spk_hip_init_seeds(spk_hip_lattice *h, double rseed) {
   if (!h) return set_error(SPK_HIP_EINVAL, "null handle");
   if (!(rseed >= 0.0 && rseed < 1.0)) return set_error(SPK_HIP_EINVAL, "init_seeds: rseed outside [0,1)");
   const double xbase = rseed * (double)kIM;
-  uint32_t a100 = 1;
-  for (int w = 0; w < 100; w++) a100 = pm_mulmod(a100, kIA);
+  const uint32_t a100 = park_warmup100();
   hipLaunchKernelGGL(k_init_seeds, dim3(grid_for(h->N, 256)), dim3(256), 0, h->stream, h->seed, h->g, xbase,
                      a100);
   int rc = check_hip(hipGetLastError(), "init_seeds launch");

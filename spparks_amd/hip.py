@@ -26,6 +26,11 @@ ABI_SYMBOLS = (
     "spk_hip_set_value", "spk_hip_init_seeds", "spk_hip_sweeps",
     "spk_hip_energy", "spk_hip_sync", "spk_hip_set_launch",
     "spk_hip_last_timing",
+    "spk_hip_batch_create", "spk_hip_batch_destroy", "spk_hip_replica_bytes",
+    "spk_hip_batch_add", "spk_hip_batch_remove", "spk_hip_batch_upload",
+    "spk_hip_batch_download", "spk_hip_batch_run", "spk_hip_batch_status",
+    "spk_hip_batch_trace", "spk_hip_batch_set_launch", "spk_hip_batch_last_timing",
+    "spk_hip_batch_pool",
 )
 
 

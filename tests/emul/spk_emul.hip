@@ -4,7 +4,7 @@
 // infrastructure; not part of the library.
 //
 // usage: spk_emul app lat nx ny nz nspins kT nsweeps stype vec in.bin out.bin
-//   stype: 0 uint8, 1 uint16, 2 int32 spin storage; vec: sites per thread
+//   stype: 0 uint8, 1 uint16, 2 int32 spin storage; vec: sites per thread (4)
 //   in.bin : int32 spin[N], int32 seed[N] (reference order)
 //   out.bin: int32 spin[N], int32 seed[N], int64 naccept[nsweeps],
 //            int64 energy[nsweeps]
@@ -18,14 +18,7 @@ using namespace spk;
 
 static Geom geom(int lat, int nx, int ny, int nz) {
   Geom g{};
-  g.lat = lat; g.nx = nx; g.ny = ny; g.nz = nz;
-  switch (lat) {
-    case SPK_HIP_SQ_4N: g.K = 2; g.hx = nx / 2; g.hy = ny; g.hz = 1; break;
-    case SPK_HIP_SC_6N: g.K = 2; g.hx = nx / 2; g.hy = ny; g.hz = nz; break;
-    case SPK_HIP_SQ_8N: g.K = 4; g.hx = nx / 2; g.hy = ny / 2; g.hz = 1; break;
-    default: g.K = 8; g.hx = nx / 2; g.hy = ny / 2; g.hz = nz / 2; break;
-  }
-  g.M = nx * ny * nz / g.K;
+  geom_init(g, lat, nx, ny, nz);
   return g;
 }
 
@@ -40,7 +33,7 @@ static long long energy(const std::vector<T> &spin, const Geom &g) {
       int a = t % g.hx, rest = t / g.hx;
       int32_t nb[NN];
       gather<LAT, T>(spin.data(), g, c, a, rest % g.hy, rest / g.hy, nb);
-      for (int m = 0; m < NN; m++) e += nb[m] != (int32_t)spin.at((long long)c * g.M + t);
+      for (int m = 0; m < NN; m++) e += nb[m] != (int32_t)spin.at((long long)c * g.Mp + a + (long long)g.px * rest);
     }
   return e;
 }
@@ -48,8 +41,9 @@ static long long energy(const std::vector<T> &spin, const Geom &g) {
 template <int LAT, typename T, int V>
 static int run(int app, int nx, int ny, int nz, int q, double kt, int nsw, const char *in, const char *out) {
   Geom g = geom(LAT, nx, ny, nz);
-  if (g.hx % V) { fprintf(stderr, "hx %% V != 0\n"); return 5; }
+  if (g.px % V) { fprintf(stderr, "px %% V != 0\n"); return 5; }
   const long long N = (long long)nx * ny * nz;
+  const long long S = (long long)g.K * g.Mp;   // stored slots, row pads included
   Rule r{};
   r.app = app; r.nspins = app == SPK_HIP_POTTS ? q : 2; r.zero_t = kt == 0.0;
   int32_t thr[27];
@@ -57,28 +51,29 @@ static int run(int app, int nx, int ny, int nz, int q, double kt, int nsw, const
   r.half = thr[0];
   for (int d = 1; d <= maxneigh_of(LAT); d++) r.thr[d] = thr[d];
   std::vector<int32_t> sn(N), kn(N);
-  // exact-size heap buffers so ASan sees any out-of-range pack or segment
-  std::vector<T> s(N);
-  std::vector<int32_t> k(N);
+  // exact-size heap buffers so ASan sees any out-of-range pack or segment;
+  // pad slots get poison values that would show if a site ever read one
+  std::vector<T> s(S, (T)0);
+  std::vector<int32_t> k(S, 0x7ffffff0);
   FILE *f = fopen(in, "rb");
   if (!f || fread(sn.data(), 4, N, f) != (size_t)N || fread(kn.data(), 4, N, f) != (size_t)N) return 2;
   fclose(f);
-  std::vector<char> hit(N, 0);
+  std::vector<char> hit(S, 0);
   for (long long t = 0; t < N; t++) {
     int i = t % nx, j = (t / nx) % ny, kk = t / ((long long)nx * ny);
     long long p = blocked_index(g, i, j, kk);
-    if (p < 0 || p >= N || hit[p]) { fprintf(stderr, "blocked_index not a bijection\n"); return 3; }
+    if (p < 0 || p >= S || hit[p]) { fprintf(stderr, "blocked_index not injective\n"); return 3; }
     hit[p] = 1;
     s[p] = (T)sn[t];
     k[p] = kn[t];
   }
   std::vector<long long> nacc(nsw), eng(nsw);
   long long na = 0, e = energy<LAT, T>(s, g);
-  const int per_row = g.hx / V;
+  const int per_row = g.px / V;
   for (int w = 0; w < nsw; w++) {
     long long de = 0;
     for (int c = 0; c < g.K; c++)
-      for (int t = 0; t < g.M / V; t++) {
+      for (int t = 0; t < g.Mp / V; t++) {
         int a0 = (t % per_row) * V, rest = t / per_row;
         Counts cnt = sweep_sites<LAT, T, V>(s.data(), k.data(), g, r, r.thr, c, a0, rest % g.hy, rest / g.hy);
         na += cnt.nacc;
@@ -108,11 +103,8 @@ template <int LAT>
 static int by_type(int stype, int vec, int app, int nx, int ny, int nz, int q, double kt, int nsw, const char *in,
                    const char *out) {
   switch (stype * 10 + vec) {
-    case 1: return run<LAT, uint8_t, 1>(app, nx, ny, nz, q, kt, nsw, in, out);
     case 4: return run<LAT, uint8_t, 4>(app, nx, ny, nz, q, kt, nsw, in, out);
-    case 11: return run<LAT, uint16_t, 1>(app, nx, ny, nz, q, kt, nsw, in, out);
     case 14: return run<LAT, uint16_t, 4>(app, nx, ny, nz, q, kt, nsw, in, out);
-    case 21: return run<LAT, int32_t, 1>(app, nx, ny, nz, q, kt, nsw, in, out);
     case 24: return run<LAT, int32_t, 4>(app, nx, ny, nz, q, kt, nsw, in, out);
   }
   return 6;
