@@ -190,6 +190,67 @@ def run_ensemble(args):
     }
 
 
+def run_serving(args):
+    """cfg5: Poisson streams of 3d Potts grain-growth solves (sc/26n, edges
+    32-192, Q 100-1e6, kT 0.1-1.0, 50-500 sweeps) at offered loads of 10, 50
+    and 200 requests/s, continuously batched through spk_hip_serve against a
+    200 GB HBM budget.  Reports p50/p99 completion latency and aggregate
+    site-updates/s per load, and checks sampled requests against solo runs."""
+    import numpy as np
+
+    import spparks_amd as S
+    from spparks_amd import batch as bmod
+    from spparks_amd.workloads import serving_requests
+
+    loads = [(10.0, args.serve_n[0]), (50.0, args.serve_n[1]), (200.0, args.serve_n[2])]
+    levels = []
+    sample_bad = 0
+    sample_n = 0
+    for rate, n in loads:
+        reqs = serving_requests(rate, n)
+        res, st = bmod.serve(reqs, pool_bytes=int(args.serve_budget_gb * 1e9), max_live=4096,
+                             block_sweeps=args.serve_block, block=args.block or 256, grid=args.grid or 1024)
+        lat = np.array([o["finish"] - r["arrive"] for r, o in zip(reqs, res) if o["status"] == 0])
+        offered = rate * np.mean([np.prod(r["shape"]) * r["sweeps"] for r in reqs]) / 1e9
+        levels.append({"offered_req_per_s": rate, "requests": n, "completed": int(len(lat)),
+                       "refused": int(sum(o["status"] != 0 for o in res)),
+                       "p50_latency_s": round(float(np.percentile(lat, 50)), 4),
+                       "p99_latency_s": round(float(np.percentile(lat, 99)), 4),
+                       "offered_G_updates_per_s": round(float(offered), 3),
+                       "G_site_updates_per_s": round(st["site_updates"] / st["wall"] / 1e9, 3),
+                       "gpu_busy_frac": round(st["busy"] / st["wall"], 3),
+                       "wall_s": round(st["wall"], 3), "blocks": st["blocks"], "max_live": st["max_live"],
+                       "max_pool_GB": round(st["max_pool_used"] / 1e9, 3)})
+        # a sample of requests solved alone must give the same state
+        rng = np.random.default_rng(int(rate))
+        for i in rng.choice(n, size=min(args.serve_check, n), replace=False):
+            r, o = reqs[int(i)], res[int(i)]
+            with S.Lattice(r["app"], r["lattice"], *r["shape"], nspins=r["nspins"], temperature=r["kT"]) as L:
+                s0, rseed = S.strict_init_params(r["seed"])
+                L.set_range(s0, 1, r["nspins"])
+                L.init_seeds(rseed)
+                nacc, _, _ = L.sweeps(r["sweeps"], per_sweep=False)
+                solo = (L.checksum(), nacc, L.energy())
+            sample_n += 1
+            sample_bad += solo != (o["checksum"], o["naccept"], o["energy"])
+        print(json.dumps({"serving_level": levels[-1]}), file=sys.stderr, flush=True)
+    top = levels[-1]
+    return {
+        "metric": "G site-updates/s & % HBM peak; batched vs plain speedup; p99 solve latency",
+        "value": top["G_site_updates_per_s"], "unit": "G site-updates/s", "n_gpus": 1,
+        "steps": sum(l["requests"] for l in levels), "warmup": 0, "ms_per_step": None,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "int32",
+        "data": "synthetic: Poisson request streams from spparks_amd.workloads.serving_requests()",
+        "config": {"workload": "cfg5: serving 3d Potts sc/26n solves at 10/50/200 req/s, %g GB HBM budget, "
+                               "%d-sweep blocks" % (args.serve_budget_gb, args.serve_block)},
+        "levels": levels,
+        "p99_latency_s_at_200": top["p99_latency_s"],
+        "parity": {"sampled_requests": sample_n, "mismatches": int(sample_bad),
+                   "compared": "checksum, Naccept, energy: in the mix vs solved alone"},
+        "roofline": None, "cpu_baseline": None,
+    }
+
+
 def copy_calibration(mib):
     """A plain elementwise kernel of known traffic (torch.neg on int32: reads
     and writes mib MiB each), for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE."""
@@ -275,7 +336,12 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=None, help="timed sweeps (default: the workload's count)")
     ap.add_argument("--warmup", type=int, default=10)
-    ap.add_argument("--workload", default="cfg2", choices=sorted(WORKLOADS) + ["cfg4"])
+    ap.add_argument("--workload", default="cfg2", choices=sorted(WORKLOADS) + ["cfg4", "cfg5"])
+    ap.add_argument("--serve-n", type=int, nargs=3, default=[150, 500, 1000], metavar=("N10", "N50", "N200"),
+                    help="cfg5: requests per offered load")
+    ap.add_argument("--serve-block", type=int, default=10, help="cfg5: sweeps per batch block")
+    ap.add_argument("--serve-budget-gb", type=float, default=200.0)
+    ap.add_argument("--serve-check", type=int, default=8, help="cfg5: requests per load re-solved alone")
     ap.add_argument("--block-sweeps", type=int, default=25, help="cfg4: sweeps per graph block")
     ap.add_argument("--grid", type=int, default=0, help="cfg4: persistent workgroups (with --block)")
     ap.add_argument("--block", type=int, default=0, help="sweep-kernel workgroup size (0 = default)")
@@ -295,6 +361,8 @@ def main():
         sys.exit(2)
     if args.workload == "cfg4":
         print(json.dumps(run_ensemble(args)), flush=True)
+    elif args.workload == "cfg5":
+        print(json.dumps(run_serving(args)), flush=True)
     else:
         print(json.dumps(run(args)), flush=True)
 

@@ -128,6 +128,13 @@ int spk_hip_sweeps(spk_hip_lattice *h, int nsweeps, int64_t *naccept_total,
    (src/diag_energy.cpp:53-66, exact in double). */
 int spk_hip_energy(spk_hip_lattice *h, double *energy);
 
+/* State checksum, independent of the device layout: the wrapping sum over
+   sites g (0-based, reference order) of mix64(g*0xD1B54A32D192ED03 +
+   spin*0xABC98388FB8FAC03 + seed), with mix64 the splitmix64 finaliser
+   (spin and seed taken as unsigned 32-bit).  Lets a caller compare
+   states without copying them back. */
+int spk_hip_checksum(spk_hip_lattice *h, uint64_t *sum);
+
 /* Block until all work queued on the handle's stream has finished. */
 int spk_hip_sync(spk_hip_lattice *h);
 
@@ -207,6 +214,49 @@ int spk_hip_batch_last_timing(spk_hip_batch *b, double *ms, int64_t *updates, in
 
 /* Pool occupancy: bytes in use, largest free extent, live replicas. */
 int spk_hip_batch_pool(spk_hip_batch *b, int64_t *used, int64_t *largest_free, int *live);
+
+/* spk_hip_checksum of one replica. */
+int spk_hip_batch_checksum(spk_hip_batch *b, int id, uint64_t *sum);
+
+/* ======================================================================
+   Serving: continuous batching of a stream of solves.  Requests are
+   admitted in arrival order as soon as they have arrived and the pool
+   (the HBM budget) and a replica slot have room; every block advances all
+   admitted requests together through spk_hip_batch_run; a request whose
+   sweeps are done is checksummed, reported and retired at once, freeing
+   its memory and slot for the next arrivals.  Times are seconds on a
+   monotonic clock from the start of the call; arrivals in the future are
+   waited for.  A request larger than the whole pool is refused.  No
+   counterpart in the reference.
+   ====================================================================== */
+
+typedef struct spk_hip_request {
+  spk_hip_replica_desc replica;   /* the solve: lattice, seed, set, sweeps */
+  double arrive;                  /* seconds after the start of the call   */
+} spk_hip_request;
+
+typedef struct spk_hip_result {
+  double admit;                   /* admitted into the pool               */
+  double finish;                  /* sweeps done and state checksummed    */
+  int64_t naccept;
+  double energy;
+  uint64_t checksum;              /* spk_hip_checksum of the final state  */
+  int status;                     /* 0 done; SPK_HIP_ENOMEM refused       */
+} spk_hip_result;
+
+typedef struct spk_hip_serve_stats {
+  double wall;                    /* start to last completion             */
+  int64_t site_updates;           /* over completed requests              */
+  int blocks;                     /* batch runs                           */
+  int max_live;                   /* most requests resident at once       */
+  int64_t max_pool_used;          /* bytes                                */
+  double busy;                    /* seconds inside batch runs            */
+} spk_hip_serve_stats;
+
+/* req must be sorted by arrival time.  block_sweeps: sweeps per block;
+   block, grid: launch shape of the step kernels (0, 0 = autotune). */
+int spk_hip_serve(const spk_hip_request *req, int n, int64_t pool_bytes, int max_live, int block_sweeps,
+                  int block, int grid, spk_hip_result *out, spk_hip_serve_stats *stats);
 
 #ifdef __cplusplus
 }

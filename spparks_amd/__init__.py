@@ -4,9 +4,9 @@ The compute path is libspk_hip.so (hand-written HIP kernels behind the C ABI in
 include/spk_hip.h); this package holds its ctypes binding, the build script
 and the Python side of the batched / serving drivers.
 """
-from .hip import (APPS, LATTICES, HipError, Lattice, load_library,  # noqa: F401
+from .hip import (APPS, LATTICES, HipError, Lattice, checksum, load_library,  # noqa: F401
                   master_draws, metropolis_table, ncolors,
                   strict_init_params)
 
-__all__ = ["APPS", "LATTICES", "HipError", "Lattice", "load_library",
+__all__ = ["APPS", "LATTICES", "HipError", "Lattice", "checksum", "load_library",
            "master_draws", "metropolis_table", "ncolors", "strict_init_params"]

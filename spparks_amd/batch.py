@@ -15,6 +15,20 @@ class ReplicaDesc(C.Structure):
                 ("nsweeps", C.c_int)]
 
 
+class Request(C.Structure):
+    _fields_ = [("replica", ReplicaDesc), ("arrive", C.c_double)]
+
+
+class Result(C.Structure):
+    _fields_ = [("admit", C.c_double), ("finish", C.c_double), ("naccept", C.c_int64),
+                ("energy", C.c_double), ("checksum", C.c_uint64), ("status", C.c_int)]
+
+
+class ServeStats(C.Structure):
+    _fields_ = [("wall", C.c_double), ("site_updates", C.c_int64), ("blocks", C.c_int),
+                ("max_live", C.c_int), ("max_pool_used", C.c_int64), ("busy", C.c_double)]
+
+
 _bound = False
 
 
@@ -40,6 +54,9 @@ def _bind():
     L.spk_hip_batch_last_timing.argtypes = [vp, C.POINTER(dbl), C.POINTER(i64), C.POINTER(i64),
                                             C.POINTER(i32), C.POINTER(i32)]
     L.spk_hip_batch_pool.argtypes = [vp, C.POINTER(i64), C.POINTER(i64), C.POINTER(i32)]
+    L.spk_hip_batch_checksum.argtypes = [vp, i32, C.POINTER(C.c_uint64)]
+    L.spk_hip_serve.argtypes = [C.POINTER(Request), i32, i64, i32, i32, i32, i32, C.POINTER(Result),
+                                C.POINTER(ServeStats)]
     _bound = True
     return L
 
@@ -141,7 +158,33 @@ class Batch:
         return {"ms": ms.value, "updates": upd.value, "launches": nl.value, "block": bs.value,
                 "grid": gr.value}
 
+    def checksum(self, rid):
+        v = C.c_uint64()
+        _check(self.L.spk_hip_batch_checksum(self.h, int(rid), C.byref(v)))
+        return v.value
+
     def pool(self):
         used, largest, live = C.c_int64(), C.c_int64(), C.c_int()
         _check(self.L.spk_hip_batch_pool(self.h, C.byref(used), C.byref(largest), C.byref(live)))
         return used.value, largest.value, live.value
+
+
+def serve(requests, pool_bytes, max_live, block_sweeps=10, block=0, grid=0):
+    """Continuous batching of `requests` (dicts with app, lattice, shape,
+    nspins, kT, seed, sweeps, arrive; sorted by arrive) through
+    spk_hip_serve.  Returns (results as dicts, stats dict)."""
+    L = _bind()
+    n = len(requests)
+    arr = (Request * max(n, 1))()
+    for i, r in enumerate(requests):
+        arr[i].replica = replica_desc(r["app"], r["lattice"], *r["shape"], nspins=r["nspins"],
+                                      temperature=r["kT"], seed=r["seed"], nsweeps=r["sweeps"])
+        arr[i].arrive = float(r["arrive"])
+    out = (Result * max(n, 1))()
+    st = ServeStats()
+    _check(L.spk_hip_serve(arr, n, int(pool_bytes), int(max_live), int(block_sweeps), int(block), int(grid),
+                           out, C.byref(st)))
+    res = [{"admit": o.admit, "finish": o.finish, "naccept": o.naccept, "energy": o.energy,
+            "checksum": o.checksum, "status": o.status} for o in out[:n]]
+    stats = {f: getattr(st, f) for f, _ in ServeStats._fields_}
+    return res, stats

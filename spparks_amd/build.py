@@ -17,7 +17,7 @@ CSRC = os.path.join(HERE, "csrc")
 LIBDIR = os.path.join(HERE, "lib")
 INCLUDE = os.path.join(ROOT, "include")
 
-HIP_SOURCES = ["spk_common.hip", "spk_plain.hip", "spk_batch.hip"]
+HIP_SOURCES = ["spk_common.hip", "spk_plain.hip", "spk_batch.hip", "spk_serve.hip"]
 FRONT_SOURCES = ["front/spk_front.cpp"]
 CXX_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-Wall", "-Wextra", "-ffp-contract=off"]
 

@@ -789,4 +789,25 @@ int spk_hip_batch_pool(spk_hip_batch *b, int64_t *used, int64_t *largest_free, i
   return SPK_HIP_OK;
 }
 
+int spk_hip_batch_checksum(spk_hip_batch *b, int id, uint64_t *sum) {
+  int rc = check_id(b, id);
+  if (rc) return rc;
+  if (!sum) return set_error(SPK_HIP_EINVAL, "null argument");
+  const Replica &R = b->reps[id];
+  const int64_t n = (int64_t)R.g.M * R.g.K;
+  if ((rc = check_hip(hipMemsetAsync(b->scratch, 0, sizeof(unsigned long long), b->stream), "memset"))) return rc;
+  with_type(R.stype, [&]<typename T>() {
+    hipLaunchKernelGGL((k_checksum<T>), dim3(grid_for(n, 256)), dim3(256), 0, b->stream,
+                       reinterpret_cast<const T *>(b->pool + R.spin_off),
+                       reinterpret_cast<const int32_t *>(b->pool + R.seed_off), R.g, b->scratch);
+  });
+  if ((rc = check_hip(hipGetLastError(), "checksum launch"))) return rc;
+  unsigned long long v = 0;
+  if ((rc = check_hip(hipMemcpyAsync(&v, b->scratch, sizeof v, hipMemcpyDeviceToHost, b->stream), "readback")) ||
+      (rc = check_hip(hipStreamSynchronize(b->stream), "sync")))
+    return rc;
+  *sum = v;
+  return SPK_HIP_OK;
+}
+
 }  // extern "C"

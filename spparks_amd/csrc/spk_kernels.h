@@ -114,5 +114,30 @@ __global__ void k_init_seeds(int32_t *__restrict__ seed, Geom g, double xbase, u
 }
 
 
+// Layout-independent state checksum: sum over sites g (0-based, reference
+// order) of mix64(g*C1 + spin*C2 + seed), wrapping mod 2^64; mix64 is the
+// splitmix64 finaliser.  Equal (spin, seed) arrays give equal sums.
+__host__ __device__ __forceinline__ uint64_t mix64(uint64_t z) {
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+
+template <typename T>
+__global__ void k_checksum(const T *__restrict__ spin, const int32_t *__restrict__ seed, Geom g,
+                           unsigned long long *__restrict__ out) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  uint64_t h = 0;
+  if (t < (int64_t)g.M * g.K) {
+    int i, j, k;
+    natural_ijk(g, t, i, j, k);
+    const int64_t p = blocked_index(g, i, j, k);
+    h = mix64((uint64_t)t * 0xD1B54A32D192ED03ull + (uint64_t)(uint32_t)(int32_t)spin[p] * 0xABC98388FB8FAC03ull +
+              (uint64_t)(uint32_t)seed[p]);
+  }
+  h = (uint64_t)wave_sum64((long long)h);
+  if ((threadIdx.x & 63) == 0) atomicAdd(out, (unsigned long long)h);
+}
+
 }  // namespace
 }  // namespace spk

@@ -400,6 +400,24 @@ int spk_hip_energy(spk_hip_lattice *h, double *energy) {
   return SPK_HIP_OK;
 }
 
+int spk_hip_checksum(spk_hip_lattice *h, uint64_t *sum) {
+  if (!h || !sum) return set_error(SPK_HIP_EINVAL, "null argument");
+  int rc = ensure_acc(h, 1);
+  if (rc) return rc;
+  if ((rc = check_hip(hipMemsetAsync(h->acc, 0, sizeof(unsigned long long), h->stream), "memset"))) return rc;
+  with_type(h->stype, [&]<typename T>() {
+    hipLaunchKernelGGL((k_checksum<T>), dim3(grid_for(h->N, 256)), dim3(256), 0, h->stream,
+                       static_cast<const T *>(h->spin), h->seed, h->g, h->acc);
+  });
+  if ((rc = check_hip(hipGetLastError(), "checksum launch"))) return rc;
+  unsigned long long v = 0;
+  if ((rc = check_hip(hipMemcpyAsync(&v, h->acc, sizeof v, hipMemcpyDeviceToHost, h->stream), "readback")) ||
+      (rc = check_hip(hipStreamSynchronize(h->stream), "sync")))
+    return rc;
+  *sum = v;
+  return SPK_HIP_OK;
+}
+
 int spk_hip_sync(spk_hip_lattice *h) {
   if (!h) return set_error(SPK_HIP_EINVAL, "null handle");
   return check_hip(hipStreamSynchronize(h->stream), "sync");

@@ -30,7 +30,7 @@ ABI_SYMBOLS = (
     "spk_hip_batch_add", "spk_hip_batch_remove", "spk_hip_batch_upload",
     "spk_hip_batch_download", "spk_hip_batch_run", "spk_hip_batch_status",
     "spk_hip_batch_trace", "spk_hip_batch_set_launch", "spk_hip_batch_last_timing",
-    "spk_hip_batch_pool",
+    "spk_hip_batch_pool", "spk_hip_checksum", "spk_hip_batch_checksum", "spk_hip_serve",
 )
 
 
@@ -77,6 +77,7 @@ def load_library(path=None):
     L.spk_hip_energy.argtypes = [vp, C.POINTER(C.c_double)]
     L.spk_hip_sync.argtypes = [vp]
     L.spk_hip_set_launch.argtypes = [vp, C.c_int]
+    L.spk_hip_checksum.argtypes = [vp, C.POINTER(C.c_uint64)]
     L.spk_hip_last_timing.argtypes = [vp, C.POINTER(C.c_double), C.POINTER(C.c_int64)]
     if path is None:
         _lib = L
@@ -112,6 +113,21 @@ def metropolis_table(temperature, maxd):
     _check(load_library().spk_hip_metropolis_table(float(temperature), int(maxd),
                                                    tab.ctypes.data, thr.ctypes.data))
     return tab, thr
+
+
+def checksum(spin, seed):
+    """spk_hip_checksum of host arrays in reference site order: the wrapping
+    sum over sites g of mix64(g*C1 + spin*C2 + seed), mix64 = splitmix64's
+    finaliser."""
+    g = np.arange(len(spin), dtype=np.uint64)
+    with np.errstate(over="ignore"):
+        z = (g * np.uint64(0xD1B54A32D192ED03)
+             + np.asarray(spin).astype(np.uint32).astype(np.uint64) * np.uint64(0xABC98388FB8FAC03)
+             + np.asarray(seed).astype(np.uint32).astype(np.uint64))
+        z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+        z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+        z = z ^ (z >> np.uint64(31))
+        return int(z.sum(dtype=np.uint64))
 
 
 def master_draws(seed, n):
@@ -213,6 +229,11 @@ class Lattice:
 
     def set_launch(self, block):
         _check(self.L.spk_hip_set_launch(self.h, int(block)))
+
+    def checksum(self):
+        v = C.c_uint64()
+        _check(self.L.spk_hip_checksum(self.h, C.byref(v)))
+        return v.value
 
     def last_timing(self):
         ms = C.c_double()
