@@ -9,6 +9,8 @@
 #include <cstring>
 #include <new>
 #include <string>
+#include <algorithm>
+#include <utility>
 #include <vector>
 
 #include "spk_device.h"
@@ -23,7 +25,7 @@ using namespace spk;
 // Thread t owns V consecutive sites of one sub-grid row (x fastest), so its
 // own spins/seeds are one aligned pack and the wavefront's accesses are
 // unit-stride.  Workgroups are remapped so neighbouring tiles share an XCD.
-template <int LAT, typename T, int V>
+template <int LAT, typename T, int V, bool TWO>
 __global__ __launch_bounds__(1024) void k_sweep_color(T *__restrict__ spin, int32_t *__restrict__ seed,
                                                       Geom g, Rule r, int color,
                                                       unsigned long long *__restrict__ acc) {
@@ -39,9 +41,89 @@ __global__ __launch_bounds__(1024) void k_sweep_color(T *__restrict__ spin, int3
     const int ti = (int)t;
     const int a0 = (ti % per_row) * V;
     const int rest = ti / per_row;
-    cnt = sweep_sites<LAT, T, V>(spin, seed, g, r, s_thr, color, a0, rest % g.hy, rest / g.hy);
+    cnt = sweep_sites<LAT, T, V, TWO>(spin, seed, g, r, s_thr, color, a0, rest % g.hy, rest / g.hy);
   }
   block_accumulate(cnt.nacc, cnt.de, acc, blockIdx.x & (kAccSlots - 1));
+}
+
+// S whole sweeps of a 2d lattice per launch, one extended tile per
+// workgroup in LDS (see FusedTile).  Reads one state buffer, writes the
+// other; per-sweep Naccept / dE go to acc rows 0..S-1.
+template <int LAT, typename T, int S, bool TWO>
+__global__ __launch_bounds__(512) void k_sweep_fused2d(const T *__restrict__ spin_in, const int32_t *__restrict__ seed_in,
+                                                       T *__restrict__ spin_out, int32_t *__restrict__ seed_out,
+                                                       Geom g, Rule r, int TX, int TY, int ntiles, int lds_sites,
+                                                       unsigned long long *__restrict__ acc) {
+  constexpr int K = LAT == SPK_HIP_SQ_4N ? 2 : 4;
+  constexpr int H = K * S;
+  extern __shared__ int32_t lds_seed[];               // lds_sites seeds, then lds_sites spins
+  __shared__ int s_thr[32];
+  const int tile = xcd_tile(blockIdx.x, gridDim.x);
+  if (tile >= ntiles) return;                          // whole workgroup, before any barrier
+  const FusedTile t = fused_tile(g, TX, TY, H, tile);
+  int32_t *sd = lds_seed;
+  T *sp = reinterpret_cast<T *>(lds_seed + lds_sites);
+  if (threadIdx.x < 28) s_thr[threadIdx.x] = r.thr[threadIdx.x];
+  const int gx0 = t.X0 - H, gy0 = t.Y0 - H;
+  const int ext = t.EW * t.EH;
+  {
+    int ly = threadIdx.x / t.EW, lx = threadIdx.x - ly * t.EW;   // then stepped without division
+    const int step_y = blockDim.x / t.EW, step_x = blockDim.x - step_y * t.EW;
+    for (int idx = threadIdx.x; idx < ext; idx += blockDim.x) {
+      const int64_t at = blocked_index2d<LAT>(g, wrap_near(gx0 + lx, g.nx), wrap_near(gy0 + ly, g.ny));
+      sp[idx] = spin_in[at];
+      sd[idx] = seed_in[at];
+      lx += step_x;
+      ly += step_y;
+      if (lx >= t.EW) {
+        lx -= t.EW;
+        ly++;
+      }
+    }
+  }
+  __syncthreads();
+  int nacc[S], de[S];
+#pragma unroll
+  for (int sw = 0; sw < S; sw++) {
+    nacc[sw] = 0;
+    de[sw] = 0;
+#pragma unroll
+    for (int c = 0; c < K; c++) {
+      const FusedPass f = fused_pass<LAT>(t, sw * K + c + 1, c);
+      for (int q = threadIdx.x; q < f.nslots; q += blockDim.x) {
+        int lx, ly;
+        if (!fused_site<LAT>(t, f, q, lx, ly)) continue;
+        const SiteOutcome o = fused_event<LAT, T, TWO>(sp, sd, t.EW, lx, ly, r, s_thr);
+        if (fused_in_core(t, lx, ly)) {
+          nacc[sw] += o.changed;
+          de[sw] += o.dE;
+        }
+      }
+      __syncthreads();
+    }
+  }
+  {
+    const int core = t.cw * t.ch;
+    int cy = threadIdx.x / t.cw, cx = threadIdx.x - cy * t.cw;
+    const int step_y = blockDim.x / t.cw, step_x = blockDim.x - step_y * t.cw;
+    for (int idx = threadIdx.x; idx < core; idx += blockDim.x) {
+      const int at = (H + cy) * t.EW + H + cx;
+      const int64_t gp = blocked_index2d<LAT>(g, t.X0 + cx, t.Y0 + cy);
+      spin_out[gp] = sp[at];
+      seed_out[gp] = sd[at];
+      cx += step_x;
+      cy += step_y;
+      if (cx >= t.cw) {
+        cx -= t.cw;
+        cy++;
+      }
+    }
+  }
+#pragma unroll
+  for (int sw = 0; sw < S; sw++) {
+    __syncthreads();   // the previous reduction's LDS slots are free again
+    block_accumulate(nacc[sw], de[sw], acc + (size_t)sw * kAccSlots * 2, blockIdx.x & (kAccSlots - 1));
+  }
 }
 
 // ===================================================================== host
@@ -57,6 +139,10 @@ struct spk_hip_lattice {
   hipStream_t stream = nullptr;
   void *spin = nullptr;                // slots spins, colour-blocked, stype
   int32_t *seed = nullptr;             // slots seeds, colour-blocked
+  void *spin_alt = nullptr;            // second state buffer for sweep-fused launches
+  int32_t *seed_alt = nullptr;
+  int fuse = 0;                        // sweeps per fused launch (0: per-colour kernel only)
+  int tile_x = 0, tile_y = 0;          // fused core tile
   int32_t *stage = nullptr;            // N int32, natural order (transfers)
   unsigned long long *acc = nullptr;   // [cap][kAccSlots][2]
   int acc_cap = 0;
@@ -81,13 +167,69 @@ int launch_sweep(spk_hip_lattice *h, int color, unsigned long long *acc) {
   const int64_t threads = h->g.Mp / kVec;
   unsigned grid = grid_for(threads, bs);
   grid = (grid + kNumXcd - 1) / kNumXcd * kNumXcd;   // xcd_tile needs a multiple of 8
+  const bool two = h->rule.nspins == 2;
   with_lattice(h->g.lat, [&]<int LAT>() {
     with_type(h->stype, [&]<typename T>() {
-      hipLaunchKernelGGL((k_sweep_color<LAT, T, kVec>), dim3(grid), dim3(bs), 0, h->stream,
-                         static_cast<T *>(h->spin), h->seed, h->g, h->rule, color, acc);
+      if (two)
+        hipLaunchKernelGGL((k_sweep_color<LAT, T, kVec, true>), dim3(grid), dim3(bs), 0, h->stream,
+                           static_cast<T *>(h->spin), h->seed, h->g, h->rule, color, acc);
+      else
+        hipLaunchKernelGGL((k_sweep_color<LAT, T, kVec, false>), dim3(grid), dim3(bs), 0, h->stream,
+                           static_cast<T *>(h->spin), h->seed, h->g, h->rule, color, acc);
     });
   });
   return check_hip(hipGetLastError(), "sweep kernel launch");
+}
+
+constexpr int kFusedBlock = 512;
+constexpr size_t kFusedLdsMax = 64 * 1024;
+
+// picks the fused tile for a 2d lattice: S sweeps per launch, halo K*S, the
+// largest core that keeps the extended tile within 64 KB of LDS
+void choose_fusion(spk_hip_lattice *h) {
+  h->fuse = 0;
+  if (h->g.lat != SPK_HIP_SQ_4N && h->g.lat != SPK_HIP_SQ_8N) return;
+  const int S = h->g.lat == SPK_HIP_SQ_4N ? 4 : 2;
+  const int H = (h->g.lat == SPK_HIP_SQ_4N ? 2 : 4) * S;
+  const size_t per = 4 + spin_bytes(h->stype);
+  const int tys[] = {64, 64, 64};
+  const int txs[] = {128, 96, 64};
+  const int TY = tys[h->stype], TX = txs[h->stype];
+  const size_t lds = (size_t)(std::min(TX, h->g.nx) + 2 * H) * (std::min(TY, h->g.ny) + 2 * H) * per;
+  if (lds > kFusedLdsMax) return;
+  h->fuse = S;
+  h->tile_x = TX;
+  h->tile_y = TY;
+}
+
+int launch_fused(spk_hip_lattice *h, unsigned long long *acc) {
+  const int S = h->fuse, H = (h->g.lat == SPK_HIP_SQ_4N ? 2 : 4) * S;
+  const int TX = h->tile_x, TY = h->tile_y;
+  const int ntiles = ((h->g.nx + TX - 1) / TX) * ((h->g.ny + TY - 1) / TY);
+  const unsigned grid = (unsigned)((ntiles + kNumXcd - 1) / kNumXcd * kNumXcd);
+  const int lds_sites = (std::min(TX, h->g.nx) + 2 * H) * (std::min(TY, h->g.ny) + 2 * H);
+  const size_t lds = (size_t)lds_sites * (4 + spin_bytes(h->stype));
+  const bool two = h->rule.nspins == 2;
+  with_type(h->stype, [&]<typename T>() {
+    const T *si = static_cast<const T *>(h->spin);
+    T *so = static_cast<T *>(h->spin_alt);
+    auto go = [&](auto kern) {
+      hipLaunchKernelGGL(kern, dim3(grid), dim3(kFusedBlock), lds, h->stream, si, h->seed, so, h->seed_alt, h->g,
+                         h->rule, TX, TY, ntiles, lds_sites, acc);
+    };
+    if (h->g.lat == SPK_HIP_SQ_4N) {
+      if (two) go(k_sweep_fused2d<SPK_HIP_SQ_4N, T, 4, true>);
+      else go(k_sweep_fused2d<SPK_HIP_SQ_4N, T, 4, false>);
+    } else {
+      if (two) go(k_sweep_fused2d<SPK_HIP_SQ_8N, T, 2, true>);
+      else go(k_sweep_fused2d<SPK_HIP_SQ_8N, T, 2, false>);
+    }
+  });
+  int rc = check_hip(hipGetLastError(), "fused sweep launch");
+  if (rc) return rc;
+  std::swap(h->spin, h->spin_alt);
+  std::swap(h->seed, h->seed_alt);
+  return SPK_HIP_OK;
 }
 
 int launch_energy(spk_hip_lattice *h, unsigned long long *acc) {
@@ -205,9 +347,12 @@ int spk_hip_create(const spk_hip_desc *d, spk_hip_lattice **out) {
     spk_hip_destroy(h);
     return rc;
   }
+  choose_fusion(h);
   if (hipMalloc(&h->spin, (size_t)h->slots * spin_bytes(h->stype)) != hipSuccess ||
       hipMalloc(&h->seed, (size_t)h->slots * sizeof(int32_t)) != hipSuccess ||
-      hipMalloc(&h->stage, (size_t)h->N * sizeof(int32_t)) != hipSuccess) {
+      hipMalloc(&h->stage, (size_t)h->N * sizeof(int32_t)) != hipSuccess ||
+      (h->fuse && (hipMalloc(&h->spin_alt, (size_t)h->slots * spin_bytes(h->stype)) != hipSuccess ||
+                   hipMalloc(&h->seed_alt, (size_t)h->slots * sizeof(int32_t)) != hipSuccess))) {
     spk_hip_destroy(h);
     return set_error(SPK_HIP_ENOMEM, "device allocation of %lld sites failed", (long long)h->N);
   }
@@ -231,6 +376,8 @@ void spk_hip_destroy(spk_hip_lattice *h) {
   if (h->spin) (void)hipFree(h->spin);
   if (h->seed) (void)hipFree(h->seed);
   if (h->stage) (void)hipFree(h->stage);
+  if (h->spin_alt) (void)hipFree(h->spin_alt);
+  if (h->seed_alt) (void)hipFree(h->seed_alt);
   if (h->acc) (void)hipFree(h->acc);
   if (h->ev0) (void)hipEventDestroy(h->ev0);
   if (h->ev1) (void)hipEventDestroy(h->ev1);
@@ -361,8 +508,13 @@ int spk_hip_sweeps(spk_hip_lattice *h, int nsweeps, int64_t *naccept_total, int6
   const size_t row = kAccSlots * 2 * sizeof(unsigned long long);
   if ((rc = check_hip(hipMemsetAsync(h->acc, 0, row * nsweeps, h->stream), "memset"))) return rc;
   if ((rc = check_hip(hipEventRecord(h->ev0, h->stream), "event"))) return rc;
-  for (int s = 0; s < nsweeps; s++)
-    for (int c = 0; c < h->g.K; c++)
+  int64_t nlaunch = 0;
+  int s = 0;
+  if (h->fuse)   // whole groups of S sweeps through the fused kernel
+    for (; s + h->fuse <= nsweeps; s += h->fuse, nlaunch++)
+      if ((rc = launch_fused(h, h->acc + (size_t)s * kAccSlots * 2))) return rc;
+  for (; s < nsweeps; s++)
+    for (int c = 0; c < h->g.K; c++, nlaunch++)
       if ((rc = launch_sweep(h, c, h->acc + (size_t)s * kAccSlots * 2))) return rc;
   if ((rc = check_hip(hipEventRecord(h->ev1, h->stream), "event"))) return rc;
   if ((rc = check_hip(hipMemcpyAsync(h->hacc.data(), h->acc, row * nsweeps, hipMemcpyDeviceToHost, h->stream),
@@ -372,7 +524,7 @@ int spk_hip_sweeps(spk_hip_lattice *h, int nsweeps, int64_t *naccept_total, int6
   float ms = 0.f;
   if ((rc = check_hip(hipEventElapsedTime(&ms, h->ev0, h->ev1), "event time"))) return rc;
   h->last_ms = ms;
-  h->last_nlaunch = (int64_t)nsweeps * h->g.K;
+  h->last_nlaunch = nlaunch;
 
   int64_t nacc = 0, eng = e0;
   const bool track = h->energy_valid || energy_per_sweep;
@@ -428,6 +580,31 @@ int spk_hip_set_launch(spk_hip_lattice *h, int block) {
   if (block < 64 || block > 1024 || block % 64)
     return set_error(SPK_HIP_EINVAL, "block must be a multiple of 64 in [64, 1024]");
   h->block = block;
+  return SPK_HIP_OK;
+}
+
+int spk_hip_set_fusion(spk_hip_lattice *h, int enable) {
+  if (!h) return set_error(SPK_HIP_EINVAL, "null handle");
+  if (!enable) {
+    h->fuse = 0;
+    return SPK_HIP_OK;
+  }
+  if (!h->spin_alt) {
+    choose_fusion(h);
+    if (!h->fuse) return SPK_HIP_OK;   // not a 2d lattice, or the tile does not fit
+    if (hipMalloc(&h->spin_alt, (size_t)h->slots * spin_bytes(h->stype)) != hipSuccess ||
+        hipMalloc(&h->seed_alt, (size_t)h->slots * sizeof(int32_t)) != hipSuccess) {
+      h->fuse = 0;
+      return set_error(SPK_HIP_ENOMEM, "second state buffer for fused sweeps not available");
+    }
+  }
+  choose_fusion(h);
+  return SPK_HIP_OK;
+}
+
+int spk_hip_get_fusion(spk_hip_lattice *h, int *sweeps_per_launch) {
+  if (!h || !sweeps_per_launch) return set_error(SPK_HIP_EINVAL, "null argument");
+  *sweeps_per_launch = h->fuse;
   return SPK_HIP_OK;
 }
 

@@ -31,6 +31,7 @@ ABI_SYMBOLS = (
     "spk_hip_batch_download", "spk_hip_batch_run", "spk_hip_batch_status",
     "spk_hip_batch_trace", "spk_hip_batch_set_launch", "spk_hip_batch_last_timing",
     "spk_hip_batch_pool", "spk_hip_checksum", "spk_hip_batch_checksum", "spk_hip_serve",
+    "spk_hip_set_fusion", "spk_hip_get_fusion",
 )
 
 
@@ -78,6 +79,8 @@ def load_library(path=None):
     L.spk_hip_sync.argtypes = [vp]
     L.spk_hip_set_launch.argtypes = [vp, C.c_int]
     L.spk_hip_checksum.argtypes = [vp, C.POINTER(C.c_uint64)]
+    L.spk_hip_set_fusion.argtypes = [vp, C.c_int]
+    L.spk_hip_get_fusion.argtypes = [vp, C.POINTER(C.c_int)]
     L.spk_hip_last_timing.argtypes = [vp, C.POINTER(C.c_double), C.POINTER(C.c_int64)]
     if path is None:
         _lib = L
@@ -229,6 +232,15 @@ class Lattice:
 
     def set_launch(self, block):
         _check(self.L.spk_hip_set_launch(self.h, int(block)))
+
+    def set_fusion(self, enable):
+        _check(self.L.spk_hip_set_fusion(self.h, 1 if enable else 0))
+
+    @property
+    def fusion(self):
+        v = C.c_int()
+        _check(self.L.spk_hip_get_fusion(self.h, C.byref(v)))
+        return v.value
 
     def checksum(self):
         v = C.c_uint64()

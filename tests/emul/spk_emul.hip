@@ -5,16 +5,23 @@
 //
 // usage: spk_emul app lat nx ny nz nspins kT nsweeps stype vec in.bin out.bin
 //   stype: 0 uint8, 1 uint16, 2 int32 spin storage; vec: sites per thread (4)
+//   SPK_EMUL_FUSE=TXxTY in the environment (2d lattices) runs the
+//   sweep-fused tile algorithm instead, S sweeps per launch, the rest of
+//   the sweeps per colour
 //   in.bin : int32 spin[N], int32 seed[N] (reference order)
 //   out.bin: int32 spin[N], int32 seed[N], int64 naccept[nsweeps],
 //            int64 energy[nsweeps]
 #include <cstdio>
 #include <cstdlib>
+#include <algorithm>
+#include <utility>
 #include <vector>
 
 #include "spk_device.h"
 
 using namespace spk;
+
+static long long S_slots(const Geom &g) { return (long long)g.K * g.Mp; }
 
 static Geom geom(int lat, int nx, int ny, int nz) {
   Geom g{};
@@ -36,6 +43,52 @@ static long long energy(const std::vector<T> &spin, const Geom &g) {
       for (int m = 0; m < NN; m++) e += nb[m] != (int32_t)spin.at((long long)c * g.Mp + a + (long long)g.px * rest);
     }
   return e;
+}
+
+// the fused kernel's work for one launch: every tile, every pass, in order
+template <int LAT, typename T, int S>
+static void fused_launch(const std::vector<T> &sin, const std::vector<int32_t> &kin, std::vector<T> &sout,
+                         std::vector<int32_t> &kout, const Geom &g, const Rule &r, int TX, int TY,
+                         std::vector<long long> &nacc, std::vector<long long> &de) {
+  constexpr int K = LAT == SPK_HIP_SQ_4N ? 2 : 4;
+  constexpr int H = K * S;
+  const int ntiles = ((g.nx + TX - 1) / TX) * ((g.ny + TY - 1) / TY);
+  const int lds_sites = (std::min(TX, g.nx) + 2 * H) * (std::min(TY, g.ny) + 2 * H);
+  for (int tile = 0; tile < ntiles; tile++) {
+    const FusedTile t = fused_tile(g, TX, TY, H, tile);
+    std::vector<T> sp(lds_sites);          // exact-size stand-ins for LDS
+    std::vector<int32_t> sd(lds_sites);
+    const int gx0 = t.X0 - H, gy0 = t.Y0 - H;
+    for (int idx = 0; idx < t.EW * t.EH; idx++) {
+      const int ly = idx / t.EW, lx = idx - ly * t.EW;
+      const long long at = blocked_index2d<LAT>(g, wrap_near(gx0 + lx, g.nx), wrap_near(gy0 + ly, g.ny));
+      if (at != blocked_index(g, wrap_mod(gx0 + lx, g.nx), wrap_mod(gy0 + ly, g.ny), 0)) exit(9);
+      sp.at(idx) = sin.at(at);
+      sd.at(idx) = kin.at(at);
+    }
+    for (int sw = 0; sw < S; sw++)
+      for (int c = 0; c < K; c++) {
+        const FusedPass f = fused_pass<LAT>(t, sw * K + c + 1, c);
+        for (int q = 0; q < f.nslots; q++) {
+          int lx, ly;
+          if (!fused_site<LAT>(t, f, q, lx, ly)) continue;
+          if (fused_color<LAT>(t, lx, ly) != c) { fprintf(stderr, "fused_site gave a wrong colour\n"); exit(7); }
+          const SiteOutcome o = r.nspins == 2 ? fused_event<LAT, T, true>(sp.data(), sd.data(), t.EW, lx, ly, r, r.thr)
+                                              : fused_event<LAT, T, false>(sp.data(), sd.data(), t.EW, lx, ly, r, r.thr);
+          if (fused_in_core(t, lx, ly)) {
+            nacc[sw] += o.changed;
+            de[sw] += o.dE;
+          }
+        }
+      }
+    for (int idx = 0; idx < t.cw * t.ch; idx++) {
+      const int cy = idx / t.cw, cx = idx - cy * t.cw;
+      const int at = (H + cy) * t.EW + H + cx;
+      const long long gp = blocked_index(g, t.X0 + cx, t.Y0 + cy, 0);
+      sout.at(gp) = sp.at(at);
+      kout.at(gp) = sd.at(at);
+    }
+  }
 }
 
 template <int LAT, typename T, int V>
@@ -70,12 +123,37 @@ static int run(int app, int nx, int ny, int nz, int q, double kt, int nsw, const
   std::vector<long long> nacc(nsw), eng(nsw);
   long long na = 0, e = energy<LAT, T>(s, g);
   const int per_row = g.px / V;
-  for (int w = 0; w < nsw; w++) {
+  int w = 0;
+  const char *fz = getenv("SPK_EMUL_FUSE");
+  if (fz && (LAT == SPK_HIP_SQ_4N || LAT == SPK_HIP_SQ_8N)) {
+    int TX = 0, TY = 0;
+    if (sscanf(fz, "%dx%d", &TX, &TY) != 2) return 8;
+    constexpr int S = LAT == SPK_HIP_SQ_4N ? 4 : 2;
+    std::vector<T> s2(S_slots(g), (T)0);
+    std::vector<int32_t> k2(S_slots(g), 0x7ffffff0);
+    for (; w + S <= nsw; w += S) {
+      std::vector<long long> na_s(S, 0), de_s(S, 0);
+      if constexpr (LAT == SPK_HIP_SQ_4N || LAT == SPK_HIP_SQ_8N)
+        fused_launch<LAT, T, S>(s, k, s2, k2, g, r, TX, TY, na_s, de_s);
+      std::swap(s, s2);
+      std::swap(k, k2);
+      for (int q = 0; q < S; q++) {
+        na += na_s[q];
+        e += 2 * de_s[q];
+        nacc[w + q] = na;
+        eng[w + q] = e;
+      }
+      if (e != energy<LAT, T>(s, g)) { fprintf(stderr, "incremental energy drifted (fused)\n"); return 4; }
+    }
+  }
+  for (; w < nsw; w++) {
     long long de = 0;
     for (int c = 0; c < g.K; c++)
       for (int t = 0; t < g.Mp / V; t++) {
         int a0 = (t % per_row) * V, rest = t / per_row;
-        Counts cnt = sweep_sites<LAT, T, V>(s.data(), k.data(), g, r, r.thr, c, a0, rest % g.hy, rest / g.hy);
+        Counts cnt = r.nspins == 2
+                         ? sweep_sites<LAT, T, V, true>(s.data(), k.data(), g, r, r.thr, c, a0, rest % g.hy, rest / g.hy)
+                         : sweep_sites<LAT, T, V, false>(s.data(), k.data(), g, r, r.thr, c, a0, rest % g.hy, rest / g.hy);
         na += cnt.nacc;
         de += cnt.de;
       }

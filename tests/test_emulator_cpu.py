@@ -52,6 +52,55 @@ def storage_variants(app, q, nx):
     return [(t, 4) for t in range(narrow, 3)]
 
 
+def run_emul(orc, tmp_path, case, env_extra=None, variants=None):
+    app, lat, shape, q, kt, seed, n = case
+    d = orc.make_desc(app, lat, *shape, q, kt)
+    spin, seeds = orc.script_init(d, seed, 1, q)
+    fin, fout = tmp_path / "in.bin", tmp_path / "out.bin"
+    with open(fin, "wb") as f:
+        f.write(spin.tobytes())
+        f.write(seeds.tobytes())
+    nacc, eng = orc.sweeps(d, spin, seeds, n)
+    env = dict(os.environ, ASAN_OPTIONS="detect_leaks=0", **(env_extra or {}))
+    for stype, vec in variants or storage_variants(app, q, shape[0]):
+        r = subprocess.run([EMUL, str(orc.APP[app]), str(orc.LATTICE[lat]), *map(str, shape), str(q),
+                            repr(kt), str(n), str(stype), str(vec), str(fin), str(fout)],
+                           env=env, capture_output=True, text=True)
+        assert r.returncode == 0, (stype, vec, r.stderr[-2000:])
+        N = spin.size
+        raw = open(fout, "rb").read()
+        assert np.array_equal(np.frombuffer(raw, np.int32, N, 0), spin), (stype, vec)
+        assert np.array_equal(np.frombuffer(raw, np.int32, N, 4 * N), seeds), (stype, vec)
+        assert np.array_equal(np.frombuffer(raw, np.int64, n, 8 * N), nacc), (stype, vec)
+        assert np.array_equal(np.frombuffer(raw, np.int64, n, 8 * N + 8 * n).astype(np.float64), eng), (stype, vec)
+
+
+FUSED_CASES = [
+    # app, lattice, shape, Q, kT, seed, sweeps (fused: 4 per launch for sq/4n, 2 for sq/8n)
+    ("ising", "sq/4n", (64, 48, 1), 2, 2.269, 777, 9),
+    ("ising", "sq/4n", (2, 2, 1), 2, 2.0, 3, 8),
+    ("ising", "sq/4n", (10, 6, 1), 2, 1.5, 4, 5),
+    ("potts", "sq/4n", (18, 10, 1), 5, 1.1, 5, 8),
+    ("potts", "sq/4n", (40, 34, 1), 65535, 0.9, 47, 4),
+    ("potts", "sq/8n", (16, 12, 1), 100, 0.5, 56789, 5),
+    ("potts", "sq/8n", (2, 2, 1), 3, 0.8, 11, 4),
+    ("potts", "sq/8n", (22, 6, 1), 9, 0.9, 78, 4),
+    ("potts", "sq/8n", (34, 66, 1), 1000000, 0.5, 31, 4),
+    ("ising", "sq/8n", (12, 12, 1), 2, 0.0, 37, 4),
+]
+
+
+@pytest.mark.parametrize("tiles", ["8x4", "16x8", "128x64"])
+@pytest.mark.parametrize("case", FUSED_CASES, ids=lambda c: "%s-%s-%s-q%d" % (c[0], c[1], "x".join(map(str, c[2])), c[3]))
+def test_emulated_fused_tiles_match_oracle(orc, tmp_path, case, tiles):
+    """The sweep-fused tile algorithm (halo K*S in LDS, ping-pong buffers),
+    with tiles smaller than, equal to and larger than the lattice: partial
+    edge tiles and halos that wrap onto the same sites several times."""
+    if not os.path.exists(EMUL):
+        pytest.skip("tests/emul/spk_emul not built")
+    run_emul(orc, tmp_path, case, {"SPK_EMUL_FUSE": tiles})
+
+
 @pytest.mark.parametrize("case", CASES, ids=lambda c: "%s-%s-%s-q%d" % (c[0], c[1], "x".join(map(str, c[2])), c[3]))
 def test_emulated_kernel_matches_oracle(orc, tmp_path, case):
     if not os.path.exists(EMUL):

@@ -139,6 +139,29 @@ def test_launch_shape_independence(block):
     assert np.array_equal(nacc, nacc2) and np.array_equal(eng, eng2)
 
 
+@pytest.mark.parametrize("case", [("ising", "sq/4n", (256, 192, 1), 2, 2.269, 5, 13),
+                                  ("potts", "sq/8n", (130, 66, 1), 300, 0.6, 6, 7),
+                                  ("potts", "sq/4n", (18, 10, 1), 70000, 1.1, 7, 9),
+                                  ("potts", "sq/8n", (2, 2, 1), 3, 0.8, 8, 6)],
+                         ids=["ising-256x192", "potts-sq8n-130x66", "potts-sq4n-i32", "potts-sq8n-2x2"])
+def test_fused_sweeps_equal_oracle_and_unfused(orc, case):
+    """Sweep-fused launches (several sweeps per launch in LDS tiles) against
+    the oracle and against the per-colour kernels, per sweep."""
+    import spparks_amd as S
+    app, lat, shape, q, kt, seed, n = case
+    _, _, spin1, seeds1, nacc, eng = oracle_run(orc, app, lat, shape, q, kt, seed, n)
+    out = []
+    for fuse in (True, False):
+        with gpu_init(S, app, lat, shape, q, kt, seed) as L:
+            L.set_fusion(fuse)
+            assert (L.fusion > 0) == fuse
+            _, gn, ge = L.sweeps(n)
+            out.append((gn, ge) + L.download())
+    for gn, ge, gs, gk in out:
+        assert np.array_equal(gn, nacc) and np.array_equal(ge, eng)
+        assert np.array_equal(gs, spin1) and np.array_equal(gk, seeds1)
+
+
 def test_split_calls_equal_one_call():
     """Consecutive runs continue exactly (seeds persist between runs)."""
     import spparks_amd as S
@@ -177,7 +200,7 @@ def test_temperature_change_between_runs(orc):
 # ------------------------------------------------------- full-size shapes
 
 FULL = [
-    ("ising", "sq/4n", (4096, 4096, 1), 2, 2.269, 12345, 2),
+    ("ising", "sq/4n", (4096, 4096, 1), 2, 2.269, 12345, 5),
     ("potts", "sc/26n", (256, 256, 256), 10000, 0.5, 24680, 2),
 ]
 
