@@ -278,6 +278,9 @@ def run(args):
         L.init_seeds(rseed)
         if args.block:
             L.set_launch(args.block)
+        if args.fusion:
+            L.set_fusion(True)
+        L.set_graph(args.chunk)
         if args.warmup:
             L.sweeps(args.warmup, per_sweep=False)
         L.sync()
@@ -314,7 +317,7 @@ def run(args):
                    "naccept": int(nacc), "final_energy": energy},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": load_traffic(args.workload),
+                     "traffic": load_traffic(args.workload + ("-fused" if args.fusion else "")),
                      "kernel": "k_sweep_color<%s, %s, 4>" % (lattice, {1: "u8", 2: "u16", 4: "i32"}[spin_bytes(app, q)]),
                      "bytes_per_launch": bytes_per_launch,
                      "avg_launch_us": round(per_launch_s * 1e6, 3),
@@ -335,7 +338,8 @@ def main():
     ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=None, help="timed sweeps (default: the workload's count)")
-    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=64,
+                    help="untimed sweeps first (captures the sweep graphs)")
     ap.add_argument("--workload", default="cfg2", choices=sorted(WORKLOADS) + ["cfg4", "cfg5"])
     ap.add_argument("--serve-n", type=int, nargs=3, default=[150, 500, 1000], metavar=("N10", "N50", "N200"),
                     help="cfg5: requests per offered load")
@@ -345,6 +349,8 @@ def main():
     ap.add_argument("--block-sweeps", type=int, default=25, help="cfg4: sweeps per graph block")
     ap.add_argument("--grid", type=int, default=0, help="cfg4: persistent workgroups (with --block)")
     ap.add_argument("--block", type=int, default=0, help="sweep-kernel workgroup size (0 = default)")
+    ap.add_argument("--fusion", action="store_true", help="2d lattices: several sweeps per launch in LDS tiles")
+    ap.add_argument("--chunk", type=int, default=32, help="sweeps per replayed graph (0 = eager launches)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--cpu-threads", type=int, default=0)

@@ -151,6 +151,11 @@ int spk_hip_set_launch(spk_hip_lattice *h, int block);
 int spk_hip_set_fusion(spk_hip_lattice *h, int enable);
 int spk_hip_get_fusion(spk_hip_lattice *h, int *sweeps_per_launch);
 
+/* Sweeps run as replays of a captured HIP graph of chunk_sweeps sweeps
+   (default 32; a multiple of 8); 0 launches every kernel from the host.
+   Results are identical either way. */
+int spk_hip_set_graph(spk_hip_lattice *h, int chunk_sweeps);
+
 /* Device-side timing of the last spk_hip_sweeps call: milliseconds between
    HIP events recorded on the handle's stream around the kernel sequence,
    and the number of sweep-kernel launches it made. */

@@ -153,6 +153,15 @@ struct spk_hip_lattice {
   bool energy_valid = false;
   int64_t energy = 0;                  // total energy, exact integer
   std::vector<unsigned long long> hacc;
+  // graph replay of whole chunks of sweeps
+  int chunk = 32;                      // sweeps per captured chunk (0: eager launches only)
+  struct Chunk {
+    void *spin_start;                  // state buffer the chunk starts from
+    hipGraphExec_t exec;
+  };
+  std::vector<Chunk> graphs;
+  unsigned long long *pinned = nullptr;   // per-sweep rows of a call, host pinned
+  int pinned_rows = 0;
 };
 
 namespace {
@@ -192,7 +201,8 @@ void choose_fusion(spk_hip_lattice *h) {
   const int S = h->g.lat == SPK_HIP_SQ_4N ? 4 : 2;
   const int H = (h->g.lat == SPK_HIP_SQ_4N ? 2 : 4) * S;
   const size_t per = 4 + spin_bytes(h->stype);
-  const int tys[] = {64, 64, 64};
+  // cores sized so three workgroups' tiles fit one CU's 160 KB of LDS
+  const int tys[] = {56, 48, 40};
   const int txs[] = {128, 96, 64};
   const int TY = tys[h->stype], TX = txs[h->stype];
   const size_t lds = (size_t)(std::min(TX, h->g.nx) + 2 * H) * (std::min(TY, h->g.ny) + 2 * H) * per;
@@ -232,6 +242,80 @@ int launch_fused(spk_hip_lattice *h, unsigned long long *acc) {
   return SPK_HIP_OK;
 }
 
+void drop_graphs(spk_hip_lattice *h) {
+  for (auto &c : h->graphs) (void)hipGraphExecDestroy(c.exec);
+  h->graphs.clear();
+}
+
+__global__ void k_zero_u64(unsigned long long *__restrict__ v, int64_t n) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n) v[t] = 0ull;
+}
+
+int ensure_pinned(spk_hip_lattice *h, int rows) {
+  if (rows <= h->pinned_rows) return SPK_HIP_OK;
+  int cap = std::max(rows, 2 * h->pinned_rows);
+  unsigned long long *p = nullptr;
+  if (hipHostMalloc(&p, (size_t)cap * kAccSlots * 2 * sizeof(unsigned long long), hipHostMallocDefault) !=
+      hipSuccess)
+    return set_error(SPK_HIP_ENOMEM, "pinned host allocation failed");
+  if (h->pinned) {
+    (void)hipStreamSynchronize(h->stream);
+    (void)hipHostFree(h->pinned);
+  }
+  h->pinned = p;
+  h->pinned_rows = cap;
+  return SPK_HIP_OK;
+}
+
+// launches of sweeps whose accumulator rows start at acc; returns launches
+int launch_sweeps(spk_hip_lattice *h, int n, unsigned long long *acc, int64_t *nlaunch) {
+  int rc;
+  int s = 0;
+  if (h->fuse)   // whole groups of S sweeps through the fused kernel
+    for (; s + h->fuse <= n; s += h->fuse, (*nlaunch)++)
+      if ((rc = launch_fused(h, acc + (size_t)s * kAccSlots * 2))) return rc;
+  for (; s < n; s++)
+    for (int c = 0; c < h->g.K; c++, (*nlaunch)++)
+      if ((rc = launch_sweep(h, c, acc + (size_t)s * kAccSlots * 2))) return rc;
+  return SPK_HIP_OK;
+}
+
+// the graph of one chunk starting from the current state buffer: zero the
+// chunk's accumulator rows, then its sweeps.  A chunk holds an even number
+// of fused launches, so it ends in the buffer it started from.
+int chunk_graph(spk_hip_lattice *h, hipGraphExec_t *out) {
+  for (auto &c : h->graphs)
+    if (c.spin_start == h->spin) {
+      *out = c.exec;
+      return SPK_HIP_OK;
+    }
+  int rc;
+  hipGraph_t graph;
+  void *start = h->spin;
+  if ((rc = check_hip(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal), "begin capture")))
+    return rc;
+  const int64_t n = (int64_t)h->chunk * kAccSlots * 2;
+  hipLaunchKernelGGL(k_zero_u64, dim3(grid_for(n, 256)), dim3(256), 0, h->stream, h->acc, n);
+  int64_t nl = 0;
+  rc = check_hip(hipGetLastError(), "zero launch");
+  if (!rc) rc = launch_sweeps(h, h->chunk, h->acc, &nl);
+  hipError_t e = hipStreamEndCapture(h->stream, &graph);
+  if (rc) return rc;
+  if ((rc = check_hip(e, "end capture"))) return rc;
+  if (h->spin != start) {   // odd number of fused launches: not replayable as is
+    (void)hipGraphDestroy(graph);
+    return set_error(SPK_HIP_ESTATE, "chunk does not return to its start buffer");
+  }
+  hipGraphExec_t exec;
+  rc = check_hip(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0), "graph instantiate");
+  (void)hipGraphDestroy(graph);
+  if (rc) return rc;
+  h->graphs.push_back({start, exec});
+  *out = exec;
+  return SPK_HIP_OK;
+}
+
 int launch_energy(spk_hip_lattice *h, unsigned long long *acc) {
   with_lattice(h->g.lat, [&]<int LAT>() {
     with_type(h->stype, [&]<typename T>() {
@@ -261,6 +345,7 @@ int ensure_acc(spk_hip_lattice *h, int nsweeps) {
     (void)hipStreamSynchronize(h->stream);
     (void)hipFree(h->acc);
   }
+  drop_graphs(h);   // the accumulator address is baked into captured launches
   h->acc = p;
   h->acc_cap = cap;
   h->hacc.resize((size_t)cap * kAccSlots * 2);
@@ -347,12 +432,9 @@ int spk_hip_create(const spk_hip_desc *d, spk_hip_lattice **out) {
     spk_hip_destroy(h);
     return rc;
   }
-  choose_fusion(h);
   if (hipMalloc(&h->spin, (size_t)h->slots * spin_bytes(h->stype)) != hipSuccess ||
       hipMalloc(&h->seed, (size_t)h->slots * sizeof(int32_t)) != hipSuccess ||
-      hipMalloc(&h->stage, (size_t)h->N * sizeof(int32_t)) != hipSuccess ||
-      (h->fuse && (hipMalloc(&h->spin_alt, (size_t)h->slots * spin_bytes(h->stype)) != hipSuccess ||
-                   hipMalloc(&h->seed_alt, (size_t)h->slots * sizeof(int32_t)) != hipSuccess))) {
+      hipMalloc(&h->stage, (size_t)h->N * sizeof(int32_t)) != hipSuccess) {
     spk_hip_destroy(h);
     return set_error(SPK_HIP_ENOMEM, "device allocation of %lld sites failed", (long long)h->N);
   }
@@ -373,6 +455,8 @@ int spk_hip_create(const spk_hip_desc *d, spk_hip_lattice **out) {
 void spk_hip_destroy(spk_hip_lattice *h) {
   if (!h) return;
   if (h->stream) (void)hipStreamSynchronize(h->stream);
+  drop_graphs(h);
+  if (h->pinned) (void)hipHostFree(h->pinned);
   if (h->spin) (void)hipFree(h->spin);
   if (h->seed) (void)hipFree(h->seed);
   if (h->stage) (void)hipFree(h->stage);
@@ -392,6 +476,7 @@ int spk_hip_set_temperature(spk_hip_lattice *h, double temperature) {
   if (rc) return rc;
   h->rule = r;
   h->d.temperature = temperature;
+  drop_graphs(h);   // thresholds are kernel arguments of the captured launches
   return SPK_HIP_OK;
 }
 
@@ -504,27 +589,43 @@ int spk_hip_sweeps(spk_hip_lattice *h, int nsweeps, int64_t *naccept_total, int6
     if (h->energy_valid) e0 = h->energy;
     else if ((rc = compute_energy(h, &e0))) return rc;
   }
-  if ((rc = ensure_acc(h, nsweeps))) return rc;
+  // whole chunks replay a captured graph; the rest launch eagerly.  Chunk
+  // rows live in device rows [0, chunk), eager rows after them; every row
+  // reaches the pinned host copy in stream order.
+  const int C = h->chunk;
+  const int nchunks = C > 0 ? nsweeps / C : 0;
+  const int rem = nsweeps - nchunks * C;
+  if ((rc = ensure_acc(h, C + rem)) || (rc = ensure_pinned(h, nsweeps))) return rc;
   const size_t row = kAccSlots * 2 * sizeof(unsigned long long);
-  if ((rc = check_hip(hipMemsetAsync(h->acc, 0, row * nsweeps, h->stream), "memset"))) return rc;
+  unsigned long long *rem_acc = h->acc + (size_t)(nchunks ? C : 0) * kAccSlots * 2;
   if ((rc = check_hip(hipEventRecord(h->ev0, h->stream), "event"))) return rc;
   int64_t nlaunch = 0;
-  int s = 0;
-  if (h->fuse)   // whole groups of S sweeps through the fused kernel
-    for (; s + h->fuse <= nsweeps; s += h->fuse, nlaunch++)
-      if ((rc = launch_fused(h, h->acc + (size_t)s * kAccSlots * 2))) return rc;
-  for (; s < nsweeps; s++)
-    for (int c = 0; c < h->g.K; c++, nlaunch++)
-      if ((rc = launch_sweep(h, c, h->acc + (size_t)s * kAccSlots * 2))) return rc;
+  for (int k = 0; k < nchunks; k++) {
+    hipGraphExec_t exec;
+    if ((rc = chunk_graph(h, &exec))) return rc;
+    if ((rc = check_hip(hipGraphLaunch(exec, h->stream), "graph launch"))) return rc;
+    if ((rc = check_hip(hipMemcpyAsync(h->pinned + (size_t)k * C * kAccSlots * 2, h->acc, row * C,
+                                       hipMemcpyDeviceToHost, h->stream),
+                        "accumulator readback")))
+      return rc;
+  }
+  if (nchunks) nlaunch += (int64_t)nchunks * (h->fuse ? C / h->fuse : (int64_t)C * h->g.K);
+  if (rem) {
+    if ((rc = check_hip(hipMemsetAsync(rem_acc, 0, row * rem, h->stream), "memset"))) return rc;
+    if ((rc = launch_sweeps(h, rem, rem_acc, &nlaunch))) return rc;
+  }
   if ((rc = check_hip(hipEventRecord(h->ev1, h->stream), "event"))) return rc;
-  if ((rc = check_hip(hipMemcpyAsync(h->hacc.data(), h->acc, row * nsweeps, hipMemcpyDeviceToHost, h->stream),
-                      "accumulator readback")))
+  if (rem && (rc = check_hip(hipMemcpyAsync(h->pinned + (size_t)nchunks * C * kAccSlots * 2, rem_acc, row * rem,
+                                            hipMemcpyDeviceToHost, h->stream),
+                             "accumulator readback")))
     return rc;
   if ((rc = check_hip(hipStreamSynchronize(h->stream), "sweeps sync"))) return rc;
   float ms = 0.f;
   if ((rc = check_hip(hipEventElapsedTime(&ms, h->ev0, h->ev1), "event time"))) return rc;
   h->last_ms = ms;
   h->last_nlaunch = nlaunch;
+  if (h->hacc.size() < (size_t)nsweeps * kAccSlots * 2) h->hacc.resize((size_t)nsweeps * kAccSlots * 2);
+  std::memcpy(h->hacc.data(), h->pinned, row * nsweeps);
 
   int64_t nacc = 0, eng = e0;
   const bool track = h->energy_valid || energy_per_sweep;
@@ -580,11 +681,13 @@ int spk_hip_set_launch(spk_hip_lattice *h, int block) {
   if (block < 64 || block > 1024 || block % 64)
     return set_error(SPK_HIP_EINVAL, "block must be a multiple of 64 in [64, 1024]");
   h->block = block;
+  drop_graphs(h);
   return SPK_HIP_OK;
 }
 
 int spk_hip_set_fusion(spk_hip_lattice *h, int enable) {
   if (!h) return set_error(SPK_HIP_EINVAL, "null handle");
+  drop_graphs(h);
   if (!enable) {
     h->fuse = 0;
     return SPK_HIP_OK;
@@ -599,6 +702,16 @@ int spk_hip_set_fusion(spk_hip_lattice *h, int enable) {
     }
   }
   choose_fusion(h);
+  return SPK_HIP_OK;
+}
+
+int spk_hip_set_graph(spk_hip_lattice *h, int chunk_sweeps) {
+  if (!h) return set_error(SPK_HIP_EINVAL, "null handle");
+  if (chunk_sweeps < 0) return set_error(SPK_HIP_EINVAL, "chunk_sweeps < 0");
+  // an even number of fused launches per chunk (see chunk_graph)
+  if (chunk_sweeps && chunk_sweeps % 8) return set_error(SPK_HIP_EINVAL, "chunk_sweeps must be a multiple of 8");
+  drop_graphs(h);
+  h->chunk = chunk_sweeps;
   return SPK_HIP_OK;
 }
 

@@ -162,6 +162,33 @@ def test_fused_sweeps_equal_oracle_and_unfused(orc, case):
         assert np.array_equal(gs, spin1) and np.array_equal(gk, seeds1)
 
 
+@pytest.mark.parametrize("case", [("ising", "sq/4n", (128, 96, 1), 2, 2.269, 21),
+                                  ("potts", "sc/26n", (16, 12, 10), 10000, 0.5, 22),
+                                  ("potts", "sq/8n", (66, 34, 1), 70000, 0.7, 23)],
+                         ids=["ising-fused", "potts-sc26n", "potts-sq8n-i32"])
+def test_graph_replays_equal_eager_launches(case):
+    """Chunks of sweeps replayed from captured graphs, several replays per
+    call and several calls with eager work and synchronisation in between
+    (energy, download, temperature change), against eager launches."""
+    import spparks_amd as S
+    app, lat, shape, q, kt, seed = case
+    runs = []
+    for chunk in (32, 0):
+        with gpu_init(S, app, lat, shape, q, kt, seed) as L:
+            L.set_graph(chunk)
+            out = []
+            for n in (70, 33, 64):
+                _, gn, ge = L.sweeps(n)
+                out.append((gn, ge, L.energy()) + L.download())
+            L.set_temperature(kt * 0.5)
+            _, gn, ge = L.sweeps(40)
+            out.append((gn, ge, L.energy()) + L.download())
+            runs.append(out)
+    for a, b in zip(*runs):
+        for x, y in zip(a, b):
+            assert np.array_equal(np.asarray(x), np.asarray(y))
+
+
 def test_split_calls_equal_one_call():
     """Consecutive runs continue exactly (seeds persist between runs)."""
     import spparks_amd as S
