@@ -81,17 +81,14 @@ __global__ __launch_bounds__(1024) void k_batch_step(char *__restrict__ pool, co
     Counts cnt{0, 0};
     if (ti < (int64_t)(s_rep.g.Mp / kVec)) {
       const Geom &g = s_rep.g;
-      const int per_row = g.px / kVec;
-      const int tt = (int)ti;
-      const int rest = tt / per_row;
+      int a0, b, e;
+      pack_site(g, (uint32_t)ti, a0, b, e);
       T *sp = reinterpret_cast<T *>(pool + s_rep.spin_off);
       int32_t *sd = reinterpret_cast<int32_t *>(pool + s_rep.seed_off);
       if (s_rep.rule.nspins == 2)
-        cnt = sweep_sites<LAT, T, kVec, true>(sp, sd, g, s_rep.rule, s_thr, p % K, (tt % per_row) * kVec,
-                                              rest % g.hy, rest / g.hy);
+        cnt = sweep_sites<LAT, T, kVec, true>(sp, sd, g, s_rep.rule, s_thr, p % K, a0, b, e);
       else
-        cnt = sweep_sites<LAT, T, kVec, false>(sp, sd, g, s_rep.rule, s_thr, p % K, (tt % per_row) * kVec,
-                                               rest % g.hy, rest / g.hy);
+        cnt = sweep_sites<LAT, T, kVec, false>(sp, sd, g, s_rep.rule, s_thr, p % K, a0, b, e);
     }
     block_accumulate(cnt.nacc, cnt.de, acc + 2 * (int64_t)(s_rep.acc_row + p / K), 0);
   }

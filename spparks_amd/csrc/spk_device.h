@@ -50,6 +50,37 @@ __host__ __device__ __forceinline__ uint32_t pm_mulmod(uint32_t a, uint32_t b) {
 //   sc/26n: hx=nx/2 hy=ny/2 hz=nz/2 c = 4(k&1)+2(j&1)+(i&1), site (2a+pi, 2b+pj, 2e+pk)
 // Spins are stored as uint8 / uint16 / int32 (the narrowest type holding
 // 1..Q); seeds as int32.
+// Exact unsigned division by a fixed d with one multiply-high: m = 2^32/d
+// (2^32-1 for d = 1) gives a quotient that is exact or one too small, and a
+// single correction makes it exact for every 32-bit dividend.
+struct FastDiv {
+  uint32_t d, m;
+};
+
+__host__ __device__ inline FastDiv fastdiv_make(uint32_t d) {
+  FastDiv f;
+  f.d = d;
+  f.m = d <= 1 ? 0xffffffffu : (uint32_t)(0x100000000ull / d);
+  return f;
+}
+
+__host__ __device__ __forceinline__ uint32_t mulhi32(uint32_t a, uint32_t b) {
+#ifdef __HIP_DEVICE_COMPILE__
+  return __umulhi(a, b);
+#else
+  return (uint32_t)(((uint64_t)a * b) >> 32);
+#endif
+}
+
+__host__ __device__ __forceinline__ void fastdiv(const FastDiv &f, uint32_t t, uint32_t &q, uint32_t &r) {
+  q = mulhi32(t, f.m);
+  r = t - q * f.d;
+  if (r >= f.d) {
+    q++;
+    r -= f.d;
+  }
+}
+
 struct Geom {
   int lat;
   int nx, ny, nz;
@@ -58,6 +89,8 @@ struct Geom {
   int M;       // sites per colour (hx*hy*hz)
   int px;      // row pitch: hx rounded up to a multiple of 4
   int Mp;      // stored slots per colour (px*hy*hz)
+  FastDiv row4;  // division by px/4: 4-site packs per row
+  FastDiv rows;  // division by hy
 };
 
 // Fills the colour sub-grid of a lattice style; edges must be even.
@@ -75,6 +108,19 @@ __host__ __device__ inline void geom_init(Geom &g, int lat, int nx, int ny, int 
   g.M = g.hx * g.hy * g.hz;
   g.px = (g.hx + 3) & ~3;
   g.Mp = g.px * g.hy * g.hz;
+  g.row4 = fastdiv_make((uint32_t)(g.px / 4));
+  g.rows = fastdiv_make((uint32_t)g.hy);
+}
+
+// thread t of a colour pass -> its first site a0 and sub-grid row (b, e):
+// px/4 threads per row, 4 consecutive sites each
+__host__ __device__ __forceinline__ void pack_site(const Geom &g, uint32_t t, int &a0, int &b, int &e) {
+  uint32_t row, a4, zz, yy;
+  fastdiv(g.row4, t, row, a4);
+  fastdiv(g.rows, row, zz, yy);
+  a0 = (int)a4 * 4;
+  b = (int)yy;
+  e = (int)zz;
 }
 
 __host__ __device__ inline int maxneigh_of(int lat) {
@@ -552,32 +598,35 @@ __host__ __device__ __forceinline__ bool fused_in_core(const FusedTile &t, int l
 }
 
 // --------------------------------------------------------- reductions
-__device__ __forceinline__ int wave_sum(int v) {
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
-  return v;
-}
-__device__ __forceinline__ long long wave_sum64(long long v) {
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+__device__ __forceinline__ int wave_sum(int v) { return __ockl_wfred_add_i32(v); }
+// 64-bit wrapping sum (mod 2^64) for checksums; not on a hot path
+__device__ __forceinline__ unsigned long long wave_sum64(unsigned long long v) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
   return v;
 }
 
 // Adds (x, y) of every thread of the block into acc[2*slot], acc[2*slot+1]:
-// a wavefront shuffle sum, one LDS slot per wave, one atomic per block.
-// All threads of the block must call it.  Needs blockDim.x % 64 == 0.
-__device__ __forceinline__ void block_accumulate(long long x, long long y,
-                                                 unsigned long long *acc, int slot) {
-  __shared__ long long red[2][16];
-  x = wave_sum64(x);
-  y = wave_sum64(y);
+// a wavefront sum (DPP, 32-bit: per-wave totals stay far below 2^31), one
+// LDS slot per wave, one atomic per block and value.  All threads of the
+// block must call it; blockDim.x % 64 == 0.  Callers that reduce again put
+// a barrier between consecutive calls (the LDS slots are reused).
+__device__ __forceinline__ void block_accumulate(int x, int y, unsigned long long *acc, int slot) {
+  __shared__ int red[2][16];
+  x = wave_sum(x);
+  y = wave_sum(y);
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-  if (lane == 0) { red[0][wid] = x; red[1][wid] = y; }
+  if (lane == 0) {
+    red[0][wid] = x;
+    red[1][wid] = y;
+  }
   __syncthreads();
   if (threadIdx.x == 0) {
     long long sx = 0, sy = 0;
     const int nw = blockDim.x >> 6;
-    for (int w = 0; w < nw; w++) { sx += red[0][w]; sy += red[1][w]; }
+    for (int w = 0; w < nw; w++) {
+      sx += red[0][w];
+      sy += red[1][w];
+    }
     if (sx) atomicAdd(&acc[2 * slot], (unsigned long long)sx);
     if (sy) atomicAdd(&acc[2 * slot + 1], (unsigned long long)sy);
   }

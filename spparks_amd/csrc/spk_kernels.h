@@ -135,7 +135,7 @@ __global__ void k_checksum(const T *__restrict__ spin, const int32_t *__restrict
     h = mix64((uint64_t)t * 0xD1B54A32D192ED03ull + (uint64_t)(uint32_t)(int32_t)spin[p] * 0xABC98388FB8FAC03ull +
               (uint64_t)(uint32_t)seed[p]);
   }
-  h = (uint64_t)wave_sum64((long long)h);
+  h = wave_sum64(h);
   if ((threadIdx.x & 63) == 0) atomicAdd(out, (unsigned long long)h);
 }
 

@@ -33,15 +33,14 @@ __global__ __launch_bounds__(1024) void k_sweep_color(T *__restrict__ spin, int3
   if (threadIdx.x < 28) s_thr[threadIdx.x] = r.thr[threadIdx.x];
   __syncthreads();
 
+  static_assert(V == 4, "pack_site maps 4 sites per thread");
   const int tile = xcd_tile(blockIdx.x, gridDim.x);
   const int64_t t = (int64_t)tile * blockDim.x + threadIdx.x;
-  const int per_row = g.px / V;
   Counts cnt{0, 0};
   if (t < (int64_t)(g.Mp / V)) {
-    const int ti = (int)t;
-    const int a0 = (ti % per_row) * V;
-    const int rest = ti / per_row;
-    cnt = sweep_sites<LAT, T, V, TWO>(spin, seed, g, r, s_thr, color, a0, rest % g.hy, rest / g.hy);
+    int a0, b, e;
+    pack_site(g, (uint32_t)t, a0, b, e);
+    cnt = sweep_sites<LAT, T, V, TWO>(spin, seed, g, r, s_thr, color, a0, b, e);
   }
   block_accumulate(cnt.nacc, cnt.de, acc, blockIdx.x & (kAccSlots - 1));
 }

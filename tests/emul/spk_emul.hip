@@ -122,7 +122,6 @@ static int run(int app, int nx, int ny, int nz, int q, double kt, int nsw, const
   }
   std::vector<long long> nacc(nsw), eng(nsw);
   long long na = 0, e = energy<LAT, T>(s, g);
-  const int per_row = g.px / V;
   int w = 0;
   const char *fz = getenv("SPK_EMUL_FUSE");
   if (fz && (LAT == SPK_HIP_SQ_4N || LAT == SPK_HIP_SQ_8N)) {
@@ -150,10 +149,15 @@ static int run(int app, int nx, int ny, int nz, int q, double kt, int nsw, const
     long long de = 0;
     for (int c = 0; c < g.K; c++)
       for (int t = 0; t < g.Mp / V; t++) {
-        int a0 = (t % per_row) * V, rest = t / per_row;
-        Counts cnt = r.nspins == 2
-                         ? sweep_sites<LAT, T, V, true>(s.data(), k.data(), g, r, r.thr, c, a0, rest % g.hy, rest / g.hy)
-                         : sweep_sites<LAT, T, V, false>(s.data(), k.data(), g, r, r.thr, c, a0, rest % g.hy, rest / g.hy);
+        int a0, b, e;
+        pack_site(g, (uint32_t)t, a0, b, e);
+        // the fast division must agree with the plain one
+        if (a0 != (t % (g.px / V)) * V || b != (t / (g.px / V)) % g.hy || e != (t / (g.px / V)) / g.hy) {
+          fprintf(stderr, "pack_site disagrees with / and %%\n");
+          return 10;
+        }
+        Counts cnt = r.nspins == 2 ? sweep_sites<LAT, T, V, true>(s.data(), k.data(), g, r, r.thr, c, a0, b, e)
+                                   : sweep_sites<LAT, T, V, false>(s.data(), k.data(), g, r, r.thr, c, a0, b, e);
         na += cnt.nacc;
         de += cnt.de;
       }
