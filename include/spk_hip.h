@@ -194,7 +194,8 @@ int spk_hip_replica_bytes(const spk_hip_desc *d, int64_t *bytes);
 /* Admit a replica: allocate it in the pool and build its initial state on
    the device (set range / value, strict seeds) exactly as its script would
    (seed, set, first run's draws).  SPK_HIP_ENOMEM when the pool has no room
-   or all replica slots are taken; the handle is unchanged then. */
+   or all replica slots are taken; SPK_HIP_EINVAL for a replica of 2^30 or
+   more sites (run such a lattice alone).  The handle is unchanged then. */
 int spk_hip_batch_add(spk_hip_batch *b, const spk_hip_replica_desc *r, int *id);
 /* Retire a replica and return its pool space. */
 int spk_hip_batch_remove(spk_hip_batch *b, int id);

@@ -83,12 +83,15 @@ __global__ __launch_bounds__(1024) void k_batch_step(char *__restrict__ pool, co
       const Geom &g = s_rep.g;
       int a0, b, e;
       pack_site(g, (uint32_t)ti, a0, b, e);
-      T *sp = reinterpret_cast<T *>(pool + s_rep.spin_off);
-      int32_t *sd = reinterpret_cast<int32_t *>(pool + s_rep.seed_off);
-      if (s_rep.rule.nspins == 2)
-        cnt = sweep_sites<LAT, T, kVec, true>(sp, sd, g, s_rep.rule, s_thr, p % K, a0, b, e);
+      // the replica's arrays, as workgroup-uniform (scalar) bases
+      T *sp = reinterpret_cast<T *>(uniform_ptr(pool + s_rep.spin_off));
+      int32_t *sd = reinterpret_cast<int32_t *>(uniform_ptr(pool + s_rep.seed_off));
+      bool two = false;
+      if constexpr (sizeof(T) == 1) two = s_rep.rule.nspins == 2;   // two states are stored as u8
+      if (two)
+        cnt = sweep_sites<LAT, T, kVec, true, uint32_t>(sp, sd, g, s_rep.rule, s_thr, p % K, a0, b, e);
       else
-        cnt = sweep_sites<LAT, T, kVec, false>(sp, sd, g, s_rep.rule, s_thr, p % K, a0, b, e);
+        cnt = sweep_sites<LAT, T, kVec, false, uint32_t>(sp, sd, g, s_rep.rule, s_thr, p % K, a0, b, e);
     }
     block_accumulate(cnt.nacc, cnt.de, acc + 2 * (int64_t)(s_rep.acc_row + p / K), 0);
   }
@@ -503,6 +506,9 @@ int spk_hip_batch_add(spk_hip_batch *b, const spk_hip_replica_desc *r, int *id) 
       break;
     }
   if (slot < 0) return set_error(SPK_HIP_ENOMEM, "all %d replica slots are in use", (int)b->reps.size());
+  if (!offsets_fit_32(R.g))
+    return set_error(SPK_HIP_EINVAL, "a batched replica holds fewer than 2^30 sites (got %lld); run it alone",
+                     (long long)R.g.K * R.g.M);
   R.stype = spin_type_for(r->lattice.app, r->lattice.nspins);
   R.variant = R.g.lat * 3 + R.stype;
   const int64_t slots = (int64_t)R.g.K * R.g.Mp;

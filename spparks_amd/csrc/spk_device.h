@@ -13,6 +13,16 @@
 
 namespace spk {
 
+// A pointer whose value is the same in every lane of the wavefront, moved
+// to scalar registers so loads from it take the scalar-base form.
+template <typename T>
+__device__ __forceinline__ T *uniform_ptr(T *p) {
+  const uint64_t v = reinterpret_cast<uint64_t>(p);
+  const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)v);
+  const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(v >> 32));
+  return reinterpret_cast<T *>(((uint64_t)hi << 32) | lo);
+}
+
 // ---------------------------------------------------------------- RNG
 // Park-Miller minimal standard: s' = 16807*s mod (2^31-1).  The reference
 // evaluates it with Schrage's decomposition (src/random_park.cpp:79-86); both
@@ -292,112 +302,307 @@ __host__ __device__ __forceinline__ SiteOutcome site_event(const Rule &r, const 
 }
 
 // ------------------------------------------------ vectorised colour pass
-// V consecutive sites of one sub-grid row per thread; px % V == 0, so packs
-// never straddle a row and stay V-aligned.  The last pack of a row may run
-// into the row's pad slots: those lanes are computed on pad data and
-// discarded.
-template <typename T, int V>
-struct alignas(sizeof(T) * V) Pack {
-  T v[V];
+// Each thread owns 4 consecutive sites of one sub-grid row (px % 4 == 0, so
+// a thread's sites never straddle a row).  Their spins, and each of their
+// neighbour values, are handled as 4 lanes packed into 32-bit words: one
+// word of bytes (u8 storage), two words of halfwords (u16) or four words
+// (i32).  Neighbours in y and z sit at the same lane positions of another
+// row; the two x-neighbours of a lane are that lane and the one beside it
+// in the other x-parity's row, so they come from a word-wide funnel shift.
+// The last thread of a row may own pad lanes (a0+v >= hx): they are
+// computed on pad data and discarded, and their junk cannot reach a real
+// lane (see Tally).
+
+// True when every element offset of the lattice's spin and seed arrays
+// (K*Mp elements, at most 4 bytes each) fits a 32-bit byte offset.
+__host__ __device__ inline bool offsets_fit_32(const Geom &g) {
+  return (uint64_t)g.K * (uint64_t)g.Mp * 4u < (1ull << 32);
+}
+
+// 32-bit or 64-bit element offsets.  With 32-bit offsets the byte offset is
+// formed in 32 bits, so loads take the (scalar base + 32-bit offset) form;
+// valid while the array holds fewer than 2^32 bytes.
+template <typename I, typename T>
+__host__ __device__ __forceinline__ T *elem(T *base, I idx) {
+  if constexpr (sizeof(I) == 4)
+    return reinterpret_cast<T *>(reinterpret_cast<char *>(base) + (uint32_t)(idx * (uint32_t)sizeof(T)));
+  else
+    return base + idx;
+}
+template <typename I, typename T>
+__host__ __device__ __forceinline__ const T *elem(const T *base, I idx) {
+  if constexpr (sizeof(I) == 4)
+    return reinterpret_cast<const T *>(reinterpret_cast<const char *>(base) +
+                                       (uint32_t)(idx * (uint32_t)sizeof(T)));
+  else
+    return base + idx;
+}
+
+template <int NW>
+struct alignas(4 * NW) Words {
+  uint32_t w[NW];
 };
 
-template <typename T, int V>
-__host__ __device__ __forceinline__ void load_pack(const T *p, int32_t (&out)[V]) {
-  const Pack<T, V> pk = *reinterpret_cast<const Pack<T, V> *>(p);
+// 4 lanes of storage type T: sizeof(T) words, lane v in bits
+// [8*sizeof(T)*(v % (4/sizeof(T))), ...) of word v / (4/sizeof(T)).
+template <typename T>
+struct Lanes {
+  static constexpr int NW = (int)sizeof(T);
+  static constexpr int PER = 4 / NW;           // lanes per word
+  static constexpr int BITS = 8 * NW;          // bits per lane
+  static constexpr uint32_t MASK = NW == 4 ? 0xffffffffu : (1u << BITS) - 1;
+  uint32_t w[NW];
+
+  __host__ __device__ __forceinline__ int32_t get(int v) const {
+    if constexpr (NW == 4) return (int32_t)w[v];
+    else return (int32_t)((w[v / PER] >> (BITS * (v % PER))) & MASK);
+  }
+  __host__ __device__ __forceinline__ void set(int v, uint32_t x) {
+    if constexpr (NW == 4) w[v] = x;
+    else {
+      const int sh = BITS * (v % PER);
+      w[v / PER] = (w[v / PER] & ~(MASK << sh)) | ((x & MASK) << sh);
+    }
+  }
+  __host__ __device__ __forceinline__ static Lanes pack(const int32_t (&x)[4]) {
+    Lanes l;
+    if constexpr (NW == 1) l.w[0] = (uint32_t)x[0] | (uint32_t)x[1] << 8 | (uint32_t)x[2] << 16 | (uint32_t)x[3] << 24;
+    else if constexpr (NW == 2) {
+      l.w[0] = (uint32_t)x[0] | (uint32_t)x[1] << 16;
+      l.w[1] = (uint32_t)x[2] | (uint32_t)x[3] << 16;
+    } else
+      for (int v = 0; v < 4; v++) l.w[v] = (uint32_t)x[v];
+    return l;
+  }
+};
+
+template <typename T, typename I>
+__host__ __device__ __forceinline__ Lanes<T> load_lanes(const T *base, I idx) {
+  const Words<Lanes<T>::NW> x = *reinterpret_cast<const Words<Lanes<T>::NW> *>(elem<I>(base, idx));
+  Lanes<T> l;
 #pragma unroll
-  for (int v = 0; v < V; v++) out[v] = (int32_t)pk.v[v];
+  for (int i = 0; i < Lanes<T>::NW; i++) l.w[i] = x.w[i];
+  return l;
 }
 
-// Row segment of V+1 values of the other x-parity: positions
-// a0+off-1 .. a0+off+V-1 (off = 0 or 1) with periodic wrap at hx.  Site v of
-// the thread (a0+v < hx) has its two x-neighbours at seg[v] and seg[v+1].
-template <typename T, int V>
-__host__ __device__ __forceinline__ void load_segment(const T *row, int a0, int hx, int off,
-                                                      int32_t (&seg)[V + 1]) {
-  int32_t pk[V];
-  load_pack<T, V>(row + a0, pk);
-  // the one element outside the pack: a0-1 (off 0) or a0+V (off 1)
-  const int xe = off ? (a0 + V >= hx ? 0 : a0 + V) : (a0 == 0 ? hx - 1 : a0 - 1);
-  const int32_t extra = (int32_t)row[xe];
+template <typename T, typename I>
+__host__ __device__ __forceinline__ void store_lanes(T *base, I idx, const Lanes<T> &l) {
+  Words<Lanes<T>::NW> x;
 #pragma unroll
-  for (int v = 0; v <= V; v++) {
-    const int32_t lo = v == 0 ? extra : pk[v - 1];     // off == 0
-    // off == 1: position a0+v, which wraps to 0 at a0+v == hx (the last
-    // site's right neighbour); pad positions beyond hx feed only pad lanes
-    const int32_t hi = (v == V || a0 + v == hx) ? extra : pk[v];
-    seg[v] = off ? hi : lo;
-  }
+  for (int i = 0; i < Lanes<T>::NW; i++) x.w[i] = l.w[i];
+  *reinterpret_cast<Words<Lanes<T>::NW> *>(elem<I>(base, idx)) = x;
 }
+
+// Lanes of positions p-1 (off 0) or p+1 (off 1), p = a0+v, from the lanes P
+// of positions a0..a0+3 and the element E one beyond them on that side (the
+// periodic image: a0-1 or hx-1; a0+4 or 0).
+template <typename T>
+__host__ __device__ __forceinline__ Lanes<T> shifted(const Lanes<T> &P, uint32_t E, int off) {
+  Lanes<T> S;
+  if constexpr (Lanes<T>::NW == 1) {
+    S.w[0] = off ? (P.w[0] >> 8) | (E << 24) : (P.w[0] << 8) | E;
+  } else if constexpr (Lanes<T>::NW == 2) {
+    const uint32_t mid = (P.w[0] >> 16) | (P.w[1] << 16);   // lanes 1, 2
+    S.w[0] = off ? mid : (P.w[0] << 16) | E;
+    S.w[1] = off ? (P.w[1] >> 16) | (E << 16) : mid;
+  } else {
+#pragma unroll
+    for (int v = 0; v < 4; v++) {
+      const int src = off ? v + 1 : v - 1;
+      S.w[v] = (src < 0 || src > 3) ? E : P.w[src];
+    }
+  }
+  return S;
+}
+
+// The thread whose row ends inside its 4 lanes (k = hx-a0 < 4 real lanes;
+// only when hx % 4 != 0): for off 1, lane k-1 must read position hx, i.e.
+// the wrap to position 0, which E holds.  keep[i] marks the lanes of word i
+// below k-1; the others take E (lanes from k on are pad lanes anyway).
+template <typename T>
+struct RowEnd {
+  uint32_t keep[Lanes<T>::NW];
+  __host__ __device__ __forceinline__ explicit RowEnd(int k) {
+#pragma unroll
+    for (int i = 0; i < Lanes<T>::NW; i++) {
+      const int n = min(max(k - 1 - i * Lanes<T>::PER, 0), Lanes<T>::PER);
+      keep[i] = n == Lanes<T>::PER ? 0xffffffffu : (1u << (n * Lanes<T>::BITS)) - 1u;
+    }
+  }
+  __host__ __device__ __forceinline__ void apply(Lanes<T> &S, uint32_t E) const {
+    uint32_t Ew = E;
+    if constexpr (Lanes<T>::NW == 1) Ew = E * 0x01010101u;
+    else if constexpr (Lanes<T>::NW == 2) Ew = E | (E << 16);
+#pragma unroll
+    for (int i = 0; i < Lanes<T>::NW; i++) S.w[i] = (S.w[i] & keep[i]) | (Ew & ~keep[i]);
+  }
+};
+
+// Keeps the compiler's instruction scheduler from moving code across this
+// point (no instruction is emitted).
+__host__ __device__ __forceinline__ void fence_schedule() {
+#ifdef __HIP_DEVICE_COMPILE__
+  __builtin_amdgcn_sched_barrier(0);
+#endif
+}
+
+// acc += per-halfword "differs" flags of d (1 where the halfword of d is
+// nonzero), and of d and e for the two-word form.  The sum is formed in the
+// same asm statement, so the flags die at once instead of being gathered
+// into one late sum tree that keeps every one of them live.
+__host__ __device__ __forceinline__ void add_ne_halves(uint32_t &acc, uint32_t d) {
+#ifdef __HIP_DEVICE_COMPILE__
+  uint32_t t;
+  asm("v_pk_min_u16 %1, %2, 1\n\tv_add_u32 %0, %0, %1" : "+v"(acc), "=&v"(t) : "v"(d));
+#else
+  acc += ((d & 0xffffu) ? 1u : 0u) | ((d >> 16) ? 0x10000u : 0u);
+#endif
+}
+
+__host__ __device__ __forceinline__ void add_ne_halves2(uint32_t &acc, uint32_t d, uint32_t e) {
+#ifdef __HIP_DEVICE_COMPILE__
+  uint32_t t, u;
+  asm("v_pk_min_u16 %1, %3, 1\n\tv_pk_min_u16 %2, %4, 1\n\tv_add3_u32 %0, %0, %1, %2"
+      : "+v"(acc), "=&v"(t), "=&v"(u)
+      : "v"(d), "v"(e));
+#else
+  add_ne_halves(acc, d);
+  add_ne_halves(acc, e);
+#endif
+}
+
+__host__ __device__ __forceinline__ uint32_t ne_bytes(uint32_t d) {
+  // bit 7 of each byte of ((d & 0x7f..) + 0x7f..) | d is set iff the byte is
+  // nonzero; no carry crosses a byte (each byte sum is at most 0xfe)
+  return ((((d & 0x7f7f7f7fu) + 0x7f7f7f7fu) | d) >> 7) & 0x01010101u;
+}
+
+// Per-lane neighbour counts: ei = #neighbours unlike the old spin, ef =
+// #unlike the proposal.  Lane flags of 0/1 are summed by plain 32-bit adds:
+// a lane count never exceeds 26, so no byte or halfword field carries.  TWO (two states, values 1 and 2):
+// x ^ old is 0 or 3 per byte, so one word sum carries 3*ei per byte, and ef
+// follows from ei.  Pad lanes may hold other values; their sums can carry
+// only into higher lanes, which are pad lanes as well.
+template <typename T, bool TWO>
+struct Tally {
+  static constexpr bool kXorSum = TWO && sizeof(T) == 1;
+  static constexpr int NW = Lanes<T>::NW;
+  uint32_t o[NW], p[NW];
+
+  __host__ __device__ __forceinline__ Tally() {
+#pragma unroll
+    for (int i = 0; i < NW; i++) o[i] = p[i] = 0;
+  }
+  __host__ __device__ __forceinline__ void add(const Lanes<T> &x, const Lanes<T> &old, const Lanes<T> &prop) {
+    if constexpr (kXorSum) {
+      o[0] += x.w[0] ^ old.w[0];
+    } else if constexpr (NW == 1) {
+      o[0] += ne_bytes(x.w[0] ^ old.w[0]);
+      if constexpr (!TWO) p[0] += ne_bytes(x.w[0] ^ prop.w[0]);
+    } else if constexpr (NW == 2) {
+#pragma unroll
+      for (int i = 0; i < 2; i++) {
+        add_ne_halves(o[i], x.w[i] ^ old.w[i]);
+        if constexpr (!TWO) add_ne_halves(p[i], x.w[i] ^ prop.w[i]);
+      }
+    } else {
+#pragma unroll
+      for (int i = 0; i < 4; i++) {
+        o[i] += x.w[i] != old.w[i];
+        if constexpr (!TWO) p[i] += x.w[i] != prop.w[i];
+      }
+    }
+  }
+  // two neighbour words at once (the two x-neighbours of a segment)
+  __host__ __device__ __forceinline__ void add2(const Lanes<T> &x, const Lanes<T> &y, const Lanes<T> &old,
+                                                const Lanes<T> &prop) {
+    if constexpr (NW == 2 && !kXorSum) {
+#pragma unroll
+      for (int i = 0; i < 2; i++) {
+        add_ne_halves2(o[i], x.w[i] ^ old.w[i], y.w[i] ^ old.w[i]);
+        if constexpr (!TWO) add_ne_halves2(p[i], x.w[i] ^ prop.w[i], y.w[i] ^ prop.w[i]);
+      }
+    } else {
+      add(x, old, prop);
+      add(y, old, prop);
+    }
+  }
+  __host__ __device__ __forceinline__ int ei(int v) const {
+    if constexpr (kXorSum) return (int)((((o[0] >> (8 * v)) & 0xffu) * 171u) >> 9);   // byte / 3
+    else if constexpr (NW == 4) return (int)o[v];
+    else return (int)((o[v / Lanes<T>::PER] >> (Lanes<T>::BITS * (v % Lanes<T>::PER))) & Lanes<T>::MASK);
+  }
+  __host__ __device__ __forceinline__ int ef(int v) const {
+    if constexpr (NW == 4) return (int)p[v];
+    else return (int)((p[v / Lanes<T>::PER] >> (Lanes<T>::BITS * (v % Lanes<T>::PER))) & Lanes<T>::MASK);
+  }
+};
 
 struct Counts {
   int nacc;   // accepted flips
   int de;     // sum of (ef - ei) over accepted flips
 };
 
-// Strict events for sites a0..a0+V-1 of sub-grid row (b, e) of colour c.
-// The first draw and the proposal come first, so the neighbour values are
-// streamed into two running counts (unlike the old / the proposed spin).
-// TWO: the lattice has two states (Ising, or Potts Q=2), so every neighbour
-// equals either the old spin or the proposal and ef follows from ei.
-template <int LAT, typename T, int V, bool TWO>
+// Strict events for sites a0..a0+3 of sub-grid row (b, e) of colour c.  The
+// first draws and the proposals come first, so every neighbour word is
+// compared with the old and the proposed lanes as it arrives.  TWO: the
+// lattice has two states (Ising, or Potts Q=2), so every neighbour equals
+// either the old spin or the proposal and ef follows from ei.  I: element
+// offset type (uint32_t while the seed array holds fewer than 2^32 bytes).
+template <int LAT, typename T, int V, bool TWO, typename I = int64_t>
 __host__ __device__ __forceinline__ Counts sweep_sites(T *__restrict__ spin, int32_t *__restrict__ seed,
                                                        const Geom &g, const Rule &r, const int *thr,
                                                        int c, int a0, int b, int e) {
-  const int64_t self = (int64_t)c * g.Mp + a0 + (int64_t)g.px * (b + (int64_t)g.hy * e);
-  int32_t old[V], s[V], prop[V], ei[V], ef[V];
-  load_pack<T, V>(spin + self, old);
+  static_assert(V == 4, "4 sites per thread");
+  const I px = (I)g.px, pxy = (I)g.px * (I)g.hy, Mp = (I)g.Mp;
+  const I self = (I)c * Mp + (I)a0 + px * (I)b + pxy * (I)e;
+  const Lanes<T> old = load_lanes<T, I>(spin, self);
+  int32_t s[4], pr[4];
   {
-    const Pack<int32_t, V> sp = *reinterpret_cast<const Pack<int32_t, V> *>(seed + self);
+    const Words<4> sd = *reinterpret_cast<const Words<4> *>(elem<I>(seed, self));
 #pragma unroll
-    for (int v = 0; v < V; v++) {
-      s[v] = pm_step(sp.v[v]);
-      prop[v] = propose(r, s[v]);
-      ei[v] = 0;
-      ef[v] = 0;
+    for (int v = 0; v < 4; v++) {
+      s[v] = pm_step((int32_t)sd.w[v]);
+      pr[v] = propose(r, s[v]);
     }
   }
-  auto count1 = [&](int v, int32_t x) {
-    ei[v] += (x != old[v]);
-    if constexpr (!TWO) ef[v] += (x != prop[v]);
+  const Lanes<T> prop = Lanes<T>::pack(pr);
+  Tally<T, TWO> tally;
+  const int k = g.hx - a0;   // real lanes of this thread (>= 4 except at a row's end)
+  const bool ragged = (g.hx & 3) != 0;   // same for the whole launch
+  const RowEnd<T> rowend(k);
+
+  auto count_pack = [&](I row) { tally.add(load_lanes<T, I>(spin, row + (I)a0), old, prop); };
+  auto count_segment = [&](I row, int off) {
+    const Lanes<T> P = load_lanes<T, I>(spin, row + (I)a0);
+    const int xe = off ? (a0 + 4 >= g.hx ? 0 : a0 + 4) : (a0 == 0 ? g.hx - 1 : a0 - 1);
+    const uint32_t E = (uint32_t)(*elem<I>(spin, row + (I)xe)) & Lanes<T>::MASK;
+    Lanes<T> S = shifted<T>(P, E, off);
+    if (ragged && off) rowend.apply(S, E);
+    tally.add2(P, S, old, prop);
   };
-  auto count_pack = [&](const T *row) {
-    int32_t x[V];
-    load_pack<T, V>(row + a0, x);
-#pragma unroll
-    for (int v = 0; v < V; v++) count1(v, x[v]);
-  };
-  auto count_segment = [&](const T *row, int off) {
-    int32_t sg[V + 1];
-    load_segment<T, V>(row, a0, g.hx, off, sg);
-#pragma unroll
-    for (int v = 0; v < V; v++) {
-      count1(v, sg[v]);
-      count1(v, sg[v + 1]);
-    }
-  };
-  const int64_t px = g.px, pxy = (int64_t)g.px * g.hy;
 
   if constexpr (LAT == SPK_HIP_SQ_4N) {
-    const T *o = spin + (int64_t)(1 - c) * g.Mp;
-    count_segment(o + px * b, (b + c) & 1);
-    count_pack(o + px * wrap_dec(b, g.hy));
-    count_pack(o + px * wrap_inc(b, g.hy));
+    const I o = (I)(1 - c) * Mp;
+    count_segment(o + px * (I)b, (b + c) & 1);
+    count_pack(o + px * (I)wrap_dec(b, g.hy));
+    count_pack(o + px * (I)wrap_inc(b, g.hy));
   } else if constexpr (LAT == SPK_HIP_SC_6N) {
-    const T *o = spin + (int64_t)(1 - c) * g.Mp;
-    count_segment(o + px * b + pxy * e, (b + e + c) & 1);
-    count_pack(o + px * wrap_dec(b, g.hy) + pxy * e);
-    count_pack(o + px * wrap_inc(b, g.hy) + pxy * e);
-    count_pack(o + px * b + pxy * wrap_dec(e, g.hz));
-    count_pack(o + px * b + pxy * wrap_inc(e, g.hz));
+    const I o = (I)(1 - c) * Mp;
+    count_segment(o + px * (I)b + pxy * (I)e, (b + e + c) & 1);
+    count_pack(o + px * (I)wrap_dec(b, g.hy) + pxy * (I)e);
+    count_pack(o + px * (I)wrap_inc(b, g.hy) + pxy * (I)e);
+    count_pack(o + px * (I)b + pxy * (I)wrap_dec(e, g.hz));
+    count_pack(o + px * (I)b + pxy * (I)wrap_inc(e, g.hz));
   } else if constexpr (LAT == SPK_HIP_SQ_8N) {
     const int pi = c & 1, pj = c >> 1;
     const int ys[3] = {pj ? b : wrap_dec(b, g.hy), b, pj ? wrap_inc(b, g.hy) : b};
 #pragma unroll
     for (int dy = 0; dy < 3; dy++) {
       const int qy = pj ^ (dy != 1);
-      const int64_t row = px * ys[dy];
-      if (dy != 1) count_pack(spin + (int64_t)(pi + 2 * qy) * g.Mp + row);
-      count_segment(spin + (int64_t)((1 - pi) + 2 * qy) * g.Mp + row, pi);
+      const I row = px * (I)ys[dy];
+      if (dy != 1) count_pack((I)(pi + 2 * qy) * Mp + row);
+      count_segment((I)((1 - pi) + 2 * qy) * Mp + row, pi);
     }
   } else {
     const int pi = c & 1, pj = (c >> 1) & 1, pk = c >> 2;
@@ -408,34 +613,34 @@ __host__ __device__ __forceinline__ Counts sweep_sites(T *__restrict__ spin, int
 #pragma unroll
       for (int dy = 0; dy < 3; dy++) {
         const int qyz = 2 * (pj ^ (dy != 1)) + 4 * (pk ^ (dz != 1));
-        const int64_t row = px * ys[dy] + pxy * zs[dz];
-        if (dy != 1 || dz != 1) count_pack(spin + (int64_t)(pi + qyz) * g.Mp + row);
-        count_segment(spin + (int64_t)((1 - pi) + qyz) * g.Mp + row, pi);
+        const I row = px * (I)ys[dy] + pxy * (I)zs[dz];
+        if (dy != 1 || dz != 1) count_pack((I)(pi + qyz) * Mp + row);
+        count_segment((I)((1 - pi) + qyz) * Mp + row, pi);
+        // one z-plane's loads in flight at a time: bounds the live words
+        if (dy == 2) fence_schedule();
       }
   }
 
-  if constexpr (TWO) {
-    constexpr int NN = NeighCount<LAT>::value;
-#pragma unroll
-    for (int v = 0; v < V; v++) ef[v] = prop[v] == old[v] ? ei[v] : NN - ei[v];
-  }
+  constexpr int NN = NeighCount<LAT>::value;
   Counts out{0, 0};
-  Pack<T, V> nsp;
-  Pack<int32_t, V> nsd;
+  Lanes<T> nsp = old;
+  Words<4> nsd;
   bool any = false;
 #pragma unroll
-  for (int v = 0; v < V; v++) {
-    const bool accept = metropolis(r, thr, ei[v], ef[v], s[v]);
-    const bool site = a0 + v < g.hx;            // pad lanes count nothing
-    const bool changed = site && accept && prop[v] != old[v];
-    nsp.v[v] = (T)(changed ? prop[v] : old[v]);
-    nsd.v[v] = s[v];
+  for (int v = 0; v < 4; v++) {
+    const int o_v = old.get(v);
+    const int ei = tally.ei(v);
+    const int ef = TWO ? (pr[v] == o_v ? ei : NN - ei) : tally.ef(v);
+    const bool accept = metropolis(r, thr, ei, ef, s[v]);
+    const bool changed = v < k && accept && pr[v] != o_v;   // pad lanes count nothing
+    if (changed) nsp.set(v, (uint32_t)pr[v]);
+    nsd.w[v] = (uint32_t)s[v];
     any |= changed;
     out.nacc += changed;
-    out.de += changed ? ef[v] - ei[v] : 0;
+    out.de += changed ? ef - ei : 0;
   }
-  *reinterpret_cast<Pack<int32_t, V> *>(seed + self) = nsd;
-  if (any) *reinterpret_cast<Pack<T, V> *>(spin + self) = nsp;
+  *reinterpret_cast<Words<4> *>(elem<I>(seed, self)) = nsd;
+  if (any) store_lanes<T, I>(spin, self, nsp);
   return out;
 }
 
@@ -621,7 +826,7 @@ __device__ __forceinline__ void block_accumulate(int x, int y, unsigned long lon
   }
   __syncthreads();
   if (threadIdx.x == 0) {
-    long long sx = 0, sy = 0;
+    int sx = 0, sy = 0;   // a block's totals stay far below 2^31 as well
     const int nw = blockDim.x >> 6;
     for (int w = 0; w < nw; w++) {
       sx += red[0][w];

@@ -25,7 +25,7 @@ using namespace spk;
 // Thread t owns V consecutive sites of one sub-grid row (x fastest), so its
 // own spins/seeds are one aligned pack and the wavefront's accesses are
 // unit-stride.  Workgroups are remapped so neighbouring tiles share an XCD.
-template <int LAT, typename T, int V, bool TWO>
+template <int LAT, typename T, int V, bool TWO, typename I>
 __global__ __launch_bounds__(1024) void k_sweep_color(T *__restrict__ spin, int32_t *__restrict__ seed,
                                                       Geom g, Rule r, int color,
                                                       unsigned long long *__restrict__ acc) {
@@ -40,7 +40,7 @@ __global__ __launch_bounds__(1024) void k_sweep_color(T *__restrict__ spin, int3
   if (t < (int64_t)(g.Mp / V)) {
     int a0, b, e;
     pack_site(g, (uint32_t)t, a0, b, e);
-    cnt = sweep_sites<LAT, T, V, TWO>(spin, seed, g, r, s_thr, color, a0, b, e);
+    cnt = sweep_sites<LAT, T, V, TWO, I>(spin, seed, g, r, s_thr, color, a0, b, e);
   }
   block_accumulate(cnt.nacc, cnt.de, acc, blockIdx.x & (kAccSlots - 1));
 }
@@ -175,15 +175,23 @@ int launch_sweep(spk_hip_lattice *h, int color, unsigned long long *acc) {
   const int64_t threads = h->g.Mp / kVec;
   unsigned grid = grid_for(threads, bs);
   grid = (grid + kNumXcd - 1) / kNumXcd * kNumXcd;   // xcd_tile needs a multiple of 8
-  const bool two = h->rule.nspins == 2;
+  const bool two = h->rule.nspins == 2;   // two states are always stored as u8
+  const bool off32 = offsets_fit_32(h->g);
   with_lattice(h->g.lat, [&]<int LAT>() {
     with_type(h->stype, [&]<typename T>() {
-      if (two)
-        hipLaunchKernelGGL((k_sweep_color<LAT, T, kVec, true>), dim3(grid), dim3(bs), 0, h->stream,
-                           static_cast<T *>(h->spin), h->seed, h->g, h->rule, color, acc);
-      else
-        hipLaunchKernelGGL((k_sweep_color<LAT, T, kVec, false>), dim3(grid), dim3(bs), 0, h->stream,
-                           static_cast<T *>(h->spin), h->seed, h->g, h->rule, color, acc);
+      auto go = [&](auto kern) {
+        hipLaunchKernelGGL(kern, dim3(grid), dim3(bs), 0, h->stream, static_cast<T *>(h->spin), h->seed, h->g,
+                           h->rule, color, acc);
+      };
+      if constexpr (sizeof(T) == 1) {
+        if (two) {
+          if (off32) go(k_sweep_color<LAT, T, kVec, true, uint32_t>);
+          else go(k_sweep_color<LAT, T, kVec, true, int64_t>);
+          return;
+        }
+      }
+      if (off32) go(k_sweep_color<LAT, T, kVec, false, uint32_t>);
+      else go(k_sweep_color<LAT, T, kVec, false, int64_t>);
     });
   });
   return check_hip(hipGetLastError(), "sweep kernel launch");

@@ -156,8 +156,16 @@ static int run(int app, int nx, int ny, int nz, int q, double kt, int nsw, const
           fprintf(stderr, "pack_site disagrees with / and %%\n");
           return 10;
         }
-        Counts cnt = r.nspins == 2 ? sweep_sites<LAT, T, V, true>(s.data(), k.data(), g, r, r.thr, c, a0, b, e)
-                                   : sweep_sites<LAT, T, V, false>(s.data(), k.data(), g, r, r.thr, c, a0, b, e);
+        // the kernels' dispatch: the two-state shortcut for u8 storage;
+        // 32-bit offsets on even sweeps, 64-bit on odd ones (both forms run)
+        const bool two = r.nspins == 2 && sizeof(T) == 1;
+        Counts cnt;
+        if (w % 2 == 0)
+          cnt = two ? sweep_sites<LAT, T, V, true, uint32_t>(s.data(), k.data(), g, r, r.thr, c, a0, b, e)
+                    : sweep_sites<LAT, T, V, false, uint32_t>(s.data(), k.data(), g, r, r.thr, c, a0, b, e);
+        else
+          cnt = two ? sweep_sites<LAT, T, V, true, int64_t>(s.data(), k.data(), g, r, r.thr, c, a0, b, e)
+                    : sweep_sites<LAT, T, V, false, int64_t>(s.data(), k.data(), g, r, r.thr, c, a0, b, e);
         na += cnt.nacc;
         de += cnt.de;
       }

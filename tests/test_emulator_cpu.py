@@ -41,6 +41,12 @@ CASES = [
     ("potts", "sc/6n", (2, 2, 2), 9, 0.9, 24, 4),
     ("ising", "sc/6n", (4, 4, 2), 2, 1.0, 29, 4),
     ("ising", "sq/8n", (12, 12, 1), 2, 0.0, 37, 4),
+    # Potts with two states takes the two-state shortcut too
+    ("potts", "sq/4n", (10, 6, 1), 2, 0.8, 41, 4),
+    ("potts", "sc/26n", (6, 4, 2), 2, 1.2, 42, 4),
+    # byte lanes holding values >= 0x80 (u8 storage up to Q = 255)
+    ("potts", "sc/26n", (6, 4, 4), 255, 0.6, 43, 4),
+    ("potts", "sq/8n", (10, 6, 1), 200, 0.7, 44, 4),
 ]
 
 
