@@ -52,7 +52,6 @@ __global__ __launch_bounds__(1024) void k_batch_step(char *__restrict__ pool, co
                                                      const int *__restrict__ pre, int j, int L,
                                                      unsigned long long *__restrict__ acc) {
   __shared__ int s_thr[32];
-  __shared__ BatchRep s_rep;
   const StepDesc sd = steps[j];
   const int G = gridDim.x;
   const int lw = xcd_tile(blockIdx.x, G);
@@ -60,40 +59,40 @@ __global__ __launch_bounds__(1024) void k_batch_step(char *__restrict__ pool, co
   const int t0 = lw * chunk;
   const int t1 = min(sd.total, t0 + chunk);
   // every thread walks the same tiles, so i, r and cur are uniform across
-  // the workgroup and all threads take the same branches to each barrier
+  // the workgroup and all threads take the same branches to each barrier;
+  // the replica's table entry is read with uniform addresses (scalar loads)
   int i = 0;     // position in the step's replica list (monotone over the chunk)
-  int cur = -1;  // replica whose table entry is in LDS
+  int cur = -1;  // replica whose thresholds are in LDS
   for (int t = t0; t < t1; t++) {
     while (pre[sd.pre_first + i + 1] <= t) i++;
     const int r = act[sd.first + i];
+    const BatchRep &rep = reps[r];
     __syncthreads();   // previous tile's reduction and LDS reads are done
     if (r != cur) {
-      if (threadIdx.x == 0) s_rep = reps[r];
-      __syncthreads();
-      if (threadIdx.x < 28) s_thr[threadIdx.x] = s_rep.rule.thr[threadIdx.x];
+      if (threadIdx.x < 28) s_thr[threadIdx.x] = rep.rule.thr[threadIdx.x];
       __syncthreads();
       cur = r;
     }
     const int local = t - pre[sd.pre_first + i];
-    const int p = (int)(((int64_t)j * s_rep.P) / L);
-    const int K = s_rep.g.K;
+    const int p = (int)(((int64_t)j * rep.P) / L);
+    const int K = rep.g.K;
     const int64_t ti = (int64_t)local * blockDim.x + threadIdx.x;
     Counts cnt{0, 0};
-    if (ti < (int64_t)(s_rep.g.Mp / kVec)) {
-      const Geom &g = s_rep.g;
+    if (ti < (int64_t)(rep.g.Mp / kVec)) {
+      const Geom g = rep.g;
       int a0, b, e;
       pack_site(g, (uint32_t)ti, a0, b, e);
-      // the replica's arrays, as workgroup-uniform (scalar) bases
-      T *sp = reinterpret_cast<T *>(uniform_ptr(pool + s_rep.spin_off));
-      int32_t *sd = reinterpret_cast<int32_t *>(uniform_ptr(pool + s_rep.seed_off));
+      T *sp = reinterpret_cast<T *>(pool + rep.spin_off);
+      int32_t *sdp = reinterpret_cast<int32_t *>(pool + rep.seed_off);
+      const Rule &rule = rep.rule;
       bool two = false;
-      if constexpr (sizeof(T) == 1) two = s_rep.rule.nspins == 2;   // two states are stored as u8
+      if constexpr (sizeof(T) == 1) two = rule.nspins == 2;   // two states are stored as u8
       if (two)
-        cnt = sweep_sites<LAT, T, kVec, true, uint32_t>(sp, sd, g, s_rep.rule, s_thr, p % K, a0, b, e);
+        cnt = sweep_sites<LAT, T, kVec, true, uint32_t>(sp, sdp, g, rule, s_thr, p % K, a0, b, e);
       else
-        cnt = sweep_sites<LAT, T, kVec, false, uint32_t>(sp, sd, g, s_rep.rule, s_thr, p % K, a0, b, e);
+        cnt = sweep_sites<LAT, T, kVec, false, uint32_t>(sp, sdp, g, rule, s_thr, p % K, a0, b, e);
     }
-    block_accumulate(cnt.nacc, cnt.de, acc + 2 * (int64_t)(s_rep.acc_row + p / K), 0);
+    block_accumulate(cnt.nacc, cnt.de, acc + 2 * (int64_t)(rep.acc_row + p / K), 0);
   }
 }
 

@@ -13,16 +13,6 @@
 
 namespace spk {
 
-// A pointer whose value is the same in every lane of the wavefront, moved
-// to scalar registers so loads from it take the scalar-base form.
-template <typename T>
-__device__ __forceinline__ T *uniform_ptr(T *p) {
-  const uint64_t v = reinterpret_cast<uint64_t>(p);
-  const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)v);
-  const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(v >> 32));
-  return reinterpret_cast<T *>(((uint64_t)hi << 32) | lo);
-}
-
 // ---------------------------------------------------------------- RNG
 // Park-Miller minimal standard: s' = 16807*s mod (2^31-1).  The reference
 // evaluates it with Schrage's decomposition (src/random_park.cpp:79-86); both
@@ -447,13 +437,15 @@ __host__ __device__ __forceinline__ void fence_schedule() {
 }
 
 // acc += per-halfword "differs" flags of d (1 where the halfword of d is
-// nonzero), and of d and e for the two-word form.  The sum is formed in the
+// nonzero), and of d and e for the two-word form.  op_sel_hi:[1,0] makes
+// the high lane read the constant's low half (1) as well; by default it
+// would read its high half (0).  The sum is formed in the
 // same asm statement, so the flags die at once instead of being gathered
 // into one late sum tree that keeps every one of them live.
 __host__ __device__ __forceinline__ void add_ne_halves(uint32_t &acc, uint32_t d) {
 #ifdef __HIP_DEVICE_COMPILE__
   uint32_t t;
-  asm("v_pk_min_u16 %1, %2, 1\n\tv_add_u32 %0, %0, %1" : "+v"(acc), "=&v"(t) : "v"(d));
+  asm("v_pk_min_u16 %1, %2, 1 op_sel_hi:[1,0]\n\tv_add_u32 %0, %0, %1" : "+v"(acc), "=&v"(t) : "v"(d));
 #else
   acc += ((d & 0xffffu) ? 1u : 0u) | ((d >> 16) ? 0x10000u : 0u);
 #endif
@@ -462,12 +454,33 @@ __host__ __device__ __forceinline__ void add_ne_halves(uint32_t &acc, uint32_t d
 __host__ __device__ __forceinline__ void add_ne_halves2(uint32_t &acc, uint32_t d, uint32_t e) {
 #ifdef __HIP_DEVICE_COMPILE__
   uint32_t t, u;
-  asm("v_pk_min_u16 %1, %3, 1\n\tv_pk_min_u16 %2, %4, 1\n\tv_add3_u32 %0, %0, %1, %2"
+  asm("v_pk_min_u16 %1, %3, 1 op_sel_hi:[1,0]\n\tv_pk_min_u16 %2, %4, 1 op_sel_hi:[1,0]\n\t"
+      "v_add3_u32 %0, %0, %1, %2"
       : "+v"(acc), "=&v"(t), "=&v"(u)
       : "v"(d), "v"(e));
 #else
   add_ne_halves(acc, d);
   add_ne_halves(acc, e);
+#endif
+}
+
+// Pins a running sum in a register at this point, so later terms are added
+// one by one rather than gathered into a late sum tree.
+__host__ __device__ __forceinline__ void keep_sum(uint32_t &acc) {
+#ifdef __HIP_DEVICE_COMPILE__
+  asm("" : "+v"(acc));
+#endif
+}
+
+// acc += (d != 0) + (e != 0) for whole 32-bit lanes, in the same way
+__host__ __device__ __forceinline__ void add_ne_words2(uint32_t &acc, uint32_t d, uint32_t e) {
+#ifdef __HIP_DEVICE_COMPILE__
+  uint32_t t, u;
+  asm("v_min_u32 %1, %3, 1\n\tv_min_u32 %2, %4, 1\n\tv_add3_u32 %0, %0, %1, %2"
+      : "+v"(acc), "=&v"(t), "=&v"(u)
+      : "v"(d), "v"(e));
+#else
+  acc += (d != 0) + (e != 0);
 #endif
 }
 
@@ -496,9 +509,14 @@ struct Tally {
   __host__ __device__ __forceinline__ void add(const Lanes<T> &x, const Lanes<T> &old, const Lanes<T> &prop) {
     if constexpr (kXorSum) {
       o[0] += x.w[0] ^ old.w[0];
+      keep_sum(o[0]);
     } else if constexpr (NW == 1) {
       o[0] += ne_bytes(x.w[0] ^ old.w[0]);
-      if constexpr (!TWO) p[0] += ne_bytes(x.w[0] ^ prop.w[0]);
+      keep_sum(o[0]);
+      if constexpr (!TWO) {
+        p[0] += ne_bytes(x.w[0] ^ prop.w[0]);
+        keep_sum(p[0]);
+      }
     } else if constexpr (NW == 2) {
 #pragma unroll
       for (int i = 0; i < 2; i++) {
@@ -521,6 +539,12 @@ struct Tally {
       for (int i = 0; i < 2; i++) {
         add_ne_halves2(o[i], x.w[i] ^ old.w[i], y.w[i] ^ old.w[i]);
         if constexpr (!TWO) add_ne_halves2(p[i], x.w[i] ^ prop.w[i], y.w[i] ^ prop.w[i]);
+      }
+    } else if constexpr (NW == 4) {
+#pragma unroll
+      for (int i = 0; i < 4; i++) {
+        add_ne_words2(o[i], x.w[i] ^ old.w[i], y.w[i] ^ old.w[i]);
+        if constexpr (!TWO) add_ne_words2(p[i], x.w[i] ^ prop.w[i], y.w[i] ^ prop.w[i]);
       }
     } else {
       add(x, old, prop);
@@ -555,11 +579,16 @@ __host__ __device__ __forceinline__ Counts sweep_sites(T *__restrict__ spin, int
                                                        int c, int a0, int b, int e) {
   static_assert(V == 4, "4 sites per thread");
   const I px = (I)g.px, pxy = (I)g.px * (I)g.hy, Mp = (I)g.Mp;
-  const I self = (I)c * Mp + (I)a0 + px * (I)b + pxy * (I)e;
-  const Lanes<T> old = load_lanes<T, I>(spin, self);
+  // colour q's sub-grid: a base that is the same for the whole launch, so
+  // loads of one row from several colours share one per-thread offset
+  auto colour = [&](int q) { return elem<I>(static_cast<const T *>(spin), (I)q * Mp); };
+  const I self = (I)a0 + px * (I)b + pxy * (I)e;   // within colour c
+  T *const spin_c = elem<I>(spin, (I)c * Mp);
+  int32_t *const seed_c = elem<I>(seed, (I)c * Mp);
+  const Lanes<T> old = load_lanes<T, I>(spin_c, self);
   int32_t s[4], pr[4];
   {
-    const Words<4> sd = *reinterpret_cast<const Words<4> *>(elem<I>(seed, self));
+    const Words<4> sd = *reinterpret_cast<const Words<4> *>(elem<I>(seed_c, self));
 #pragma unroll
     for (int v = 0; v < 4; v++) {
       s[v] = pm_step((int32_t)sd.w[v]);
@@ -572,28 +601,28 @@ __host__ __device__ __forceinline__ Counts sweep_sites(T *__restrict__ spin, int
   const bool ragged = (g.hx & 3) != 0;   // same for the whole launch
   const RowEnd<T> rowend(k);
 
-  auto count_pack = [&](I row) { tally.add(load_lanes<T, I>(spin, row + (I)a0), old, prop); };
-  auto count_segment = [&](I row, int off) {
-    const Lanes<T> P = load_lanes<T, I>(spin, row + (I)a0);
+  auto count_pack = [&](const T *base, I row) { tally.add(load_lanes<T, I>(base, row + (I)a0), old, prop); };
+  auto count_segment = [&](const T *base, I row, int off) {
+    const Lanes<T> P = load_lanes<T, I>(base, row + (I)a0);
     const int xe = off ? (a0 + 4 >= g.hx ? 0 : a0 + 4) : (a0 == 0 ? g.hx - 1 : a0 - 1);
-    const uint32_t E = (uint32_t)(*elem<I>(spin, row + (I)xe)) & Lanes<T>::MASK;
+    const uint32_t E = (uint32_t)(*elem<I>(base, row + (I)xe)) & Lanes<T>::MASK;
     Lanes<T> S = shifted<T>(P, E, off);
     if (ragged && off) rowend.apply(S, E);
     tally.add2(P, S, old, prop);
   };
 
   if constexpr (LAT == SPK_HIP_SQ_4N) {
-    const I o = (I)(1 - c) * Mp;
-    count_segment(o + px * (I)b, (b + c) & 1);
-    count_pack(o + px * (I)wrap_dec(b, g.hy));
-    count_pack(o + px * (I)wrap_inc(b, g.hy));
+    const T *o = colour(1 - c);
+    count_segment(o, px * (I)b, (b + c) & 1);
+    count_pack(o, px * (I)wrap_dec(b, g.hy));
+    count_pack(o, px * (I)wrap_inc(b, g.hy));
   } else if constexpr (LAT == SPK_HIP_SC_6N) {
-    const I o = (I)(1 - c) * Mp;
-    count_segment(o + px * (I)b + pxy * (I)e, (b + e + c) & 1);
-    count_pack(o + px * (I)wrap_dec(b, g.hy) + pxy * (I)e);
-    count_pack(o + px * (I)wrap_inc(b, g.hy) + pxy * (I)e);
-    count_pack(o + px * (I)b + pxy * (I)wrap_dec(e, g.hz));
-    count_pack(o + px * (I)b + pxy * (I)wrap_inc(e, g.hz));
+    const T *o = colour(1 - c);
+    count_segment(o, px * (I)b + pxy * (I)e, (b + e + c) & 1);
+    count_pack(o, px * (I)wrap_dec(b, g.hy) + pxy * (I)e);
+    count_pack(o, px * (I)wrap_inc(b, g.hy) + pxy * (I)e);
+    count_pack(o, px * (I)b + pxy * (I)wrap_dec(e, g.hz));
+    count_pack(o, px * (I)b + pxy * (I)wrap_inc(e, g.hz));
   } else if constexpr (LAT == SPK_HIP_SQ_8N) {
     const int pi = c & 1, pj = c >> 1;
     const int ys[3] = {pj ? b : wrap_dec(b, g.hy), b, pj ? wrap_inc(b, g.hy) : b};
@@ -601,8 +630,8 @@ __host__ __device__ __forceinline__ Counts sweep_sites(T *__restrict__ spin, int
     for (int dy = 0; dy < 3; dy++) {
       const int qy = pj ^ (dy != 1);
       const I row = px * (I)ys[dy];
-      if (dy != 1) count_pack((I)(pi + 2 * qy) * Mp + row);
-      count_segment((I)((1 - pi) + 2 * qy) * Mp + row, pi);
+      if (dy != 1) count_pack(colour(pi + 2 * qy), row);
+      count_segment(colour((1 - pi) + 2 * qy), row, pi);
     }
   } else {
     const int pi = c & 1, pj = (c >> 1) & 1, pk = c >> 2;
@@ -614,8 +643,8 @@ __host__ __device__ __forceinline__ Counts sweep_sites(T *__restrict__ spin, int
       for (int dy = 0; dy < 3; dy++) {
         const int qyz = 2 * (pj ^ (dy != 1)) + 4 * (pk ^ (dz != 1));
         const I row = px * (I)ys[dy] + pxy * (I)zs[dz];
-        if (dy != 1 || dz != 1) count_pack((I)(pi + qyz) * Mp + row);
-        count_segment((I)((1 - pi) + qyz) * Mp + row, pi);
+        if (dy != 1 || dz != 1) count_pack(colour(pi + qyz), row);
+        count_segment(colour((1 - pi) + qyz), row, pi);
         // one z-plane's loads in flight at a time: bounds the live words
         if (dy == 2) fence_schedule();
       }
@@ -639,8 +668,8 @@ __host__ __device__ __forceinline__ Counts sweep_sites(T *__restrict__ spin, int
     out.nacc += changed;
     out.de += changed ? ef - ei : 0;
   }
-  *reinterpret_cast<Words<4> *>(elem<I>(seed, self)) = nsd;
-  if (any) store_lanes<T, I>(spin, self, nsp);
+  *reinterpret_cast<Words<4> *>(elem<I>(seed_c, self)) = nsd;
+  if (any) store_lanes<T, I>(spin_c, self, nsp);
   return out;
 }
 
