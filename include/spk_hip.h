@@ -143,11 +143,17 @@ int spk_hip_sync(spk_hip_lattice *h);
    block = threads per workgroup (64..1024, multiple of 64). */
 int spk_hip_set_launch(spk_hip_lattice *h, int block);
 
+/* 4-site packs per thread in the per-colour kernel: 1 or 2 (a thread then
+   loads both packs before deciding either).  No reference counterpart; a
+   launch setting like spk_hip_set_launch, results do not depend on it. */
+int spk_hip_set_packs(spk_hip_lattice *h, int packs);
+
 /* Sweep-fused launches for 2d lattices (sq/4n, sq/8n): each workgroup holds
    a tile plus a halo in LDS and runs several whole sweeps there (4 for
-   sq/4n, 2 for sq/8n) before writing back.  On by default where the tile
-   fits; results are identical either way.  get_fusion reports the sweeps
-   per fused launch (0 = per-colour kernels only). */
+   sq/4n, 2 for sq/8n) before writing back.  Off by default; enable takes
+   effect where the tile fits.  Results are identical either way.
+   get_fusion reports the sweeps per fused launch (0 = per-colour kernels
+   only). */
 int spk_hip_set_fusion(spk_hip_lattice *h, int enable);
 int spk_hip_get_fusion(spk_hip_lattice *h, int *sweeps_per_launch);
 

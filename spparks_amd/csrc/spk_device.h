@@ -567,39 +567,52 @@ struct Counts {
   int de;     // sum of (ef - ei) over accepted flips
 };
 
-// Strict events for sites a0..a0+3 of sub-grid row (b, e) of colour c.  The
-// first draws and the proposals come first, so every neighbour word is
-// compared with the old and the proposed lanes as it arrives.  TWO: the
-// lattice has two states (Ising, or Potts Q=2), so every neighbour equals
-// either the old spin or the proposal and ef follows from ei.  I: element
-// offset type (uint32_t while the seed array holds fewer than 2^32 bytes).
-template <int LAT, typename T, int V, bool TWO, typename I = int64_t>
-__host__ __device__ __forceinline__ Counts sweep_sites(T *__restrict__ spin, int32_t *__restrict__ seed,
-                                                       const Geom &g, const Rule &r, const int *thr,
-                                                       int c, int a0, int b, int e) {
-  static_assert(V == 4, "4 sites per thread");
+// Strict events for sites a0..a0+3 of sub-grid row (b, e) of colour c, in
+// two phases.  pack_count loads the pack's spins and seeds, takes the first
+// draws and the proposals, and streams every neighbour word into the counts
+// of unlike neighbours (old and proposed lanes) as it arrives.  pack_decide
+// applies the Metropolis rule and stores the pack.  Within a colour pass the
+// neighbour values are read-only and each pack touches only its own slots,
+// so a thread may count several packs before deciding any of them.
+// TWO: the lattice has two states (Ising, or Potts Q=2), so every neighbour
+// equals either the old spin or the proposal and ef follows from ei.  I:
+// element offset type (uint32_t while the seed array holds fewer than 2^32
+// bytes).
+template <typename T, bool TWO, typename I>
+struct PackWork {
+  I self;             // the pack's offset within its colour's sub-grid
+  int k;              // real lanes (hx - a0; >= 4 except at a row's end)
+  Lanes<T> old;
+  int32_t s[4], pr[4];
+  Tally<T, TWO> tally;
+};
+
+template <int LAT, typename T, bool TWO, typename I>
+__host__ __device__ __forceinline__ PackWork<T, TWO, I> pack_count(const T *__restrict__ spin,
+                                                                   const int32_t *__restrict__ seed,
+                                                                   const Geom &g, const Rule &r, int c, int a0,
+                                                                   int b, int e) {
+  PackWork<T, TWO, I> w;
   const I px = (I)g.px, pxy = (I)g.px * (I)g.hy, Mp = (I)g.Mp;
   // colour q's sub-grid: a base that is the same for the whole launch, so
   // loads of one row from several colours share one per-thread offset
-  auto colour = [&](int q) { return elem<I>(static_cast<const T *>(spin), (I)q * Mp); };
-  const I self = (I)a0 + px * (I)b + pxy * (I)e;   // within colour c
-  T *const spin_c = elem<I>(spin, (I)c * Mp);
-  int32_t *const seed_c = elem<I>(seed, (I)c * Mp);
-  const Lanes<T> old = load_lanes<T, I>(spin_c, self);
-  int32_t s[4], pr[4];
+  auto colour = [&](int q) { return elem<I>(spin, (I)q * Mp); };
+  w.self = (I)a0 + px * (I)b + pxy * (I)e;
+  w.k = g.hx - a0;
+  w.old = load_lanes<T, I>(colour(c), w.self);
   {
-    const Words<4> sd = *reinterpret_cast<const Words<4> *>(elem<I>(seed_c, self));
+    const Words<4> sd = *reinterpret_cast<const Words<4> *>(elem<I>(elem<I>(seed, (I)c * Mp), w.self));
 #pragma unroll
     for (int v = 0; v < 4; v++) {
-      s[v] = pm_step((int32_t)sd.w[v]);
-      pr[v] = propose(r, s[v]);
+      w.s[v] = pm_step((int32_t)sd.w[v]);
+      w.pr[v] = propose(r, w.s[v]);
     }
   }
-  const Lanes<T> prop = Lanes<T>::pack(pr);
-  Tally<T, TWO> tally;
-  const int k = g.hx - a0;   // real lanes of this thread (>= 4 except at a row's end)
+  const Lanes<T> &old = w.old;
+  const Lanes<T> prop = Lanes<T>::pack(w.pr);
+  Tally<T, TWO> &tally = w.tally;
   const bool ragged = (g.hx & 3) != 0;   // same for the whole launch
-  const RowEnd<T> rowend(k);
+  const RowEnd<T> rowend(w.k);
 
   auto count_pack = [&](const T *base, I row) { tally.add(load_lanes<T, I>(base, row + (I)a0), old, prop); };
   auto count_segment = [&](const T *base, I row, int off) {
@@ -650,6 +663,22 @@ __host__ __device__ __forceinline__ Counts sweep_sites(T *__restrict__ spin, int
       }
   }
 
+  return w;
+}
+
+template <int LAT, typename T, bool TWO, typename I>
+__host__ __device__ __forceinline__ Counts pack_decide(PackWork<T, TWO, I> &w, T *__restrict__ spin,
+                                                      int32_t *__restrict__ seed, const Geom &g, const Rule &r,
+                                                      const int *thr, int c) {
+  const I Mp = (I)g.Mp;
+  T *const spin_c = elem<I>(spin, (I)c * Mp);
+  int32_t *const seed_c = elem<I>(seed, (I)c * Mp);
+  const int k = w.k;
+  const I self = w.self;
+  const Lanes<T> &old = w.old;
+  int32_t *s = w.s;
+  const int32_t *pr = w.pr;
+  const Tally<T, TWO> &tally = w.tally;
   constexpr int NN = NeighCount<LAT>::value;
   Counts out{0, 0};
   Lanes<T> nsp = old;
@@ -671,6 +700,16 @@ __host__ __device__ __forceinline__ Counts sweep_sites(T *__restrict__ spin, int
   *reinterpret_cast<Words<4> *>(elem<I>(seed_c, self)) = nsd;
   if (any) store_lanes<T, I>(spin_c, self, nsp);
   return out;
+}
+
+
+template <int LAT, typename T, int V, bool TWO, typename I = int64_t>
+__host__ __device__ __forceinline__ Counts sweep_sites(T *__restrict__ spin, int32_t *__restrict__ seed,
+                                                       const Geom &g, const Rule &r, const int *thr,
+                                                       int c, int a0, int b, int e) {
+  static_assert(V == 4, "4 sites per thread");
+  PackWork<T, TWO, I> w = pack_count<LAT, T, TWO, I>(spin, seed, g, r, c, a0, b, e);
+  return pack_decide<LAT, T, TWO, I>(w, spin, seed, g, r, thr, c);
 }
 
 // ------------------------------------------ sweep-fused tiles (2d lattices)

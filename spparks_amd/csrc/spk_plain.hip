@@ -22,10 +22,13 @@ using namespace spk;
 // ==================================================================== kernels
 
 // One colour pass: every site of colour `color` performs one strict event.
-// Thread t owns V consecutive sites of one sub-grid row (x fastest), so its
-// own spins/seeds are one aligned pack and the wavefront's accesses are
-// unit-stride.  Workgroups are remapped so neighbouring tiles share an XCD.
-template <int LAT, typename T, int V, bool TWO, typename I>
+// A pack is 4 consecutive sites of one sub-grid row (x fastest), so its own
+// spins/seeds are one aligned load and the wavefront's accesses are
+// unit-stride.  With PACKS = 2 thread t also owns pack t + span (span =
+// half the packs, rounded up) and counts both before deciding either: twice
+// the loads in flight per wave.  Workgroups are remapped so neighbouring
+// tiles share an XCD.
+template <int LAT, typename T, int V, bool TWO, typename I, int PACKS>
 __global__ __launch_bounds__(1024) void k_sweep_color(T *__restrict__ spin, int32_t *__restrict__ seed,
                                                       Geom g, Rule r, int color,
                                                       unsigned long long *__restrict__ acc) {
@@ -36,11 +39,30 @@ __global__ __launch_bounds__(1024) void k_sweep_color(T *__restrict__ spin, int3
   static_assert(V == 4, "pack_site maps 4 sites per thread");
   const int tile = xcd_tile(blockIdx.x, gridDim.x);
   const int64_t t = (int64_t)tile * blockDim.x + threadIdx.x;
+  const int64_t npack = g.Mp / V;
+  const int64_t span = (npack + PACKS - 1) / PACKS;
   Counts cnt{0, 0};
-  if (t < (int64_t)(g.Mp / V)) {
+  if (t < span) {
     int a0, b, e;
     pack_site(g, (uint32_t)t, a0, b, e);
-    cnt = sweep_sites<LAT, T, V, TWO, I>(spin, seed, g, r, s_thr, color, a0, b, e);
+    if constexpr (PACKS == 1) {
+      cnt = sweep_sites<LAT, T, V, TWO, I>(spin, seed, g, r, s_thr, color, a0, b, e);
+    } else {
+      static_assert(PACKS == 2, "one or two packs per thread");
+      PackWork<T, TWO, I> w0 = pack_count<LAT, T, TWO, I>(spin, seed, g, r, color, a0, b, e);
+      const bool second = t + span < npack;
+      PackWork<T, TWO, I> w1;
+      if (second) {
+        pack_site(g, (uint32_t)(t + span), a0, b, e);
+        w1 = pack_count<LAT, T, TWO, I>(spin, seed, g, r, color, a0, b, e);
+      }
+      cnt = pack_decide<LAT, T, TWO, I>(w0, spin, seed, g, r, s_thr, color);
+      if (second) {
+        const Counts c1 = pack_decide<LAT, T, TWO, I>(w1, spin, seed, g, r, s_thr, color);
+        cnt.nacc += c1.nacc;
+        cnt.de += c1.de;
+      }
+    }
   }
   block_accumulate(cnt.nacc, cnt.de, acc, blockIdx.x & (kAccSlots - 1));
 }
@@ -141,6 +163,7 @@ struct spk_hip_lattice {
   void *spin_alt = nullptr;            // second state buffer for sweep-fused launches
   int32_t *seed_alt = nullptr;
   int fuse = 0;                        // sweeps per fused launch (0: per-colour kernel only)
+  int packs = 1;                       // 4-site packs per thread in the per-colour kernel
   int tile_x = 0, tile_y = 0;          // fused core tile
   int32_t *stage = nullptr;            // N int32, natural order (transfers)
   unsigned long long *acc = nullptr;   // [cap][kAccSlots][2]
@@ -172,26 +195,30 @@ unsigned grid_for(int64_t n, int bs) { return (unsigned)((n + bs - 1) / bs); }
 
 int launch_sweep(spk_hip_lattice *h, int color, unsigned long long *acc) {
   const int bs = h->block;
-  const int64_t threads = h->g.Mp / kVec;
+  const int64_t threads = (h->g.Mp / kVec + h->packs - 1) / h->packs;
   unsigned grid = grid_for(threads, bs);
   grid = (grid + kNumXcd - 1) / kNumXcd * kNumXcd;   // xcd_tile needs a multiple of 8
   const bool two = h->rule.nspins == 2;   // two states are always stored as u8
   const bool off32 = offsets_fit_32(h->g);
+  const bool two_packs = h->packs == 2;
   with_lattice(h->g.lat, [&]<int LAT>() {
     with_type(h->stype, [&]<typename T>() {
       auto go = [&](auto kern) {
         hipLaunchKernelGGL(kern, dim3(grid), dim3(bs), 0, h->stream, static_cast<T *>(h->spin), h->seed, h->g,
                            h->rule, color, acc);
       };
+      // 64-bit offsets (lattices of 2^30 sites or more) take one pack per thread
       if constexpr (sizeof(T) == 1) {
         if (two) {
-          if (off32) go(k_sweep_color<LAT, T, kVec, true, uint32_t>);
-          else go(k_sweep_color<LAT, T, kVec, true, int64_t>);
+          if (!off32) go(k_sweep_color<LAT, T, kVec, true, int64_t, 1>);
+          else if (two_packs) go(k_sweep_color<LAT, T, kVec, true, uint32_t, 2>);
+          else go(k_sweep_color<LAT, T, kVec, true, uint32_t, 1>);
           return;
         }
       }
-      if (off32) go(k_sweep_color<LAT, T, kVec, false, uint32_t>);
-      else go(k_sweep_color<LAT, T, kVec, false, int64_t>);
+      if (!off32) go(k_sweep_color<LAT, T, kVec, false, int64_t, 1>);
+      else if (two_packs) go(k_sweep_color<LAT, T, kVec, false, uint32_t, 2>);
+      else go(k_sweep_color<LAT, T, kVec, false, uint32_t, 1>);
     });
   });
   return check_hip(hipGetLastError(), "sweep kernel launch");
@@ -431,7 +458,7 @@ int spk_hip_create(const spk_hip_desc *d, spk_hip_lattice **out) {
   h->N = (int64_t)g.M * g.K;
   h->slots = (int64_t)g.Mp * g.K;
   h->stype = spin_type_for(d->app, d->nspins);
-  h->block = g.lat == SPK_HIP_SQ_4N ? 512 : 256;   // measured best on MI355X (4096^2, 256^3)
+  h->block = 512;   // measured best on MI355X for 4096^2 sq/4n and 256^3 sc/26n
   if ((rc = check_hip(hipGetDevice(&h->device), "hipGetDevice")) ||
       (rc = check_hip(hipStreamCreate(&h->stream), "hipStreamCreate")) ||
       (rc = check_hip(hipEventCreate(&h->ev0), "hipEventCreate")) ||
@@ -688,6 +715,14 @@ int spk_hip_set_launch(spk_hip_lattice *h, int block) {
   if (block < 64 || block > 1024 || block % 64)
     return set_error(SPK_HIP_EINVAL, "block must be a multiple of 64 in [64, 1024]");
   h->block = block;
+  drop_graphs(h);
+  return SPK_HIP_OK;
+}
+
+int spk_hip_set_packs(spk_hip_lattice *h, int packs) {
+  if (!h) return set_error(SPK_HIP_EINVAL, "null handle");
+  if (packs != 1 && packs != 2) return set_error(SPK_HIP_EINVAL, "packs per thread must be 1 or 2");
+  h->packs = packs;
   drop_graphs(h);
   return SPK_HIP_OK;
 }

@@ -61,13 +61,17 @@ CASES = [
 ]
 
 
+@pytest.mark.parametrize("packs", [1, 2])
 @pytest.mark.parametrize("case", CASES, ids=lambda c: "%s-%s-%s" % (c[0], c[1], "x".join(map(str, c[2]))))
-def test_small_lattice_bit_exact(orc, case):
+def test_small_lattice_bit_exact(orc, case, packs):
+    """Every lattice x app x storage type, with one or two 4-site packs per
+    thread (odd pack counts leave the last thread a single pack)."""
     import spparks_amd as S
     app, lat, shape, q, kt, seed, n = case
-    say("case", case)
+    say("case", case, "packs", packs)
     spin0, seeds0, spin1, seeds1, nacc, eng = oracle_run(orc, app, lat, shape, q, kt, seed, n)
     with gpu_init(S, app, lat, shape, q, kt, seed) as L:
+        L.set_packs(packs)
         gs0, gk0 = L.download()
         assert np.array_equal(gs0, spin0), "set site range on device"
         assert np.array_equal(gk0, seeds0), "tagreset seeds on device"
