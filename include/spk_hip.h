@@ -47,6 +47,10 @@ extern "C" {
 /* app styles (src/app_potts.h:18-19, src/app_ising.h:18-19) */
 #define SPK_HIP_POTTS 0
 #define SPK_HIP_ISING 1
+/* proposal variants (app_style potts/neigh Q, potts/neighonly Q, ising/single) */
+#define SPK_HIP_POTTS_NEIGH 2
+#define SPK_HIP_POTTS_NEIGHONLY 3
+#define SPK_HIP_ISING_SINGLE 4
 
 /* lattice styles (src/lattice.cpp:44-56): colours 2 / 4 / 2 / 8 */
 #define SPK_HIP_SQ_4N  0
@@ -55,10 +59,11 @@ extern "C" {
 #define SPK_HIP_SC_26N 3
 
 typedef struct spk_hip_desc {
-  int app;             /* SPK_HIP_POTTS or SPK_HIP_ISING                  */
+  int app;             /* SPK_HIP_POTTS, SPK_HIP_ISING or a variant       */
   int lattice;         /* SPK_HIP_SQ_4N ... SPK_HIP_SC_26N                */
   int nx, ny, nz;      /* periodic box in lattice units; nz = 1 in 2d     */
-  int nspins;          /* Potts Q (`app_style potts Q`); ignored for Ising */
+  int nspins;          /* Potts Q (`app_style potts[/neigh[only]] Q`);
+                          ignored (2) for ising and ising/single       */
   double temperature;  /* kT (`temperature` command); 0.0 = T=0 branch    */
 } spk_hip_desc;
 

@@ -16,7 +16,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "liboracle.so")
 REF_PATH = os.path.join(HERE, "_ref", "libref_random_park.so")
 
-APP = {"potts": 0, "ising": 1}
+APP = {"potts": 0, "ising": 1, "potts/neigh": 2, "potts/neighonly": 3, "ising/single": 4}
 LATTICE = {"sq/4n": 0, "sq/8n": 1, "sc/6n": 2, "sc/26n": 3}
 
 

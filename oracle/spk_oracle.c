@@ -312,10 +312,35 @@ static inline double orc_site_energy_nb(int ns, const int64_t *nb, const int *sp
   return (double)e;
 }
 
-/* AppPotts::site_event_rejection (src/app_potts.cpp:112-146) and
-   AppIsing::site_event_rejection (src/app_ising.cpp:104-136); mask
-   bookkeeping omitted (it never skips a site in color/strict mode).
-   nb holds the site's neighbour ids.  Returns 1 when the spin changed. */
+/* The distinct neighbour values unlike `me`, in order of first appearance
+   along the neighbour list (src/app_potts_neigh.cpp:78-88, the same loop in
+   src/app_potts_neighonly.cpp:78-88).  Returns their number. */
+static int orc_unique_unlike(int ns, const int64_t *nb, const int *spin, int me, int *unique)
+{
+  int nevent = 0;
+  for (int j = 0; j < ns; j++) {
+    int value = spin[nb[j]];
+    if (value == me) continue;
+    int m;
+    for (m = 0; m < nevent; m++)
+      if (value == unique[m]) break;
+    if (m < nevent) continue;
+    unique[nevent++] = value;
+  }
+  return nevent;
+}
+
+/* One strict site event; mask bookkeeping omitted (it never skips a site
+   in color/strict mode).  nb holds the site's neighbour ids in the
+   reference's order.  Returns 1 when the spin changed.
+     potts          AppPotts::site_event_rejection (src/app_potts.cpp:112-146)
+     ising          AppIsing::site_event_rejection (src/app_ising.cpp:104-136)
+     potts/neigh    AppPottsNeigh (src/app_potts_neigh.cpp:71-116): draw
+                    iran = (int)(maxneigh*u); no event when iran >= nevent
+     potts/neighonly AppPottsNeighOnly (src/app_potts_neighonly.cpp:71-118):
+                    no draw at all when nevent == 0; iran = (int)(nevent*u)
+     ising/single   AppIsingSingle (src/app_ising_single.cpp:39-77): the flip
+                    is proposed without a draw */
 static int orc_event_nb(const orc_desc *d, int ns, const int64_t *nb, int *spin,
                         int64_t g, int *seed, double t_inverse)
 {
@@ -326,8 +351,23 @@ static int orc_event_nb(const orc_desc *d, int ns, const int64_t *nb, int *spin,
     int iran = (int)(q * orc_pm_uniform(seed)) + 1;
     if (iran > q) iran = q;
     spin[g] = iran;
-  } else {
+  } else if (d->app == ORC_APP_ISING) {
     spin[g] = orc_pm_uniform(seed) < 0.5 ? 1 : 2;
+  } else if (d->app == ORC_APP_POTTS_NEIGH) {
+    int unique[26];
+    int nevent = orc_unique_unlike(ns, nb, spin, old, unique);
+    int iran = (int)(ns * orc_pm_uniform(seed));
+    if (iran >= nevent) return 0;
+    spin[g] = unique[iran];
+  } else if (d->app == ORC_APP_POTTS_NEIGHONLY) {
+    int unique[26];
+    int nevent = orc_unique_unlike(ns, nb, spin, old, unique);
+    if (nevent == 0) return 0;
+    int iran = (int)(nevent * orc_pm_uniform(seed));
+    if (iran >= nevent) iran = nevent - 1;
+    spin[g] = unique[iran];
+  } else {
+    spin[g] = old == 1 ? 2 : 1;
   }
   double efinal = orc_site_energy_nb(ns, nb, spin, spin[g]);
   if (efinal <= einitial) {

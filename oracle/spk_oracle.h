@@ -1,6 +1,7 @@
 /*
  * spk_oracle.h -- CPU restatement of SPPARKS' strict-mode Metropolis colour
- * sweep (sweep color/strict) for app_style potts / ising.
+ * sweep (sweep color/strict) for app_style potts / ising and the variants
+ * potts/neigh, potts/neighonly and ising/single.
  *
  * TEST INFRASTRUCTURE ONLY.  This library is the checker that the HIP path is
  * compared against; it is loaded only by tests/, __graft_entry__.smoke() and
@@ -19,14 +20,18 @@
 extern "C" {
 #endif
 
-enum { ORC_APP_POTTS = 0, ORC_APP_ISING = 1 };
+/* app_style potts, ising, and the strict-capable variants that differ only
+   in the proposed move: potts/neigh (src/app_potts_neigh.cpp), potts/neighonly
+   (src/app_potts_neighonly.cpp), ising/single (src/app_ising_single.cpp) */
+enum { ORC_APP_POTTS = 0, ORC_APP_ISING = 1, ORC_APP_POTTS_NEIGH = 2, ORC_APP_POTTS_NEIGHONLY = 3,
+       ORC_APP_ISING_SINGLE = 4 };
 enum { ORC_SQ_4N = 0, ORC_SQ_8N = 1, ORC_SC_6N = 2, ORC_SC_26N = 3 };
 
 typedef struct {
   int app;            /* ORC_APP_* */
   int lattice;        /* ORC_* lattice style */
   int nx, ny, nz;     /* nz = 1 for 2d lattices */
-  int nspins;         /* Potts Q; ignored (2) for Ising */
+  int nspins;         /* Potts Q; ignored (2) for the Ising styles */
   double temperature; /* kT; 0.0 selects the zero-temperature branch */
 } orc_desc;
 

@@ -63,7 +63,7 @@ def _bind():
 
 def replica_desc(app, lattice, nx, ny, nz=1, nspins=2, temperature=0.0, seed=1, lo=1, hi=None,
                  nsweeps=0):
-    q = int(nspins) if app == "potts" else 2
+    q = int(nspins) if app.startswith("potts") else 2
     d = Desc(APPS[app], LATTICES[lattice], int(nx), int(ny), int(nz), q, float(temperature))
     return ReplicaDesc(d, int(seed), int(lo), int(q if hi is None else hi), int(nsweeps))
 

@@ -225,21 +225,25 @@ void Spparks::cmd_seed(const std::vector<std::string> &a) {
   ranmaster_ = std::make_unique<spk::MasterRng>(s);
 }
 
-// Input::app_style (src/input.cpp:727-747), AppPotts/AppIsing constructors
-// (src/app_potts.cpp:30-54, src/app_ising.cpp:30-47)
+// Input::app_style (src/input.cpp:727-747), the app constructors
+// (src/app_potts.cpp:30-54, src/app_ising.cpp:30-47,
+// src/app_potts_neigh.cpp:29-40, src/app_potts_neighonly.cpp:29-40,
+// src/app_ising_single.cpp:26-30)
 void Spparks::cmd_app_style(const std::vector<std::string> &a) {
   if (a.empty()) throw Error("Illegal app_style command");
-  if (a[0] == "potts") {
+  if (a[0] == "potts" || a[0] == "potts/neigh" || a[0] == "potts/neighonly") {
     if (a.size() != 2) throw Error("Illegal app_style command");
     nspins_ = to_int(a[1], "app_style");
     if (nspins_ <= 0) throw Error("Illegal app_style command");
-    dt_sweep_ = 1.0 / nspins_;
-  } else if (a[0] == "ising") {
+    // potts/neigh[only] set dt_sweep = 1/maxneigh in init_app, at each run
+    dt_sweep_ = a[0] == "potts" ? 1.0 / nspins_ : 0.0;
+  } else if (a[0] == "ising" || a[0] == "ising/single") {
     if (a.size() != 1) throw Error("Illegal app_style command");
     nspins_ = 2;
-    dt_sweep_ = 1.0 / 2.0;
+    dt_sweep_ = a[0] == "ising" ? 1.0 / 2.0 : 1.0;
   } else {
-    throw Error("Unrecognized app style " + a[0] + " (this build runs potts and ising)");
+    throw Error("Unrecognized app style " + a[0] +
+                " (this build runs potts, potts/neigh, potts/neighonly, ising and ising/single)");
   }
   app_ = a[0];
   if (h_) {
@@ -352,7 +356,11 @@ void Spparks::cmd_create_sites(const std::vector<std::string> &a) {
     return;
   }
   spk_hip_desc d;
-  d.app = app_ == "potts" ? SPK_HIP_POTTS : SPK_HIP_ISING;
+  d.app = app_ == "potts"             ? SPK_HIP_POTTS
+          : app_ == "potts/neigh"     ? SPK_HIP_POTTS_NEIGH
+          : app_ == "potts/neighonly" ? SPK_HIP_POTTS_NEIGHONLY
+          : app_ == "ising/single"    ? SPK_HIP_ISING_SINGLE
+                                      : SPK_HIP_ISING;
   d.lattice = lattice_id(lattice_);
   d.nx = nx_;
   d.ny = ny_;
@@ -547,8 +555,10 @@ void Spparks::init_run() {
     hip(spk_hip_init_seeds(h_, rseed));
     ranstrict_made_ = true;
   }
-  // AppPotts/AppIsing::init_app site-value check
+  // init_app: site-value check; potts/neigh[only] take dt_sweep = 1/maxneigh
+  // (src/app_potts_neigh.cpp:54, src/app_potts_neighonly.cpp:54)
   if (!values_set_) throw Error("One or more sites have invalid values");
+  if (app_ == "potts/neigh" || app_ == "potts/neighonly") dt_sweep_ = 1.0 / maxneigh(lattice_);
   hip(spk_hip_set_temperature(h_, temperature_));
   if (dt_sweep_ == 0.0) throw Error("App did not set dt_sweep");
 }

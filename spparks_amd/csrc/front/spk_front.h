@@ -113,7 +113,7 @@ class Spparks {
   std::unique_ptr<spk::MasterRng> ranmaster_;
 
   // app
-  std::string app_;          // "potts" / "ising" / ""
+  std::string app_;          // "potts", "ising", a proposal variant, or ""
   int nspins_ = 0;
   double dt_sweep_ = 0.0;
   int dimension_ = 3;

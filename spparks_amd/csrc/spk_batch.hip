@@ -494,7 +494,7 @@ int spk_hip_batch_add(spk_hip_batch *b, const spk_hip_replica_desc *r, int *id) 
     return rc;
   if (r->seed <= 0) return set_error(SPK_HIP_EINVAL, "Illegal seed command");
   if (r->nsweeps < 0) return set_error(SPK_HIP_EINVAL, "nsweeps < 0");
-  const int qmax = r->lattice.app == SPK_HIP_POTTS ? r->lattice.nspins : 2;
+  const int qmax = app_qmax(r->lattice.app, r->lattice.nspins);
   if (r->lo < 1 || r->hi > qmax || r->hi < r->lo)
     return set_error(SPK_HIP_EVALUE, "One or more sites have invalid values (set %d %d, Q = %d)", r->lo, r->hi,
                      qmax);
@@ -505,6 +505,8 @@ int spk_hip_batch_add(spk_hip_batch *b, const spk_hip_replica_desc *r, int *id) 
       break;
     }
   if (slot < 0) return set_error(SPK_HIP_ENOMEM, "all %d replica slots are in use", (int)b->reps.size());
+  if (variant_app(r->lattice.app))
+    return set_error(SPK_HIP_EINVAL, "batched replicas run app styles potts and ising (got app %d)", r->lattice.app);
   if (!offsets_fit_32(R.g))
     return set_error(SPK_HIP_EINVAL, "a batched replica holds fewer than 2^30 sites (got %lld); run it alone",
                      (long long)R.g.K * R.g.M);
@@ -562,7 +564,7 @@ int spk_hip_batch_upload(spk_hip_batch *b, int id, const int32_t *spin, const in
   Replica &R = b->reps[id];
   const int64_t n = (int64_t)R.g.M * R.g.K;
   if ((rc = ensure_stage(b, n))) return rc;
-  const int qmax = R.desc.lattice.app == SPK_HIP_POTTS ? R.desc.lattice.nspins : 2;
+  const int qmax = app_qmax(R.desc.lattice.app, R.desc.lattice.nspins);
   for (int w = 0; w < 2; w++) {
     const int32_t *src = w == 0 ? spin : seed;
     if (!src) continue;

@@ -27,8 +27,7 @@ int check_hip(hipError_t e, const char *what) {
 
 int make_geom(const spk_hip_desc *d, Geom *g) {
   if (!d) return set_error(SPK_HIP_EINVAL, "null lattice description");
-  if (d->app != SPK_HIP_POTTS && d->app != SPK_HIP_ISING)
-    return set_error(SPK_HIP_EINVAL, "unknown app %d", d->app);
+  if (d->app < SPK_HIP_POTTS || d->app > SPK_HIP_ISING_SINGLE) return set_error(SPK_HIP_EINVAL, "unknown app %d", d->app);
   const int lat = d->lattice;
   const bool is3 = lat == SPK_HIP_SC_6N || lat == SPK_HIP_SC_26N;
   if (lat < SPK_HIP_SQ_4N || lat > SPK_HIP_SC_26N) return set_error(SPK_HIP_EINVAL, "unknown lattice %d", lat);
@@ -72,12 +71,12 @@ static int32_t first_state_at_least(double x) {
 static double (*volatile g_exp)(double) = ::exp;
 
 int make_rule(int app, int nspins, int lattice, double temperature, Rule *r) {
-  if (app == SPK_HIP_POTTS && nspins <= 0) return set_error(SPK_HIP_EINVAL, "Potts needs nspins > 0");
+  if (potts_family(app) && nspins <= 0) return set_error(SPK_HIP_EINVAL, "Potts needs nspins > 0");
   if (!(temperature >= 0.0) || std::isinf(temperature))
     return set_error(SPK_HIP_EINVAL, "temperature must be finite and >= 0");
   *r = Rule{};
   r->app = app;
-  r->nspins = app == SPK_HIP_POTTS ? nspins : 2;
+  r->nspins = app_qmax(app, nspins);
   r->zero_t = temperature == 0.0;
   r->half = first_state_at_least(0.5);
   const int maxd = maxneigh_of(lattice);

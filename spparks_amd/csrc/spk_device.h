@@ -291,6 +291,165 @@ __host__ __device__ __forceinline__ SiteOutcome site_event(const Rule &r, const 
   return o;
 }
 
+struct Counts {
+  int nacc;   // accepted flips
+  int de;     // sum of (ef - ei) over accepted flips
+};
+
+// ------------------------------------------- proposal variants (ordered)
+// potts/neigh, potts/neighonly and ising/single differ from potts / ising
+// only in the proposed move; energy and the Metropolis test are shared.
+// The first two pick among the distinct unlike neighbour values in order of
+// first appearance along the reference's neighbour list, so these rules see
+// the neighbours in that order: offsets (dx, dy, dz) with dx outermost, each
+// -1..1, kept when within the lattice's shell (CreateSites::offsets_2d /
+// offsets_3d, src/create_sites.cpp:1303-1371); the list is built in that
+// order (:562-640).  One site per thread; every neighbour is located from
+// the site's natural coordinates through blocked_index.
+
+template <int LAT>
+struct Shell {
+  static constexpr bool is3 = LAT == SPK_HIP_SC_6N || LAT == SPK_HIP_SC_26N;
+  static constexpr int far = (LAT == SPK_HIP_SQ_4N || LAT == SPK_HIP_SC_6N) ? 1 : is3 ? 3 : 2;   // max dx^2+dy^2+dz^2
+  // offset m of the reference's order, m < NeighCount<LAT>::value
+  __host__ __device__ static constexpr void offset(int m, int &dx, int &dy, int &dz) {
+    int n = 0;
+    for (int i = -1; i <= 1; i++)
+      for (int j = -1; j <= 1; j++)
+        for (int k = is3 ? -1 : 0; k <= (is3 ? 1 : 0); k++) {
+          const int r2 = i * i + j * j + k * k;
+          if (r2 < 1 || r2 > far) continue;
+          if (n++ == m) {
+            dx = i;
+            dy = j;
+            dz = k;
+            return;
+          }
+        }
+  }
+};
+
+// natural (x, y, z) of sub-grid site (a, b, e) of colour c (see Geom)
+template <int LAT>
+__host__ __device__ __forceinline__ void natural_of(int c, int a, int b, int e, int &x, int &y, int &z) {
+  if constexpr (LAT == SPK_HIP_SQ_4N) {
+    x = 2 * a + ((b + c) & 1), y = b, z = 0;
+  } else if constexpr (LAT == SPK_HIP_SC_6N) {
+    x = 2 * a + ((b + e + c) & 1), y = b, z = e;
+  } else if constexpr (LAT == SPK_HIP_SQ_8N) {
+    x = 2 * a + (c & 1), y = 2 * b + (c >> 1), z = 0;
+  } else {
+    x = 2 * a + (c & 1), y = 2 * b + ((c >> 1) & 1), z = 2 * e + (c >> 2);
+  }
+}
+
+// neighbour values of natural site (x, y, z) in the reference's order
+template <int LAT, typename T>
+__host__ __device__ __forceinline__ void gather_ordered(const T *__restrict__ spin, const Geom &g, int x, int y,
+                                                        int z, int32_t (&nb)[NeighCount<LAT>::value]) {
+#pragma unroll
+  for (int m = 0; m < NeighCount<LAT>::value; m++) {
+    int dx = 0, dy = 0, dz = 0;
+    Shell<LAT>::offset(m, dx, dy, dz);
+    const int xi = dx < 0 ? wrap_dec(x, g.nx) : dx > 0 ? wrap_inc(x, g.nx) : x;
+    const int yj = dy < 0 ? wrap_dec(y, g.ny) : dy > 0 ? wrap_inc(y, g.ny) : y;
+    const int zk = dz < 0 ? wrap_dec(z, g.nz) : dz > 0 ? wrap_inc(z, g.nz) : z;
+    nb[m] = (int32_t)spin[blocked_index(g, xi, yj, zk)];
+  }
+}
+
+// One strict event of a proposal variant, on ordered neighbour values:
+//   potts/neigh     AppPottsNeigh::site_event_rejection
+//                   (src/app_potts_neigh.cpp:71-116): draw
+//                   iran = (int)(maxneigh*u); no event if iran >= nevent
+//   potts/neighonly AppPottsNeighOnly::site_event_rejection
+//                   (src/app_potts_neighonly.cpp:71-118): no draw when no
+//                   neighbour is unlike; iran = (int)(nevent*u), clamped
+//   ising/single    AppIsingSingle::site_event_rejection
+//                   (src/app_ising_single.cpp:39-77): propose the flip
+// then the shared Metropolis tail (metropolis).  unique[iran] is found
+// without a list: neighbour m is a new distinct value when it is unlike the
+// old spin and unlike every earlier neighbour.
+template <int APP, int NN>
+__host__ __device__ __forceinline__ SiteOutcome variant_event(const Rule &r, const int *thr, int32_t old,
+                                                              int32_t seed, const int32_t (&nb)[NN]) {
+  int32_t s = seed;
+  int32_t prop = old;
+  bool event = true;
+  if constexpr (APP == SPK_HIP_ISING_SINGLE) {
+    prop = old == 1 ? 2 : 1;
+  } else {
+    bool fresh[NN];
+    int nevent = 0;
+#pragma unroll
+    for (int m = 0; m < NN; m++) {
+      bool f = nb[m] != old;
+#pragma unroll
+      for (int i = 0; i < m; i++) f = f && nb[i] != nb[m];
+      fresh[m] = f;
+      nevent += f;
+    }
+    int iran;
+    if constexpr (APP == SPK_HIP_POTTS_NEIGH) {
+      s = pm_step(s);
+      iran = (int)((double)NN * (kAM * (double)s));
+      event = iran < nevent;
+    } else {
+      event = nevent > 0;
+      if (event) {
+        s = pm_step(s);
+        iran = (int)((double)nevent * (kAM * (double)s));
+        if (iran >= nevent) iran = nevent - 1;
+      }
+    }
+    if (event) {
+      int n = 0;
+#pragma unroll
+      for (int m = 0; m < NN; m++) {
+        if (fresh[m] && n == iran) prop = nb[m];
+        n += fresh[m];
+      }
+    }
+  }
+  SiteOutcome o{old, s, 0, 0};
+  if (!event) return o;
+  int ei = 0, ef = 0;
+#pragma unroll
+  for (int m = 0; m < NN; m++) {
+    ei += nb[m] != old;
+    ef += nb[m] != prop;
+  }
+  const bool accept = metropolis(r, thr, ei, ef, s);
+  o.seed = s;
+  o.changed = accept && prop != old;
+  o.spin = o.changed ? prop : old;
+  o.dE = o.changed ? ef - ei : 0;
+  return o;
+}
+
+// Slot t of colour c (pad slots do nothing): one variant event.
+template <int LAT, typename T, int APP>
+__host__ __device__ __forceinline__ Counts variant_site(T *__restrict__ spin, int32_t *__restrict__ seed,
+                                                        const Geom &g, const Rule &r, const int *thr, int c,
+                                                        uint32_t t) {
+  Counts out{0, 0};
+  int a0, b, e;
+  pack_site(g, t >> 2, a0, b, e);
+  const int a = a0 + (int)(t & 3);
+  if (a >= g.hx) return out;
+  int x, y, z;
+  natural_of<LAT>(c, a, b, e, x, y, z);
+  int32_t nb[NeighCount<LAT>::value];
+  gather_ordered<LAT, T>(spin, g, x, y, z, nb);
+  const int64_t self = (int64_t)c * g.Mp + a + (int64_t)g.px * (b + (int64_t)g.hy * e);
+  const SiteOutcome o = variant_event<APP, NeighCount<LAT>::value>(r, thr, (int32_t)spin[self], seed[self], nb);
+  seed[self] = o.seed;
+  if (o.changed) spin[self] = (T)o.spin;
+  out.nacc = o.changed;
+  out.de = o.dE;
+  return out;
+}
+
 // ------------------------------------------------ vectorised colour pass
 // Each thread owns 4 consecutive sites of one sub-grid row (px % 4 == 0, so
 // a thread's sites never straddle a row).  Their spins, and each of their
@@ -560,11 +719,6 @@ struct Tally {
     if constexpr (NW == 4) return (int)p[v];
     else return (int)((p[v / Lanes<T>::PER] >> (Lanes<T>::BITS * (v % Lanes<T>::PER))) & Lanes<T>::MASK);
   }
-};
-
-struct Counts {
-  int nacc;   // accepted flips
-  int de;     // sum of (ef - ei) over accepted flips
 };
 
 // Strict events for sites a0..a0+3 of sub-grid row (b, e) of colour c, in

@@ -21,8 +21,17 @@ int make_rule(int app, int nspins, int lattice, double temperature, Rule *r);
 enum SpinType { kU8 = 0, kU16 = 1, kI32 = 2 };
 
 // narrowest storage holding 1..Q
+// Largest spin value of an app: Q for the Potts styles, 2 for the Ising ones.
+inline bool potts_family(int app) {
+  return app == SPK_HIP_POTTS || app == SPK_HIP_POTTS_NEIGH || app == SPK_HIP_POTTS_NEIGHONLY;
+}
+inline int app_qmax(int app, int nspins) { return potts_family(app) ? nspins : 2; }
+// potts and ising run the packed per-colour kernels; the proposal variants
+// run the ordered one-site-per-thread kernel
+inline bool variant_app(int app) { return app >= SPK_HIP_POTTS_NEIGH && app <= SPK_HIP_ISING_SINGLE; }
+
 inline int spin_type_for(int app, int nspins) {
-  const int qmax = app == SPK_HIP_POTTS ? nspins : 2;
+  const int qmax = app_qmax(app, nspins);
   return qmax <= 255 ? kU8 : qmax <= 65535 ? kU16 : kI32;
 }
 inline size_t spin_bytes(int stype) { return stype == kU8 ? 1 : stype == kU16 ? 2 : 4; }

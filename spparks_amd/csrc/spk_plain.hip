@@ -67,6 +67,23 @@ __global__ __launch_bounds__(1024) void k_sweep_color(T *__restrict__ spin, int3
   block_accumulate(cnt.nacc, cnt.de, acc, blockIdx.x & (kAccSlots - 1));
 }
 
+// One colour pass of a proposal variant (potts/neigh, potts/neighonly,
+// ising/single): one slot per thread (4 consecutive slots per pack, so a
+// wavefront's own-site accesses stay unit-stride), neighbours read in the
+// reference's order.
+template <int LAT, typename T, int APP>
+__global__ __launch_bounds__(1024) void k_sweep_variant(T *__restrict__ spin, int32_t *__restrict__ seed, Geom g,
+                                                        Rule r, int color, unsigned long long *__restrict__ acc) {
+  __shared__ int s_thr[32];
+  if (threadIdx.x < 28) s_thr[threadIdx.x] = r.thr[threadIdx.x];
+  __syncthreads();
+  const int tile = xcd_tile(blockIdx.x, gridDim.x);
+  const int64_t t = (int64_t)tile * blockDim.x + threadIdx.x;
+  Counts cnt{0, 0};
+  if (t < (int64_t)g.Mp) cnt = variant_site<LAT, T, APP>(spin, seed, g, r, s_thr, color, (uint32_t)t);
+  block_accumulate(cnt.nacc, cnt.de, acc, blockIdx.x & (kAccSlots - 1));
+}
+
 // S whole sweeps of a 2d lattice per launch, one extended tile per
 // workgroup in LDS (see FusedTile).  Reads one state buffer, writes the
 // other; per-sweep Naccept / dE go to acc rows 0..S-1.
@@ -193,7 +210,27 @@ constexpr int kVec = 4;   // sites per thread in the sweep kernel
 
 unsigned grid_for(int64_t n, int bs) { return (unsigned)((n + bs - 1) / bs); }
 
+int launch_variant(spk_hip_lattice *h, int color, unsigned long long *acc) {
+  const int bs = h->block;
+  unsigned grid = grid_for(h->g.Mp, bs);
+  grid = (grid + kNumXcd - 1) / kNumXcd * kNumXcd;   // xcd_tile needs a multiple of 8
+  const int app = h->d.app;
+  with_lattice(h->g.lat, [&]<int LAT>() {
+    with_type(h->stype, [&]<typename T>() {
+      auto go = [&](auto kern) {
+        hipLaunchKernelGGL(kern, dim3(grid), dim3(bs), 0, h->stream, static_cast<T *>(h->spin), h->seed, h->g,
+                           h->rule, color, acc);
+      };
+      if (app == SPK_HIP_POTTS_NEIGH) go(k_sweep_variant<LAT, T, SPK_HIP_POTTS_NEIGH>);
+      else if (app == SPK_HIP_POTTS_NEIGHONLY) go(k_sweep_variant<LAT, T, SPK_HIP_POTTS_NEIGHONLY>);
+      else if constexpr (sizeof(T) == 1) go(k_sweep_variant<LAT, T, SPK_HIP_ISING_SINGLE>);   // two states: u8
+    });
+  });
+  return check_hip(hipGetLastError(), "sweep kernel launch");
+}
+
 int launch_sweep(spk_hip_lattice *h, int color, unsigned long long *acc) {
+  if (variant_app(h->d.app)) return launch_variant(h, color, acc);
   const int bs = h->block;
   const int64_t threads = (h->g.Mp / kVec + h->packs - 1) / h->packs;
   unsigned grid = grid_for(threads, bs);
@@ -232,6 +269,7 @@ constexpr size_t kFusedLdsMax = 64 * 1024;
 void choose_fusion(spk_hip_lattice *h) {
   h->fuse = 0;
   if (h->g.lat != SPK_HIP_SQ_4N && h->g.lat != SPK_HIP_SQ_8N) return;
+  if (variant_app(h->d.app)) return;   // fused tiles run the potts / ising rule only
   const int S = h->g.lat == SPK_HIP_SQ_4N ? 4 : 2;
   const int H = (h->g.lat == SPK_HIP_SQ_4N ? 2 : 4) * S;
   const size_t per = 4 + spin_bytes(h->stype);
@@ -518,7 +556,7 @@ int spk_hip_upload(spk_hip_lattice *h, const int32_t *spin, const int32_t *seed)
   if (!h) return set_error(SPK_HIP_EINVAL, "null handle");
   const size_t bytes = (size_t)h->N * sizeof(int32_t);
   int rc;
-  const int qmax = h->d.app == SPK_HIP_POTTS ? h->d.nspins : 2;
+  const int qmax = app_qmax(h->d.app, h->d.nspins);
   if (spin) {
     if ((rc = check_hip(hipMemcpyAsync(h->stage, spin, bytes, hipMemcpyHostToDevice, h->stream), "upload")))
       return rc;
@@ -574,7 +612,7 @@ int spk_hip_set_range(spk_hip_lattice *h, int seed0, int lo, int hi) {
   if (!h) return set_error(SPK_HIP_EINVAL, "null handle");
   if (hi < lo) return set_error(SPK_HIP_EINVAL, "set range: hi < lo");
   if (seed0 < 1 || seed0 >= kIM) return set_error(SPK_HIP_EINVAL, "set range: seed0 outside [1, 2^31-2]");
-  const int qmax = h->d.app == SPK_HIP_POTTS ? h->d.nspins : 2;
+  const int qmax = app_qmax(h->d.app, h->d.nspins);
   if (lo < 1 || hi > qmax)
     return set_error(SPK_HIP_EVALUE, "One or more sites have invalid values (set range %d %d, Q = %d)", lo, hi,
                      qmax);
@@ -591,7 +629,7 @@ int spk_hip_set_range(spk_hip_lattice *h, int seed0, int lo, int hi) {
 
 int spk_hip_set_value(spk_hip_lattice *h, int value) {
   if (!h) return set_error(SPK_HIP_EINVAL, "null handle");
-  const int qmax = h->d.app == SPK_HIP_POTTS ? h->d.nspins : 2;
+  const int qmax = app_qmax(h->d.app, h->d.nspins);
   if (value < 1 || value > qmax)
     return set_error(SPK_HIP_EVALUE, "One or more sites have invalid values (set value %d, Q = %d)", value, qmax);
   int rc = launch_fill_spins(h, value);

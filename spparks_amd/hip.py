@@ -13,7 +13,7 @@ import numpy as np
 LIB_DIR = os.path.join(os.path.dirname(os.path.abspath(__file__)), "lib")
 LIB_PATH = os.path.join(LIB_DIR, "libspk_hip.so")
 
-APPS = {"potts": 0, "ising": 1}
+APPS = {"potts": 0, "ising": 1, "potts/neigh": 2, "potts/neighonly": 3, "ising/single": 4}
 LATTICES = {"sq/4n": 0, "sq/8n": 1, "sc/6n": 2, "sc/26n": 3}
 MAXNEIGH = {"sq/4n": 4, "sq/8n": 8, "sc/6n": 6, "sc/26n": 26}
 

@@ -98,7 +98,8 @@ static int run(int app, int nx, int ny, int nz, int q, double kt, int nsw, const
   const long long N = (long long)nx * ny * nz;
   const long long S = (long long)g.K * g.Mp;   // stored slots, row pads included
   Rule r{};
-  r.app = app; r.nspins = app == SPK_HIP_POTTS ? q : 2; r.zero_t = kt == 0.0;
+  r.app = app; r.nspins = (app == SPK_HIP_POTTS || app == SPK_HIP_POTTS_NEIGH || app == SPK_HIP_POTTS_NEIGHONLY) ? q : 2;
+  r.zero_t = kt == 0.0;
   int32_t thr[27];
   spk_hip_metropolis_table(kt > 0 ? kt : 1.0, maxneigh_of(LAT), nullptr, thr);
   r.half = thr[0];
@@ -143,6 +144,28 @@ static int run(int app, int nx, int ny, int nz, int q, double kt, int nsw, const
         eng[w + q] = e;
       }
       if (e != energy<LAT, T>(s, g)) { fprintf(stderr, "incremental energy drifted (fused)\n"); return 4; }
+    }
+  }
+  if (app >= SPK_HIP_POTTS_NEIGH) {
+    // proposal variants: the ordered one-site-per-slot path
+    for (; w < nsw; w++) {
+      long long de = 0;
+      for (int c = 0; c < g.K; c++)
+        for (int t = 0; t < g.Mp; t++) {
+          Counts cnt{0, 0};
+          if (app == SPK_HIP_POTTS_NEIGH)
+            cnt = variant_site<LAT, T, SPK_HIP_POTTS_NEIGH>(s.data(), k.data(), g, r, r.thr, c, (uint32_t)t);
+          else if (app == SPK_HIP_POTTS_NEIGHONLY)
+            cnt = variant_site<LAT, T, SPK_HIP_POTTS_NEIGHONLY>(s.data(), k.data(), g, r, r.thr, c, (uint32_t)t);
+          else
+            cnt = variant_site<LAT, T, SPK_HIP_ISING_SINGLE>(s.data(), k.data(), g, r, r.thr, c, (uint32_t)t);
+          na += cnt.nacc;
+          de += cnt.de;
+        }
+      e += 2 * de;
+      if (e != energy<LAT, T>(s, g)) { fprintf(stderr, "incremental energy drifted (variant)\n"); return 4; }
+      nacc[w] = na;
+      eng[w] = e;
     }
   }
   for (; w < nsw; w++) {

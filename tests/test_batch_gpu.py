@@ -120,3 +120,11 @@ def test_batch_upload_then_run(orc):
         nacc, eng = orc.sweeps(d, spin, seeds, 4)
         gs, gk = B.download(rid)
     assert np.array_equal(gs, spin) and np.array_equal(gk, seeds)
+
+
+def test_batch_refuses_proposal_variants():
+    from spparks_amd import batch as bmod
+    import spparks_amd as S
+    with bmod.Batch(8 << 20, 4) as B:
+        with pytest.raises(S.HipError, match="potts and ising"):
+            B.add(bmod.replica_desc("potts/neigh", "sq/8n", 8, 8, 1, nspins=5, temperature=0.5, seed=3, nsweeps=2))

@@ -44,6 +44,20 @@ CASES = [
     # Potts with two states takes the two-state shortcut too
     ("potts", "sq/4n", (10, 6, 1), 2, 0.8, 41, 4),
     ("potts", "sc/26n", (6, 4, 2), 2, 1.2, 42, 4),
+    # proposal variants (ordered neighbour path), every lattice, edge-2 boxes
+    # (duplicate neighbours), T = 0
+    ("potts/neigh", "sq/4n", (10, 6, 1), 5, 0.8, 51, 4),
+    ("potts/neigh", "sq/8n", (2, 6, 1), 4, 0.6, 52, 4),
+    ("potts/neigh", "sc/26n", (6, 4, 4), 300, 0.5, 53, 3),
+    ("potts/neigh", "sc/6n", (4, 4, 2), 7, 0.0, 54, 3),
+    ("potts/neighonly", "sq/8n", (16, 12, 1), 100, 0.5, 55, 4),
+    ("potts/neighonly", "sc/26n", (2, 2, 2), 3, 0.7, 56, 4),
+    ("potts/neighonly", "sq/4n", (18, 10, 1), 70000, 1.1, 57, 3),
+    ("potts/neighonly", "sc/6n", (8, 6, 4), 9, 0.0, 58, 3),
+    ("ising/single", "sq/4n", (64, 48, 1), 2, 2.269, 59, 5),
+    ("ising/single", "sc/26n", (8, 6, 4), 2, 3.0, 60, 4),
+    ("ising/single", "sq/8n", (12, 12, 1), 2, 0.0, 61, 4),
+    ("ising/single", "sc/6n", (2, 4, 2), 2, 1.5, 62, 4),
     # byte lanes holding values >= 0x80 (u8 storage up to Q = 255)
     ("potts", "sc/26n", (6, 4, 4), 255, 0.6, 43, 4),
     ("potts", "sq/8n", (10, 6, 1), 200, 0.7, 44, 4),
@@ -53,7 +67,7 @@ CASES = [
 def storage_variants(app, q, nx):
     """(stype, vec) pairs the library may pick for this case, plus wider
     storage types, which must give the same trajectory."""
-    qmax = q if app == "potts" else 2
+    qmax = q if app.startswith("potts") else 2
     narrow = 0 if qmax <= 255 else 1 if qmax <= 65535 else 2
     return [(t, 4) for t in range(narrow, 3)]
 

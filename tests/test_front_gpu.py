@@ -161,3 +161,42 @@ def test_invalid_values_error(tmp_path):
         r = subprocess.run([EXE, "-in", str(p), "-log", "none"], capture_output=True, text=True, cwd=tmp_path,
                            timeout=60)
         assert r.returncode != 0 and msg in r.stdout + r.stderr
+
+
+@pytest.mark.parametrize("app,q,lat,dims,run,nsw", [
+    # potts/neigh[only]: dt_sweep = 1/maxneigh, so run 1.0 on sq/8n is 8 sweeps
+    ("potts/neigh", 50, "sq/8n", (16, 12, 1), "1.0", 8),
+    ("potts/neighonly", 500, "sc/26n", (8, 6, 4), "0.2", 6),
+    # ising/single: dt_sweep = 1
+    ("ising/single", 0, "sq/4n", (20, 14, 1), "7.0", 7),
+])
+def test_variant_scripts_match_oracle(orc, tmp_path, app, q, lat, dims, run, nsw):
+    """Unchanged scripts of the proposal variants: the sweep count follows the
+    variant's dt_sweep, and every stats row (one per sweep) matches the
+    oracle's Naccept and energy."""
+    nx, ny, nz = dims
+    three = lat.startswith("sc")
+    style = "%s %d" % (app, q) if app.startswith("potts") else app
+    qq = q if app.startswith("potts") else 2
+    region = "region box block 0 %d 0 %d %s\n" % (nx, ny, ("0 %d" % nz) if three else "-0.5 0.5")
+    dt = {"potts/neigh": 1.0 / (26 if lat == "sc/26n" else 8), "potts/neighonly": 1.0 / (26 if lat == "sc/26n" else 8),
+          "ising/single": 1.0}[app]
+    text = ("seed 4321\napp_style %s\ndimension %d\nlattice %s 1.0\n%s"
+            "create_box box\ncreate_sites box\nset site range 1 %d\nsweep color/strict\n"
+            "temperature 0.6\ndiag_style energy\nstats %r\nrun %s\n"
+            % (style, 3 if three else 2, lat, region, qq, dt, run))
+    p = tmp_path / "in.variant"
+    p.write_text(text)
+    rows = stats_rows(run_exe(tmp_path, str(p)))
+    assert rows[-1][3] == nsw
+    d = orc.make_desc(app, lat, nx, ny, nz, qq, 0.6)
+    spin, seeds = orc.script_init(d, 4321, 1, qq)
+    e0 = orc.energy(d, spin)
+    nacc, eng = orc.sweeps(d, spin, seeds, nsw)
+    n = nx * ny * nz
+    assert rows[0][1] == 0 and rows[0][4] == e0
+    assert len(rows) >= 3
+    for r in rows[1:]:
+        k = r[3]   # the row's sweep count
+        assert r[1] == nacc[k - 1] and r[1] + r[2] == k * n
+        assert r[4] == eng[k - 1]

@@ -85,6 +85,44 @@ def test_small_lattice_bit_exact(orc, case, packs):
     assert tot == nacc[-1]
 
 
+VARIANT_CASES = [
+    # proposal variants on the ordered neighbour path: every lattice, every
+    # storage type, edge-2 boxes (duplicate neighbours), T = 0
+    ("potts/neigh", "sq/8n", (16, 12, 1), 100, 0.5, 71, 6),
+    ("potts/neigh", "sq/4n", (18, 10, 1), 5, 1.1, 72, 6),
+    ("potts/neigh", "sc/26n", (10, 8, 6), 10000, 0.5, 73, 4),
+    ("potts/neigh", "sc/6n", (8, 6, 4), 9, 0.0, 74, 4),
+    ("potts/neigh", "sq/8n", (2, 2, 1), 3, 0.8, 75, 5),
+    ("potts/neighonly", "sq/8n", (34, 66, 1), 70000, 0.5, 76, 4),
+    ("potts/neighonly", "sc/26n", (2, 2, 2), 4, 0.6, 77, 5),
+    ("potts/neighonly", "sq/4n", (64, 48, 1), 7, 0.7, 78, 8),
+    ("potts/neighonly", "sc/6n", (4, 4, 2), 2, 0.0, 79, 4),
+    ("ising/single", "sq/4n", (64, 48, 1), 2, 2.269, 80, 20),
+    ("ising/single", "sc/26n", (8, 6, 4), 2, 3.0, 81, 5),
+    ("ising/single", "sq/8n", (12, 12, 1), 2, 0.0, 82, 4),
+    ("ising/single", "sc/6n", (16, 4, 4), 2, 1.7, 83, 5),
+]
+
+
+@pytest.mark.parametrize("case", VARIANT_CASES, ids=lambda c: "%s-%s-%s" % (c[0], c[1], "x".join(map(str, c[2]))))
+def test_variant_bit_exact(orc, case):
+    """potts/neigh, potts/neighonly and ising/single against the oracle:
+    spins, seeds, cumulative Naccept and energy after every sweep."""
+    import spparks_amd as S
+    app, lat, shape, q, kt, seed, n = case
+    say("variant case", case)
+    spin0, seeds0, spin1, seeds1, nacc, eng = oracle_run(orc, app, lat, shape, q, kt, seed, n)
+    with gpu_init(S, app, lat, shape, q, kt, seed) as L:
+        gs0, gk0 = L.download()
+        assert np.array_equal(gs0, spin0) and np.array_equal(gk0, seeds0)
+        tot, gn, ge = L.sweeps(n)
+        gs1, gk1 = L.download()
+    assert np.array_equal(gs1, spin1)
+    assert np.array_equal(gk1, seeds1)
+    assert np.array_equal(gn, nacc)
+    assert np.array_equal(ge, eng)
+
+
 def test_cfg1_known_answers_on_gpu():
     import spparks_amd as S
     g = json.load(open(os.path.join(GOLD, "cfg1_known_answers.json")))

@@ -238,3 +238,121 @@ def test_ising_is_potts_q2(orc, lat, shape):
     np_, ep = orc.sweeps(dp, sp, kp, 6)
     assert np.array_equal(si, sp) and np.array_equal(ki, kp)
     assert np.array_equal(ni, np_) and np.array_equal(ei, ep)
+
+
+@pytest.mark.parametrize("lat", ["sq/4n", "sq/8n", "sc/6n", "sc/26n"])
+def test_stencil_order_is_reference_loop_order(orc, lat):
+    """The neighbour list order is what offsets_2d / offsets_3d produce
+    (src/create_sites.cpp:1303-1371): dx outermost, then dy, then dz, each
+    -1..1, kept when the distance is within the lattice's shell.  Order
+    matters for potts/neigh and potts/neighonly (first-appearance lists)."""
+    import math
+    is3 = lat.startswith("sc")
+    cut = {"sq/4n": 1.0, "sc/6n": 1.0, "sq/8n": math.sqrt(2.0), "sc/26n": math.sqrt(3.0)}[lat]
+    want = []
+    for i in (-1, 0, 1):
+        for j in (-1, 0, 1):
+            for k in ((-1, 0, 1) if is3 else (0,)):
+                r = math.sqrt(i * i + j * j + k * k)
+                if 1.0 - 1e-6 < r < cut + 1e-6:
+                    want.append((i, j, k))
+    nx, ny, nz = (6, 6, 6) if is3 else (6, 6, 1)
+    d = orc.make_desc("potts", lat, nx, ny, nz, 3, 1.0)
+    nb = orc.neighbors(d)
+    g = 2 + nx * (3 + ny * (2 if is3 else 0))   # an interior site: no wrap
+    i0, j0, k0 = 2, 3, (2 if is3 else 0)
+    got = [((n % nx) - i0, ((n // nx) % ny) - j0, (n // (nx * ny)) - k0) for n in nb[g]]
+    assert got == want
+
+
+def _unique_unlike(vals, me):
+    out = []
+    for v in vals:
+        if v != me and v not in out:
+            out.append(v)
+    return out
+
+
+@pytest.mark.parametrize("app", ["potts/neigh", "potts/neighonly", "ising/single"])
+@pytest.mark.parametrize("lat", ["sq/4n", "sq/8n", "sc/26n"])
+@pytest.mark.parametrize("kt", [0.9, 0.0])
+def test_variant_site_event_by_hand(orc, app, lat, kt):
+    """One event of each proposal variant recomputed from its definition with
+    Python ints and math.exp (src/app_potts_neigh.cpp:71-116,
+    src/app_potts_neighonly.cpp:71-118, src/app_ising_single.cpp:39-77):
+    which value is proposed, how many draws are taken (none for an
+    ising/single downhill flip or a potts/neighonly site with no unlike
+    neighbour), and the Metropolis test."""
+    import math
+    AM = 1.0 / 2147483647
+    step = lambda s: (16807 * s) % (2**31 - 1)
+    is3 = lat.startswith("sc")
+    shape = (4, 4, 4) if is3 else (4, 4, 1)
+    q = 2 if app == "ising/single" else 5
+    d = orc.make_desc(app, lat, *shape, q, kt)
+    nb = orc.neighbors(d)
+    ns = nb.shape[1]
+    n = nb.shape[0]
+    rng = np.random.default_rng(11)
+    for trial in range(400):
+        # low-entropy neighbourhoods too, so nevent = 0 and 1 both occur
+        hi = q if trial % 3 else 2
+        spin = rng.integers(1, hi + 1, n).astype(np.int32)
+        g = int(rng.integers(0, n))
+        s0 = int(rng.integers(1, 2**31 - 1))
+        old = int(spin[g])
+        vals = [int(spin[j]) for j in nb[g]]
+        ei = sum(v != old for v in vals)
+        s = s0
+        prop = None
+        if app == "ising/single":
+            prop = 2 if old == 1 else 1
+        else:
+            uniq = _unique_unlike(vals, old)
+            if app == "potts/neigh":
+                s = step(s)
+                iran = int(ns * (AM * s))
+                if iran < len(uniq):
+                    prop = uniq[iran]
+            elif uniq:
+                s = step(s)
+                iran = min(int(len(uniq) * (AM * s)), len(uniq) - 1)
+                prop = uniq[iran]
+        new = old
+        if prop is not None:
+            ef = sum(v != prop for v in vals)
+            new = prop
+            if ef > ei:
+                if kt == 0.0:
+                    new = old
+                else:
+                    s = step(s)
+                    if AM * s > math.exp((ei - ef) * (1.0 / kt)):
+                        new = old
+        seed = C.c_int(s0)
+        changed = orc.lib().orc_site_event(C.byref(d), spin, g, C.byref(seed))
+        assert seed.value == s, (trial, app)
+        assert spin[g] == new, (trial, app)
+        assert changed == int(new != old)
+
+
+def test_variants_energy_and_draw_invariants(orc):
+    """potts/neighonly never draws at a site with no unlike neighbour, so a
+    uniform lattice keeps every seed; potts/neigh draws exactly once there;
+    energies stay the bond count for all three variants."""
+    for app, q in (("potts/neighonly", 4), ("potts/neigh", 4)):
+        d = orc.make_desc(app, "sc/26n", 4, 4, 4, q, 0.7)
+        spin = np.full(64, 3, np.int32)
+        seeds = np.arange(1, 65, dtype=np.int32) * 1000003
+        s0 = seeds.copy()
+        nacc, _ = orc.sweeps(d, spin, seeds, 2)
+        assert nacc[-1] == 0 and (spin == 3).all()
+        k = 0 if app == "potts/neighonly" else 2
+        a = pow(16807, k, 2**31 - 1)
+        assert np.array_equal(seeds.astype(np.int64), (s0.astype(np.int64) * a) % (2**31 - 1))
+    for app, q in (("potts/neigh", 30), ("potts/neighonly", 30), ("ising/single", 2)):
+        d = orc.make_desc(app, "sq/8n", 12, 10, 1, q, 0.8)
+        spin, seeds = orc.script_init(d, 31337, 1, q)
+        nacc, eng = orc.sweeps(d, spin, seeds, 4)
+        assert eng[-1] == _bonds_energy(orc, d, spin)
+        assert (np.diff(nacc) >= 0).all() and nacc[-1] > 0
