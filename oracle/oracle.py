@@ -116,6 +116,19 @@ def pm_uniform_seq(seed, n):
     return u, st
 
 
+def master_draws(seed, n):
+    """The first n uniforms of the master generator of `seed S`."""
+    r = RanMars(seed)
+    return [r.uniform() for _ in range(n)]
+
+
+def site_seeds(rseed, n):
+    """Strict per-site seeds of a first run whose seed draw is rseed."""
+    out = np.empty(n, np.int32)
+    lib().orc_site_seeds(float(rseed), int(n), out)
+    return out
+
+
 def script_init(desc, master_seed, lo, hi):
     n = desc.nx * desc.ny * desc.nz
     spin = np.empty(n, np.int32)

@@ -9,6 +9,7 @@
 #include <cstdarg>
 #include <cstdlib>
 #include <cstring>
+#include <sstream>
 
 namespace spkfront {
 
@@ -189,6 +190,7 @@ std::string Spparks::one(const std::string &raw) {
   else if (cmd == "diag_style") cmd_diag_style(a);
   else if (cmd == "stats") cmd_stats(a);
   else if (cmd == "dump") cmd_dump(a);
+  else if (cmd == "read_sites") cmd_read_sites(a);
   else if (cmd == "undump") cmd_undump(a);
   else if (cmd == "run") cmd_run(a);
   else if (cmd == "variable") cmd_variable(a);
@@ -252,6 +254,7 @@ void Spparks::cmd_app_style(const std::vector<std::string> &a) {
   }
   nsites_ = 0;
   values_set_ = false;
+  seeds_loaded_ = false;
   sweepflag_ = 0;
   mask_ = false;
   temperature_ = 0.0;
@@ -476,26 +479,41 @@ void Spparks::cmd_stats(const std::vector<std::string> &a) {
   }
 }
 
-// dump ID text delta file [fields] (src/dump.cpp:41-130, src/dump_text.cpp)
+// dump ID text|sites delta file [fields] (src/dump.cpp:41-130,
+// src/dump_text.cpp:40-130, src/dump_sites.cpp:25-48).  A '*' in the file
+// name writes one file per snapshot, '*' replaced by the snapshot index.
+// Field `seed` (the site's strict RNG state) is this build's addition: with
+// it a `dump sites` file is a checkpoint that read_sites restores exactly.
 void Spparks::cmd_dump(const std::vector<std::string> &a) {
   if (a.size() < 4) throw Error("Illegal dump command");
-  if (a[1] != "text") throw Error("dump style " + a[1] + " is outside this build (text)");
+  if (a[1] != "text" && a[1] != "sites")
+    throw Error("dump style " + a[1] + " is outside this build (text, sites)");
   if (!box_exist_) throw Error("Dump command before simulation box is defined");
   for (auto &d : dumps_)
     if (d.id == a[0]) throw Error("Reuse of dump ID");
   Dump d;
   d.id = a[0];
+  d.style = a[1];
   d.delta = to_double(a[2], "dump");
   if (d.delta <= 0.0) throw Error("Illegal dump command");
   d.file = a[3];
+  d.multifile = d.file.find('*') != std::string::npos;
   if (a.size() == 4) d.fields = {"id", "site", "x", "y", "z"};
   for (size_t i = 4; i < a.size(); i++) {
-    if (a[i] != "id" && a[i] != "site" && a[i] != "x" && a[i] != "y" && a[i] != "z" && a[i] != "energy")
-      throw Error("Invalid attribute " + a[i] + " in dump text command");
-    d.fields.push_back(a[i]);
+    const std::string &f = a[i];
+    if (f != "id" && f != "site" && f != "i1" && f != "x" && f != "y" && f != "z" && f != "energy" && f != "seed")
+      throw Error("Invalid attribute " + f + " in dump " + d.style + " command");
+    d.fields.push_back(f);
   }
-  d.fp = std::fopen(d.file.c_str(), "w");
-  if (!d.fp) throw Error("Cannot open dump file");
+  if (d.style == "sites") {
+    if (!d.multifile) throw Error("Dump sites must write one file per snapshot");
+    if (d.fields[0] != "id") throw Error("Dump sites requires ID as first field");
+    if (d.fields.size() < 2) throw Error("Dump sites requires two or more fields");
+  }
+  if (!d.multifile) {
+    d.fp = std::fopen(d.file.c_str(), "w");
+    if (!d.fp) throw Error("Cannot open dump file");
+  }
   dumps_.push_back(std::move(d));
 }
 
@@ -552,7 +570,7 @@ void Spparks::init_run() {
   if (!ranstrict_made_) {       // ranstrict: RandomPark(uniform()); seed = uniform()
     ranmaster_->uniform();
     const double rseed = ranmaster_->uniform();
-    hip(spk_hip_init_seeds(h_, rseed));
+    if (!seeds_loaded_) hip(spk_hip_init_seeds(h_, rseed));   // a checkpoint's seeds are kept
     ranstrict_made_ = true;
   }
   // init_app: site-value check; potts/neigh[only] take dt_sweep = 1/maxneigh
@@ -609,22 +627,45 @@ void Spparks::download() {
   hip(spk_hip_download(h_, host_spin_.data(), host_seed_.data()));
 }
 
-// DumpText::write (src/dump_text.cpp:384-392 header; one line per site)
+// Dump::write (src/dump.cpp:250-356): open the snapshot's file when there
+// is one per snapshot, the style's header (DumpText::write_header,
+// src/dump_text.cpp:380-392; DumpSites::write_header,
+// src/dump_sites.cpp:52-66), then one line per site in id order, each value
+// followed by a space (DumpText::write_text, src/dump_text.cpp:405-421).
 void Spparks::write_dump(Dump &d) {
   download();
-  std::vector<double> eng;
-  std::fprintf(d.fp, "ITEM: TIMESTEP\n%d %10g\n", d.idump, time_);
-  std::fprintf(d.fp, "ITEM: NUMBER OF ATOMS\n%lld\n", (long long)nsites_);
-  std::fprintf(d.fp, "ITEM: BOX BOUNDS\n%g %g\n%g %g\n%g %g\n", boxlo_[0], boxhi_[0], boxlo_[1], boxhi_[1],
-               boxlo_[2], boxhi_[2]);
-  std::string cols;
-  for (size_t i = 0; i < d.fields.size(); i++) cols += (i ? " " : "") + (d.fields[i] == "site" ? std::string("type") : d.fields[i]);
-  std::fprintf(d.fp, "ITEM: ATOMS %s\n", cols.c_str());
+  if (d.multifile) {
+    const size_t star = d.file.find('*');
+    const std::string name = d.file.substr(0, star) + std::to_string(d.idump) + d.file.substr(star + 1);
+    d.fp = std::fopen(name.c_str(), "w");
+    if (!d.fp) throw Error("Cannot open dump file");
+  }
+  std::string cols;   // "site" is written as "type" in text dumps, as is in site files
+  for (size_t i = 0; i < d.fields.size(); i++)
+    cols += (i ? " " : "") + ((d.fields[i] == "site" && d.style == "text") ? std::string("type") : d.fields[i]);
+  if (d.style == "text") {
+    std::fprintf(d.fp, "ITEM: TIMESTEP\n%d %10g\n", d.idump, time_);
+    std::fprintf(d.fp, "ITEM: NUMBER OF ATOMS\n%lld\n", (long long)nsites_);
+    std::fprintf(d.fp, "ITEM: BOX BOUNDS\n%g %g\n%g %g\n%g %g\n", boxlo_[0], boxhi_[0], boxlo_[1], boxhi_[1],
+                 boxlo_[2], boxhi_[2]);
+    std::fprintf(d.fp, "ITEM: ATOMS %s\n", cols.c_str());
+  } else {
+    std::fprintf(d.fp, "Site file written by dump sites %s command at time: %d %g\n\n", d.id.c_str(), d.idump,
+                 time_);
+    std::fprintf(d.fp, "%d dimension\n", dimension_);
+    std::fprintf(d.fp, "%lld sites\n", (long long)nsites_);
+    std::fprintf(d.fp, "%s values\n", cols.c_str());
+    std::fprintf(d.fp, "%g %g xlo xhi\n", boxlo_[0], boxhi_[0]);
+    std::fprintf(d.fp, "%g %g ylo yhi\n", boxlo_[1], boxhi_[1]);
+    std::fprintf(d.fp, "%g %g zlo zhi\n", boxlo_[2], boxhi_[2]);
+    std::fprintf(d.fp, "\nValues\n\n");
+  }
   for (int64_t g = 0; g < nsites_; g++) {
     const int64_t i = g % nx_, j = (g / nx_) % ny_, k = g / ((int64_t)nx_ * ny_);
     for (const auto &f : d.fields) {
       if (f == "id") std::fprintf(d.fp, "%lld ", (long long)(g + 1));
-      else if (f == "site") std::fprintf(d.fp, "%d ", host_spin_[g]);
+      else if (f == "site" || f == "i1") std::fprintf(d.fp, "%d ", host_spin_[g]);
+      else if (f == "seed") std::fprintf(d.fp, "%d ", host_seed_[g]);
       else if (f == "x") std::fprintf(d.fp, "%g ", i * latconst_ + boxlo_[0]);
       else if (f == "y") std::fprintf(d.fp, "%g ", j * latconst_ + boxlo_[1]);
       else if (f == "z") std::fprintf(d.fp, "%g ", dimension_ == 3 ? k * latconst_ + boxlo_[2] : 0.5 * (boxlo_[2] + boxhi_[2]));
@@ -647,8 +688,124 @@ void Spparks::write_dump(Dump &d) {
     }
     std::fprintf(d.fp, "\n");
   }
-  std::fflush(d.fp);
+  if (d.multifile) {
+    std::fclose(d.fp);
+    d.fp = nullptr;
+  } else {
+    std::fflush(d.fp);
+  }
   d.idump++;
+}
+
+// read_sites file (src/read_sites.cpp:69-186): the free-format header
+// (:199-314) checked against the existing box and sites, then the Values
+// section (:507-640), one line per site: id and its values, in the columns
+// of the header's `values` line (site or i1; default: the one integer per
+// site).  Sites come from create_sites here, so Sites and Neighbors sections
+// are outside this build.  Column `seed` is this build's addition (see
+// cmd_dump): it sets the strict seeds, and the first colour/strict run then
+// keeps them instead of seeding by tagreset.
+void Spparks::cmd_read_sites(const std::vector<std::string> &a) {
+  if (app_.empty()) throw Error("Read_sites command before app_style set");
+  if (a.size() != 1) throw Error("Illegal read_sites command");
+  message("Reading site file ...\n");
+  FILE *fp = std::fopen(a[0].c_str(), "r");
+  if (!fp) throw Error("Cannot open file " + a[0]);
+  struct Closer {
+    FILE *f;
+    ~Closer() { std::fclose(f); }
+  } closer{fp};
+  char line[4096];
+  auto blank = [](const char *l) { return std::strspn(l, " \t\n\r") == std::strlen(l); };
+  if (!std::fgets(line, sizeof line, fp)) throw Error("Unexpected end of data file");   // title line
+  double blo[3] = {-0.5, -0.5, -0.5}, bhi[3] = {0.5, 0.5, 0.5};
+  std::string columns;
+  bool have_columns = false;
+  std::string keyword;
+  while (true) {
+    if (!std::fgets(line, sizeof line, fp)) break;
+    if (char *h = std::strchr(line, '#')) *h = '\0';
+    if (blank(line)) continue;
+    if (std::strstr(line, "dimension")) {
+      int dim = 0;
+      std::sscanf(line, "%d", &dim);
+      if (box_exist_ && dim != dimension_) throw Error("Data file dimension does not match existing box");
+    } else if (std::strstr(line, "sites")) {
+      long long n = 0;
+      std::sscanf(line, "%lld", &n);
+      if (nsites_ && n != nsites_) throw Error("Data file number of sites does not match existing sites");
+    } else if (std::strstr(line, "max neighbors")) {
+      throw Error("read_sites Neighbors data is outside this build (sites come from create_sites)");
+    } else if (char *v = std::strstr(line, "values")) {
+      *v = '\0';
+      columns = line;
+      have_columns = true;
+    } else if (std::strstr(line, "xlo xhi") || std::strstr(line, "ylo yhi") || std::strstr(line, "zlo zhi")) {
+      const int dim = std::strstr(line, "xlo") ? 0 : std::strstr(line, "ylo") ? 1 : 2;
+      std::sscanf(line, "%lg %lg", &blo[dim], &bhi[dim]);
+      if (box_exist_ && (std::fabs(boxlo_[dim] - blo[dim]) > 1.0e-6 || std::fabs(boxhi_[dim] - bhi[dim]) > 1.0e-6))
+        throw Error("Data file simulation box different that current box");
+    } else {
+      // ReadSites::parse_keyword (:680-720): the whole trimmed line is the
+      // keyword, and the one line after it is skipped
+      keyword = line;
+      keyword.erase(0, keyword.find_first_not_of(" \t\n\r"));
+      keyword.erase(keyword.find_last_not_of(" \t\n\r") + 1);
+      if (!std::fgets(line, sizeof line, fp)) keyword.clear();
+      break;
+    }
+  }
+  if (keyword.empty()) throw Error("Site file has no Sites, Neighbors, or Values");
+  if (keyword == "Sites" || keyword == "Neighbors")
+    throw Error("read_sites " + keyword + " section is outside this build (sites come from create_sites)");
+  if (keyword != "Values") throw Error("Unknown identifier in data file: " + keyword);
+  if (nsites_ == 0) throw Error("Cannot read Values before sites exist or are read");
+  if (!h_) throw Error("Color stencil is incommensurate with lattice size");
+
+  // column kinds: 0 spin (site / i1), 1 seed
+  std::vector<int> kind;
+  if (!have_columns) {
+    kind = {0};   // ninteger + ndouble = 1 for the potts and ising styles
+  } else {
+    std::istringstream ss(columns);
+    std::string w;
+    std::vector<std::string> words;
+    while (ss >> w) words.push_back(w);
+    if (words.size() < 2) throw Error("Read sites Values section has no values per site");
+    if (words[0] != "id") throw Error("Read sites values header keyword must start with id");
+    for (size_t i = 1; i < words.size(); i++) {
+      if (words[i] == "site" || words[i] == "i1") kind.push_back(0);
+      else if (words[i] == "seed") kind.push_back(1);
+      else throw Error("Invalid keyword in read sites values header");
+    }
+  }
+  download();
+  bool seeds = false;
+  for (int k : kind) seeds |= k == 1;
+  // nsites lines of id + one word per column
+  int64_t nread = 0;
+  while (nread < nsites_) {
+    if (!std::fgets(line, sizeof line, fp)) throw Error("Unexpected end of data file");
+    std::istringstream ls(line);
+    std::vector<std::string> w;
+    std::string t;
+    while (ls >> t) w.push_back(t);
+    if (w.size() != kind.size() + 1) throw Error("Incorrect value format in data file");
+    const long long id = std::atoll(w[0].c_str());
+    if (id >= 1 && id <= nsites_) {
+      for (size_t m = 0; m < kind.size(); m++) {
+        const int v = std::atoi(w[m + 1].c_str());
+        if (kind[m] == 0) host_spin_[id - 1] = v;
+        else host_seed_[id - 1] = v;
+      }
+    }
+    nread++;
+  }
+  int rc = spk_hip_upload(h_, host_spin_.data(), seeds ? host_seed_.data() : nullptr);
+  if (rc == SPK_HIP_EVALUE && !seeds) throw Error("One or more sites have invalid values");
+  hip(rc);
+  values_set_ = true;
+  if (seeds) seeds_loaded_ = true;
 }
 
 // Output::setup (src/output.cpp:82-137)

@@ -33,8 +33,10 @@ struct Region {
 
 struct Dump {
   std::string id, file;
+  std::string style;                 // "text" or "sites"
+  bool multifile = false;            // '*' in the file name: one file per snapshot
   double delta = 0.0;
-  std::vector<std::string> fields;   // id site x y z energy
+  std::vector<std::string> fields;   // id site i1 x y z energy seed
   double next_time = 0.0;
   int idump = 0;
   FILE *fp = nullptr;
@@ -84,6 +86,7 @@ class Spparks {
   void cmd_diag_style(const std::vector<std::string> &a);
   void cmd_stats(const std::vector<std::string> &a);
   void cmd_dump(const std::vector<std::string> &a);
+  void cmd_read_sites(const std::vector<std::string> &a);
   void cmd_undump(const std::vector<std::string> &a);
   void cmd_run(const std::vector<std::string> &a);
   void cmd_variable(const std::vector<std::string> &a);
@@ -126,7 +129,8 @@ class Spparks {
   int nx_ = 0, ny_ = 0, nz_ = 0;
   int64_t nsites_ = 0;
   spk_hip_lattice *h_ = nullptr;
-  bool values_set_ = false;  // every site holds a value from `set`
+  bool values_set_ = false;  // every site holds a value from `set` or read_sites
+  bool seeds_loaded_ = false;  // strict seeds came from a read_sites seed column
 
   // rejection KMC state (AppLattice)
   int sweepflag_ = 0;        // 0 none, 1 color/strict on the GPU

@@ -45,6 +45,16 @@ def run_script(tmp_path, text, *args):
      "create_box box\ncreate_sites box\n", "not a multiple of lattice spacing"),
     ("variable x index 3\nprint ${y}\n", "illegal variable"),
     ("stats -1\n", "Illegal stats command"),
+    # dump sites / read_sites (src/dump_sites.cpp:25-48, src/read_sites.cpp:69-80)
+    (PRE + "dump 1 sites 1.0 out.sites id site\n", "Dump sites must write one file per snapshot"),
+    (PRE + "dump 1 sites 1.0 out.* site id\n", "Dump sites requires ID as first field"),
+    (PRE + "dump 1 sites 1.0 out.* id\n", "Dump sites requires two or more fields"),
+    (PRE + "dump 1 text 1.0 out.txt id propensity\n", "Invalid attribute propensity"),
+    ("read_sites file.sites\n", "Read_sites command before app_style set"),
+    ("app_style potts 10\nread_sites a b\n", "Illegal read_sites command"),
+    ("app_style potts 10\nread_sites no_such_file\n", "Cannot open file"),
+    ("app_style ising/single 3\n", "Illegal app_style command"),
+    ("app_style potts/neigh\n", "Illegal app_style command"),
 ])
 def test_error_messages(tmp_path, text, msg):
     import re
@@ -71,3 +81,28 @@ def test_only_hip_suffix(tmp_path):
     assert rc != 0 and "hip" in out
     rc, out = run_script(tmp_path, "print ok\n", "-sf", "hip")
     assert rc == 0
+
+
+def _site_file(tmp_path, body_header, values):
+    p = tmp_path / "in.sites"
+    p.write_text("title line\n\n" + body_header + values)
+    return str(p)
+
+
+@pytest.mark.parametrize("header,values,msg", [
+    ("2 dimension\n64 sites\nid site values\n-0.5 8.5 xlo xhi\n", "\nValues\n\n1 1\n",
+     "Data file simulation box different"),
+    ("3 dimension\n64 sites\n", "\nValues\n\n", "dimension does not match"),
+    ("2 dimension\n64 sites\n", "\nSites\n\n", "Sites section is outside this build"),
+    ("2 dimension\n64 sites\n# a comment\n", "\nVelocities\n\n", "Unknown identifier in data file: Velocities"),
+    ("2 dimension\n64 sites\n", "\nValues\n\n1 3\n", "Cannot read Values before sites exist"),
+    ("2 dimension\n", "", "Site file has no Sites, Neighbors, or Values"),
+])
+def test_read_sites_header_errors(tmp_path, header, values, msg):
+    """read_sites' header checks against the existing box, in the reference's
+    order (src/read_sites.cpp:199-314, :122-168); the box exists, the sites
+    do not (creating them needs the device)."""
+    f = _site_file(tmp_path, header, values)
+    rc, out = run_script(tmp_path, PRE + "read_sites %s\n" % f)
+    assert rc != 0
+    assert msg in out, out

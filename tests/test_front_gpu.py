@@ -200,3 +200,96 @@ def test_variant_scripts_match_oracle(orc, tmp_path, app, q, lat, dims, run, nsw
         k = r[3]   # the row's sweep count
         assert r[1] == nacc[k - 1] and r[1] + r[2] == k * n
         assert r[4] == eng[k - 1]
+
+
+CK_BASE = ("app_style potts 50\ndimension 2\nlattice sq/8n 1.0\nregion box block 0 16 0 12 -0.5 0.5\n"
+           "create_box box\ncreate_sites box\n")
+
+
+def site_file_values(path):
+    """(header lines, values array) of a site file written by dump sites."""
+    lines = open(path).read().splitlines()
+    k = lines.index("Values")
+    vals = np.array([[int(w) for w in l.split()] for l in lines[k + 2:] if l.strip()], np.int64)
+    return lines[:k], vals
+
+
+def test_dump_sites_format_and_checkpoint_resume(orc, tmp_path):
+    """dump sites writes one site file per snapshot with the reference's
+    header (src/dump_sites.cpp:52-66); with this build's seed column such a
+    file is a checkpoint: a fresh process (another master seed, no set
+    command) that read_sites it continues the trajectory bit for bit."""
+    a = tmp_path / "in.a"
+    a.write_text("seed 901\n" + CK_BASE + "set site range 1 50\nsweep color/strict\ntemperature 0.6\n"
+                 "diag_style energy\nstats 0.02\ndump ck sites 0.1 ck.* id site seed\nrun 0.2\n")
+    rows_a = stats_rows(run_exe(tmp_path, str(a)))
+    head, v1 = site_file_values(tmp_path / "ck.1")
+    assert head[0] == "Site file written by dump sites ck command at time: 1 0.1"
+    assert head[1:] == ["", "2 dimension", "192 sites", "id site seed values", "0 16 xlo xhi", "0 12 ylo yhi",
+                        "-0.5 0.5 zlo zhi", ""]
+    _, v2 = site_file_values(tmp_path / "ck.2")
+    assert np.array_equal(v1[:, 0], np.arange(1, 193))
+    # the oracle after 5 sweeps (t = 0.1, dt_sweep = 1/50) and at the end of
+    # run 0.2: ten steps of 0.02 sum to just below 0.2, so the reference's
+    # loop (dt_rkmc fixed at setup) makes an 11th sweep, ending at t = 0.22
+    n_end = rows_a[-1][3]
+    assert n_end == orc.run_nsweeps(0.0, 0.2, 1.0 / 50)[0] == 11
+    d = orc.make_desc("potts", "sq/8n", 16, 12, 1, 50, 0.6)
+    spin, seeds = orc.script_init(d, 901, 1, 50)
+    orc.sweeps(d, spin, seeds, 5)
+    assert np.array_equal(v1[:, 1], spin) and np.array_equal(v1[:, 2], seeds)
+    s10, k10 = spin.copy(), seeds.copy()
+    nacc, eng = orc.sweeps(d, s10, k10, 5)
+    s_end, k_end = spin.copy(), seeds.copy()
+    orc.sweeps(d, s_end, k_end, n_end - 5)
+    assert np.array_equal(v2[:, 1], s_end) and np.array_equal(v2[:, 2], k_end)
+
+    b = tmp_path / "in.b"
+    b.write_text("seed 7777\n" + CK_BASE + "read_sites ck.1\nsweep color/strict\ntemperature 0.6\n"
+                 "diag_style energy\nstats 0.02\ndump fin sites 0.1 fin.* id site seed\nrun 0.1\n")
+    rows = stats_rows(run_exe(tmp_path, str(b)))
+    _, vb = site_file_values(tmp_path / "fin.1")
+    # run 0.1 from t = 0 is 5 sweeps: the continuation of ck.1 to sweep 10
+    assert rows[-1][3] == 5 and rows[-1][4] == eng[-1]
+    assert np.array_equal(vb[:, 1], s10) and np.array_equal(vb[:, 2], k10)
+
+
+def test_read_sites_values_only_keeps_tagreset_seeds(orc, tmp_path):
+    """A site file without the seed column sets spins only; the first run
+    then seeds by tagreset from this script's own master draws (no set
+    command: the seed draw is the 4th)."""
+    rng = np.random.default_rng(3)
+    spins = rng.integers(1, 51, 192).astype(np.int32)
+    f = tmp_path / "v.sites"
+    f.write_text("title\n\n2 dimension\n192 sites\nid site values\n0 16 xlo xhi\n0 12 ylo yhi\n"
+                 "-0.5 0.5 zlo zhi\n\nValues\n\n" + "".join("%d %d\n" % (i + 1, s) for i, s in enumerate(spins)))
+    b = tmp_path / "in.v"
+    b.write_text("seed 4242\n" + CK_BASE + "read_sites v.sites\nsweep color/strict\ntemperature 0.6\n"
+                 "diag_style energy\nstats 0.02\ndump fin sites 0.04 out.* id site seed\nrun 0.04\n")
+    rows = stats_rows(run_exe(tmp_path, str(b)))
+    _, v0 = site_file_values(tmp_path / "out.0")
+    assert np.array_equal(v0[:, 1], spins)
+    seeds = orc.site_seeds(orc.master_draws(4242, 4)[3], 192)
+    assert np.array_equal(v0[:, 2], seeds)
+    d = orc.make_desc("potts", "sq/8n", 16, 12, 1, 50, 0.6)
+    sp = spins.copy()
+    nacc, eng = orc.sweeps(d, sp, seeds, 2)
+    _, v1 = site_file_values(tmp_path / "out.1")
+    assert np.array_equal(v1[:, 1], sp) and np.array_equal(v1[:, 2], seeds)
+    assert rows[-1][1] == nacc[-1] and rows[-1][4] == eng[-1]
+
+
+@pytest.mark.parametrize("values,msg", [
+    ("1 3 4\n", "Incorrect value format"),
+    ("1 3\n2 3\n", "Unexpected end of data file"),
+    ("".join("%d 51\n" % (i + 1) for i in range(192)), "One or more sites have invalid values"),
+])
+def test_read_sites_value_errors(tmp_path, values, msg):
+    f = tmp_path / "e.sites"
+    f.write_text("t\n\n2 dimension\n192 sites\n\nValues\n\n" + values)
+    p = tmp_path / "in.e"
+    p.write_text("seed 1\n" + CK_BASE + "read_sites e.sites\n")
+    r = subprocess.run([EXE, "-in", str(p), "-log", "none"], capture_output=True, text=True, cwd=tmp_path,
+                       timeout=110)
+    assert r.returncode != 0
+    assert msg in r.stdout + r.stderr
