@@ -283,6 +283,7 @@ def run(args):
         if args.fusion:
             L.set_fusion(True)
         L.set_graph(args.chunk)
+        fused = L.fusion
         if args.warmup:
             L.sweeps(args.warmup, per_sweep=False)
         L.sync()
@@ -296,9 +297,22 @@ def run(args):
     updates = n * steps
     value = updates / wall / 1e9
     per_launch_s = dev_ms / 1e3 / nlaunch
-    B = bytes_per_update(lattice, spin_bytes(app, q))
+    sb = spin_bytes(app, q)
+    B = bytes_per_update(lattice, sb)
     B_survey = bytes_per_update(lattice, 4)
-    bytes_per_launch = B * n // K
+    tname = {1: "u8", 2: "u16", 4: "i32"}[sb]
+    if fused:
+        # a fused launch makes `fused` whole sweeps (K * fused colour passes)
+        # of the lattice in LDS tiles: compulsory traffic is each site's
+        # spin and seed read once and written once per launch (halo re-reads
+        # are overhead, not algorithmic bytes)
+        kernel = "k_sweep_fused2d<%s, %s, S=%d>" % (lattice, tname, fused)
+        bytes_per_launch = n * (2 * sb + 8)
+        B_kernel = (2 * sb + 8) / fused
+    else:
+        kernel = "k_sweep_color<%s, %s, 4>" % (lattice, tname)
+        bytes_per_launch = B * n // K
+        B_kernel = B
     achieved = bytes_per_launch / per_launch_s / 1e9
     out = {
         "metric": "G site-updates/s & % HBM peak; batched vs plain speedup; p99 solve latency",
@@ -319,11 +333,11 @@ def run(args):
                    "naccept": int(nacc), "final_energy": energy},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": load_traffic(args.workload + ("-fused" if args.fusion else "")),
-                     "kernel": "k_sweep_color<%s, %s, 4>" % (lattice, {1: "u8", 2: "u16", 4: "i32"}[spin_bytes(app, q)]),
+                     "traffic": load_traffic(args.workload + ("-fused" if fused else "")),
+                     "kernel": kernel,
                      "bytes_per_launch": bytes_per_launch,
                      "avg_launch_us": round(per_launch_s * 1e6, 3),
-                     "bytes_per_update": B},
+                     "bytes_per_update": round(B_kernel, 3)},
         "pct_hbm_peak": round(100 * value * 1e9 * B / (HBM_PEAK_GBS * 1e9), 2),
         "survey_model_pct_hbm": round(100 * value * 1e9 * B_survey / (HBM_PEAK_GBS * 1e9), 2),
         "spin_bytes": spin_bytes(app, q),
