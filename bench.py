@@ -278,8 +278,6 @@ def run(args):
         L.init_seeds(rseed)
         if args.block:
             L.set_launch(args.block)
-        if args.packs:
-            L.set_packs(args.packs)
         if args.fusion:
             L.set_fusion(True)
         L.set_graph(args.chunk)
@@ -365,7 +363,6 @@ def main():
     ap.add_argument("--block-sweeps", type=int, default=25, help="cfg4: sweeps per graph block")
     ap.add_argument("--grid", type=int, default=0, help="cfg4: persistent workgroups (with --block)")
     ap.add_argument("--block", type=int, default=0, help="sweep-kernel workgroup size (0 = default)")
-    ap.add_argument("--packs", type=int, default=0, help="4-site packs per thread, 1 or 2 (0 = default)")
     ap.add_argument("--fusion", action="store_true", help="2d lattices: several sweeps per launch in LDS tiles")
     ap.add_argument("--chunk", type=int, default=32, help="sweeps per replayed graph (0 = eager launches)")
     ap.add_argument("--no-cpu-baseline", action="store_true")

@@ -145,13 +145,9 @@ int spk_hip_sync(spk_hip_lattice *h);
 
 /* Launch configuration of the per-colour sweep kernel (results do not
    depend on it: colour/strict is decomposition independent).
-   block = threads per workgroup (64..1024, multiple of 64). */
+   block = threads per workgroup (a power of two in 64..1024); a workgroup
+   covers a tile of sub-grid rows (x, y and z), rows in y and z. */
 int spk_hip_set_launch(spk_hip_lattice *h, int block);
-
-/* 4-site packs per thread in the per-colour kernel: 1 or 2 (a thread then
-   loads both packs before deciding either).  No reference counterpart; a
-   launch setting like spk_hip_set_launch, results do not depend on it. */
-int spk_hip_set_packs(spk_hip_lattice *h, int packs);
 
 /* Sweep-fused launches for 2d lattices (sq/4n, sq/8n): each workgroup holds
    a tile plus a halo in LDS and runs several whole sweeps there (4 for

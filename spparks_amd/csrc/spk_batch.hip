@@ -41,6 +41,8 @@ struct StepDesc {
 };
 
 constexpr int kVec = 4;
+constexpr int kBatchSlots = 8;                         // accumulator slots per replica sweep row
+constexpr int kBatchRow = kBatchSlots * kSlotStride;   // uint64 words per row
 
 // One colour step of a block for the replicas of one (lattice, storage)
 // variant.  Workgroups (XCD-remapped) take contiguous chunks of the step's
@@ -92,7 +94,8 @@ __global__ __launch_bounds__(1024) void k_batch_step(char *__restrict__ pool, co
       else
         cnt = sweep_sites<LAT, T, kVec, false, uint32_t>(sp, sdp, g, rule, s_thr, p % K, a0, b, e);
     }
-    block_accumulate(cnt.nacc, cnt.de, acc + 2 * (int64_t)(rep.acc_row + p / K), 0);
+    // tiles of one replica pass spread over the row's slots (one line each)
+    block_accumulate(cnt.nacc, cnt.de, acc + (int64_t)(rep.acc_row + p / K) * kBatchRow, t & (kBatchSlots - 1));
   }
 }
 
@@ -186,13 +189,14 @@ struct spk_hip_batch {
   // staging for transfers and per-replica set-up
   int32_t *stage = nullptr;
   int64_t stage_n = 0;
-  unsigned long long *scratch = nullptr;   // 128 u64 for set-up reductions
+  unsigned long long *scratch = nullptr;   // kAccRow + 2 u64 for set-up reductions
   // block tables (device) and their host images
   BatchRep *d_reps = nullptr;
   StepDesc *d_steps = nullptr;
   int *d_act = nullptr, *d_pre = nullptr;
-  unsigned long long *d_acc = nullptr;
-  int cap_reps = 0, cap_steps = 0, cap_act = 0, cap_pre = 0, cap_acc = 0;
+  unsigned long long *d_acc = nullptr;    // [rows][kBatchRow] slots
+  unsigned long long *d_accc = nullptr;   // [rows][2] folded (Naccept, dE)
+  int cap_reps = 0, cap_steps = 0, cap_act = 0, cap_pre = 0, cap_acc = 0, cap_accc = 0;
   std::vector<BatchRep> h_reps;
   std::vector<StepDesc> h_steps;
   std::vector<int> h_act, h_pre;
@@ -253,7 +257,7 @@ int ensure_stage(spk_hip_batch *b, int64_t n) {
 
 // total energy of one replica (exact integer)
 int replica_energy(spk_hip_batch *b, const Replica &R, int64_t *e) {
-  int rc = check_hip(hipMemsetAsync(b->scratch, 0, 128 * sizeof(unsigned long long), b->stream), "memset");
+  int rc = check_hip(hipMemsetAsync(b->scratch, 0, kAccRow * sizeof(unsigned long long), b->stream), "memset");
   if (rc) return rc;
   const int64_t n = (int64_t)R.g.M * R.g.K;
   with_lattice(R.g.lat, [&]<int LAT>() {
@@ -263,13 +267,15 @@ int replica_energy(spk_hip_batch *b, const Replica &R, int64_t *e) {
     });
   });
   if ((rc = check_hip(hipGetLastError(), "energy launch"))) return rc;
-  unsigned long long h[128];
-  if ((rc = check_hip(hipMemcpyAsync(h, b->scratch, sizeof h, hipMemcpyDeviceToHost, b->stream), "readback")))
+  hipLaunchKernelGGL(k_fold_rows, dim3(1), dim3(64), 0, b->stream, b->scratch, 1, kAccSlots, kAccRow,
+                     b->scratch + kAccRow);
+  if ((rc = check_hip(hipGetLastError(), "fold launch"))) return rc;
+  unsigned long long h[2];
+  if ((rc = check_hip(hipMemcpyAsync(h, b->scratch + kAccRow, sizeof h, hipMemcpyDeviceToHost, b->stream),
+                      "readback")))
     return rc;
   if ((rc = check_hip(hipStreamSynchronize(b->stream), "sync"))) return rc;
-  unsigned long long s = 0;
-  for (int k = 0; k < kAccSlots; k++) s += h[2 * k];
-  *e = (int64_t)s;
+  *e = (int64_t)h[0];
   return SPK_HIP_OK;
 }
 
@@ -352,7 +358,8 @@ int upload_schedule(spk_hip_batch *b, int rows, bool clear_acc) {
       (rc = grow(&b->d_steps, &b->cap_steps, std::max<int>(1, b->h_steps.size()), "step table", &moved)) ||
       (rc = grow(&b->d_act, &b->cap_act, std::max<int>(1, b->h_act.size()), "step lists", &moved)) ||
       (rc = grow(&b->d_pre, &b->cap_pre, std::max<int>(1, b->h_pre.size()), "step lists", &moved)) ||
-      (rc = grow(&b->d_acc, &b->cap_acc, std::max(2, 2 * rows), "accumulators", &moved)))
+      (rc = grow(&b->d_acc, &b->cap_acc, std::max(1, rows) * kBatchRow, "accumulators", &moved)) ||
+      (rc = grow(&b->d_accc, &b->cap_accc, std::max(2, 2 * rows), "accumulators", &moved)))
     return rc;
   if (moved) drop_graphs(b);
   auto put = [&](void *dst, const void *src, size_t n) {
@@ -365,8 +372,8 @@ int upload_schedule(spk_hip_batch *b, int rows, bool clear_acc) {
     return rc;
   if (!clear_acc) return SPK_HIP_OK;
   // accumulators cleared by a kernel (no memset node ever enters a graph)
-  hipLaunchKernelGGL(k_fill_u64, dim3(grid_for(2 * (int64_t)rows, 256)), dim3(256), 0, b->stream, b->d_acc,
-                     2 * (int64_t)rows);
+  hipLaunchKernelGGL(k_fill_u64, dim3(grid_for((int64_t)rows * kBatchRow, 256)), dim3(256), 0, b->stream, b->d_acc,
+                     (int64_t)rows * kBatchRow);
   return check_hip(hipGetLastError(), "accumulator clear");
 }
 
@@ -449,7 +456,7 @@ int spk_hip_batch_create(int64_t pool_bytes, int max_replicas, spk_hip_batch **o
   b->ncu = prop.multiProcessorCount;
   b->pool_bytes = round_up(pool_bytes, 256);
   if (hipMalloc(&b->pool, (size_t)b->pool_bytes) != hipSuccess ||
-      hipMalloc(&b->scratch, 128 * sizeof(unsigned long long)) != hipSuccess) {
+      hipMalloc(&b->scratch, (kAccRow + 2) * sizeof(unsigned long long)) != hipSuccess) {
     spk_hip_batch_destroy(b);
     return set_error(SPK_HIP_ENOMEM, "device pool of %lld bytes not available", (long long)pool_bytes);
   }
@@ -480,6 +487,7 @@ void spk_hip_batch_destroy(spk_hip_batch *b) {
   if (b->d_act) (void)hipFree(b->d_act);
   if (b->d_pre) (void)hipFree(b->d_pre);
   if (b->d_acc) (void)hipFree(b->d_acc);
+  if (b->d_accc) (void)hipFree(b->d_accc);
   if (b->stream) (void)hipStreamDestroy(b->stream);
   delete b->alloc;
   delete b;
@@ -734,7 +742,10 @@ int spk_hip_batch_run(spk_hip_batch *b, int max_sweeps, int use_graph) {
 
   // per-replica Naccept / energy per sweep of the block
   b->h_acc.resize(2 * (size_t)rows);
-  if ((rc = check_hip(hipMemcpyAsync(b->h_acc.data(), b->d_acc, 2 * (size_t)rows * sizeof(unsigned long long),
+  hipLaunchKernelGGL(k_fold_rows, dim3(grid_for(rows, 64)), dim3(64), 0, b->stream, b->d_acc, rows, kBatchSlots,
+                     kBatchRow, b->d_accc);
+  if ((rc = check_hip(hipGetLastError(), "fold launch"))) return rc;
+  if ((rc = check_hip(hipMemcpyAsync(b->h_acc.data(), b->d_accc, 2 * (size_t)rows * sizeof(unsigned long long),
                                      hipMemcpyDeviceToHost, b->stream),
                       "accumulator readback")) ||
       (rc = check_hip(hipStreamSynchronize(b->stream), "readback sync")))

@@ -123,6 +123,58 @@ __host__ __device__ __forceinline__ void pack_site(const Geom &g, uint32_t t, in
   e = (int)zz;
 }
 
+// Workgroup tiles of the per-colour kernel.  A workgroup of `block` threads
+// covers a box of 2^lx threads (4 sites each) along x, 2^ly sub-grid rows
+// along y and 2^lz planes along z, so the rows a site reads in y and z are
+// mostly rows that other waves of the same workgroup own (L1 hits), rather
+// than rows of other workgroups.  Tiles are numbered x-chunk fastest, then
+// y-group, then z-group; threads of a tile likewise.  Slots past the edge of
+// the sub-grid (partial tiles) are idle.
+struct TileMap {
+  int lx, ly, lz;      // log2 extents of a tile in threads / rows / planes
+  int nxc, nyg, nzg;   // tiles along x, y, z
+  FastDiv fxc, fyg;    // division by nxc, nyg
+};
+
+__host__ __device__ inline int ceil_log2(int v) {
+  int l = 0;
+  while ((1 << l) < v) l++;
+  return l;
+}
+
+// block must be a power of two
+__host__ __device__ inline TileMap make_tile_map(const Geom &g, int block) {
+  TileMap m;
+  const int lb = ceil_log2(block);
+  m.lx = min(min(ceil_log2(g.px / 4), 6), lb);        // up to one wavefront along x
+  int rest = lb - m.lx;
+  const int cy = ceil_log2(g.hy), cz = ceil_log2(g.hz);
+  m.lz = min(min(rest / 2, cz), rest);                  // the rows split evenly over y and z
+  m.ly = min(rest - m.lz, cy);
+  m.lz = min(rest - m.ly, cz);
+  m.nxc = (g.px / 4 + (1 << m.lx) - 1) >> m.lx;
+  m.nyg = (g.hy + (1 << m.ly) - 1) >> m.ly;
+  m.nzg = (g.hz + (1 << m.lz) - 1) >> m.lz;
+  m.fxc = fastdiv_make((uint32_t)m.nxc);
+  m.fyg = fastdiv_make((uint32_t)m.nyg);
+  return m;
+}
+
+// tile `tile`, thread `tid` -> first site a0 and row (b, e); false for a
+// slot outside the sub-grid (or a thread beyond the tile's extent)
+__host__ __device__ __forceinline__ bool tile_site(const Geom &g, const TileMap &m, uint32_t tile, uint32_t tid,
+                                                   int &a0, int &b, int &e) {
+  uint32_t rest, xc, zg, yg;
+  fastdiv(m.fxc, tile, rest, xc);
+  fastdiv(m.fyg, rest, zg, yg);
+  const uint32_t tx = tid & ((1u << m.lx) - 1), ty = (tid >> m.lx) & ((1u << m.ly) - 1),
+                 tz = tid >> (m.lx + m.ly);
+  a0 = (int)(((xc << m.lx) + tx) * 4);
+  b = (int)((yg << m.ly) + ty);
+  e = (int)((zg << m.lz) + tz);
+  return tz < (1u << m.lz) && a0 < g.px && b < g.hy && e < g.hz;
+}
+
 __host__ __device__ inline int maxneigh_of(int lat) {
   return lat == SPK_HIP_SQ_4N ? 4 : lat == SPK_HIP_SQ_8N ? 8 : lat == SPK_HIP_SC_6N ? 6 : 26;
 }
@@ -1032,11 +1084,17 @@ __device__ __forceinline__ unsigned long long wave_sum64(unsigned long long v) {
   return v;
 }
 
-// Adds (x, y) of every thread of the block into acc[2*slot], acc[2*slot+1]:
-// a wavefront sum (DPP, 32-bit: per-wave totals stay far below 2^31), one
-// LDS slot per wave, one atomic per block and value.  All threads of the
-// block must call it; blockDim.x % 64 == 0.  Callers that reduce again put
-// a barrier between consecutive calls (the LDS slots are reused).
+// Accumulator rows: `slots` slots of (x, y) uint64 sums, each slot on its
+// own 128-byte line (kSlotStride words apart).  Device-scope atomics from
+// many workgroups serialise per cache line, so slots that shared a line
+// cost ~10% of a colour pass; one line each removes that.
+constexpr int kSlotStride = 16;
+
+// Adds (x, y) of every thread of the block into slot `slot` of the row at
+// acc: a wavefront sum (DPP, 32-bit: per-wave totals stay far below 2^31),
+// one LDS slot per wave, one atomic per block and value.  All threads of the
+// block must call it; blockDim.x % 64 == 0.  Callers that reduce again put a
+// barrier between consecutive calls (the LDS slots are reused).
 __device__ __forceinline__ void block_accumulate(int x, int y, unsigned long long *acc, int slot) {
   __shared__ int red[2][16];
   x = wave_sum(x);
@@ -1054,12 +1112,14 @@ __device__ __forceinline__ void block_accumulate(int x, int y, unsigned long lon
       sx += red[0][w];
       sy += red[1][w];
     }
-    if (sx) atomicAdd(&acc[2 * slot], (unsigned long long)sx);
-    if (sy) atomicAdd(&acc[2 * slot + 1], (unsigned long long)sy);
+    unsigned long long *a = acc + (size_t)slot * kSlotStride;
+    if (sx) atomicAdd(&a[0], (unsigned long long)(long long)sx);
+    if (sy) atomicAdd(&a[1], (unsigned long long)(long long)sy);
   }
 }
 
-constexpr int kAccSlots = 64;   // spread of per-sweep accumulator slots
+constexpr int kAccSlots = 64;                       // slots per row of the single-lattice path
+constexpr int kAccRow = kAccSlots * kSlotStride;     // uint64 words per row
 constexpr int kNumXcd = 8;      // MI355X: workgroups are dealt round-robin to 8 XCDs
 
 // Logical tile of hardware workgroup `bid` in a grid of `nb` workgroups

@@ -56,6 +56,22 @@ __global__ void k_to_natural(const T *__restrict__ blk, int32_t *__restrict__ na
   nat[t] = (int32_t)blk[blocked_index(g, i, j, k)];
 }
 
+// Compacts accumulator rows: out[2r], out[2r+1] = the sums of row r's
+// `slots` slots (row r at acc + r*row_words, slot k at +k*kSlotStride).
+__global__ void k_fold_rows(const unsigned long long *__restrict__ acc, int rows, int slots, int row_words,
+                            unsigned long long *__restrict__ out) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= rows) return;
+  const unsigned long long *p = acc + (size_t)r * row_words;
+  unsigned long long x = 0, y = 0;
+  for (int k = 0; k < slots; k++) {
+    x += p[(size_t)k * kSlotStride];
+    y += p[(size_t)k * kSlotStride + 1];
+  }
+  out[2 * (size_t)r] = x;
+  out[2 * (size_t)r + 1] = y;
+}
+
 // counts values outside [lo, hi]
 __global__ void k_count_outside(const int32_t *__restrict__ v, int64_t n, int lo, int hi,
                                 unsigned long long *__restrict__ bad) {

@@ -23,47 +23,24 @@ using namespace spk;
 
 // One colour pass: every site of colour `color` performs one strict event.
 // A pack is 4 consecutive sites of one sub-grid row (x fastest), so its own
-// spins/seeds are one aligned load and the wavefront's accesses are
-// unit-stride.  With PACKS = 2 thread t also owns pack t + span (span =
-// half the packs, rounded up) and counts both before deciding either: twice
-// the loads in flight per wave.  Workgroups are remapped so neighbouring
-// tiles share an XCD.
-template <int LAT, typename T, int V, bool TWO, typename I, int PACKS>
+// spins/seeds are one aligned load and a wavefront's accesses are
+// unit-stride.  Each workgroup covers one TileMap tile (rows of its own
+// waves supply most of the y/z neighbour rows); tiles are remapped so that
+// neighbouring tiles share an XCD.
+template <int LAT, typename T, int V, bool TWO, typename I>
 __global__ __launch_bounds__(1024) void k_sweep_color(T *__restrict__ spin, int32_t *__restrict__ seed,
-                                                      Geom g, Rule r, int color,
+                                                      Geom g, Rule r, TileMap m, int color,
                                                       unsigned long long *__restrict__ acc) {
   __shared__ int s_thr[32];
   if (threadIdx.x < 28) s_thr[threadIdx.x] = r.thr[threadIdx.x];
   __syncthreads();
 
-  static_assert(V == 4, "pack_site maps 4 sites per thread");
+  static_assert(V == 4, "tiles map 4 sites per thread");
   const int tile = xcd_tile(blockIdx.x, gridDim.x);
-  const int64_t t = (int64_t)tile * blockDim.x + threadIdx.x;
-  const int64_t npack = g.Mp / V;
-  const int64_t span = (npack + PACKS - 1) / PACKS;
   Counts cnt{0, 0};
-  if (t < span) {
-    int a0, b, e;
-    pack_site(g, (uint32_t)t, a0, b, e);
-    if constexpr (PACKS == 1) {
-      cnt = sweep_sites<LAT, T, V, TWO, I>(spin, seed, g, r, s_thr, color, a0, b, e);
-    } else {
-      static_assert(PACKS == 2, "one or two packs per thread");
-      PackWork<T, TWO, I> w0 = pack_count<LAT, T, TWO, I>(spin, seed, g, r, color, a0, b, e);
-      const bool second = t + span < npack;
-      PackWork<T, TWO, I> w1;
-      if (second) {
-        pack_site(g, (uint32_t)(t + span), a0, b, e);
-        w1 = pack_count<LAT, T, TWO, I>(spin, seed, g, r, color, a0, b, e);
-      }
-      cnt = pack_decide<LAT, T, TWO, I>(w0, spin, seed, g, r, s_thr, color);
-      if (second) {
-        const Counts c1 = pack_decide<LAT, T, TWO, I>(w1, spin, seed, g, r, s_thr, color);
-        cnt.nacc += c1.nacc;
-        cnt.de += c1.de;
-      }
-    }
-  }
+  int a0, b, e;
+  if (tile < m.nxc * m.nyg * m.nzg && tile_site(g, m, (uint32_t)tile, threadIdx.x, a0, b, e))
+    cnt = sweep_sites<LAT, T, V, TWO, I>(spin, seed, g, r, s_thr, color, a0, b, e);
   block_accumulate(cnt.nacc, cnt.de, acc, blockIdx.x & (kAccSlots - 1));
 }
 
@@ -160,7 +137,7 @@ __global__ __launch_bounds__(512) void k_sweep_fused2d(const T *__restrict__ spi
 #pragma unroll
   for (int sw = 0; sw < S; sw++) {
     __syncthreads();   // the previous reduction's LDS slots are free again
-    block_accumulate(nacc[sw], de[sw], acc + (size_t)sw * kAccSlots * 2, blockIdx.x & (kAccSlots - 1));
+    block_accumulate(nacc[sw], de[sw], acc + (size_t)sw * kAccRow, blockIdx.x & (kAccSlots - 1));
   }
 }
 
@@ -180,10 +157,10 @@ struct spk_hip_lattice {
   void *spin_alt = nullptr;            // second state buffer for sweep-fused launches
   int32_t *seed_alt = nullptr;
   int fuse = 0;                        // sweeps per fused launch (0: per-colour kernel only)
-  int packs = 1;                       // 4-site packs per thread in the per-colour kernel
   int tile_x = 0, tile_y = 0;          // fused core tile
   int32_t *stage = nullptr;            // N int32, natural order (transfers)
-  unsigned long long *acc = nullptr;   // [cap][kAccSlots][2]
+  unsigned long long *acc = nullptr;   // [cap][kAccRow]: per-sweep slots
+  unsigned long long *accc = nullptr;  // [cap][2]: rows folded to (Naccept, dE)
   int acc_cap = 0;
   int block = 256;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -232,30 +209,26 @@ int launch_variant(spk_hip_lattice *h, int color, unsigned long long *acc) {
 int launch_sweep(spk_hip_lattice *h, int color, unsigned long long *acc) {
   if (variant_app(h->d.app)) return launch_variant(h, color, acc);
   const int bs = h->block;
-  const int64_t threads = (h->g.Mp / kVec + h->packs - 1) / h->packs;
-  unsigned grid = grid_for(threads, bs);
+  const TileMap m = make_tile_map(h->g, bs);
+  unsigned grid = (unsigned)m.nxc * m.nyg * m.nzg;
   grid = (grid + kNumXcd - 1) / kNumXcd * kNumXcd;   // xcd_tile needs a multiple of 8
   const bool two = h->rule.nspins == 2;   // two states are always stored as u8
   const bool off32 = offsets_fit_32(h->g);
-  const bool two_packs = h->packs == 2;
   with_lattice(h->g.lat, [&]<int LAT>() {
     with_type(h->stype, [&]<typename T>() {
       auto go = [&](auto kern) {
         hipLaunchKernelGGL(kern, dim3(grid), dim3(bs), 0, h->stream, static_cast<T *>(h->spin), h->seed, h->g,
-                           h->rule, color, acc);
+                           h->rule, m, color, acc);
       };
-      // 64-bit offsets (lattices of 2^30 sites or more) take one pack per thread
       if constexpr (sizeof(T) == 1) {
         if (two) {
-          if (!off32) go(k_sweep_color<LAT, T, kVec, true, int64_t, 1>);
-          else if (two_packs) go(k_sweep_color<LAT, T, kVec, true, uint32_t, 2>);
-          else go(k_sweep_color<LAT, T, kVec, true, uint32_t, 1>);
+          if (off32) go(k_sweep_color<LAT, T, kVec, true, uint32_t>);
+          else go(k_sweep_color<LAT, T, kVec, true, int64_t>);
           return;
         }
       }
-      if (!off32) go(k_sweep_color<LAT, T, kVec, false, int64_t, 1>);
-      else if (two_packs) go(k_sweep_color<LAT, T, kVec, false, uint32_t, 2>);
-      else go(k_sweep_color<LAT, T, kVec, false, uint32_t, 1>);
+      if (off32) go(k_sweep_color<LAT, T, kVec, false, uint32_t>);
+      else go(k_sweep_color<LAT, T, kVec, false, int64_t>);
     });
   });
   return check_hip(hipGetLastError(), "sweep kernel launch");
@@ -328,7 +301,7 @@ int ensure_pinned(spk_hip_lattice *h, int rows) {
   if (rows <= h->pinned_rows) return SPK_HIP_OK;
   int cap = std::max(rows, 2 * h->pinned_rows);
   unsigned long long *p = nullptr;
-  if (hipHostMalloc(&p, (size_t)cap * kAccSlots * 2 * sizeof(unsigned long long), hipHostMallocDefault) !=
+  if (hipHostMalloc(&p, (size_t)cap * 2 * sizeof(unsigned long long), hipHostMallocDefault) !=
       hipSuccess)
     return set_error(SPK_HIP_ENOMEM, "pinned host allocation failed");
   if (h->pinned) {
@@ -346,15 +319,22 @@ int launch_sweeps(spk_hip_lattice *h, int n, unsigned long long *acc, int64_t *n
   int s = 0;
   if (h->fuse)   // whole groups of S sweeps through the fused kernel
     for (; s + h->fuse <= n; s += h->fuse, (*nlaunch)++)
-      if ((rc = launch_fused(h, acc + (size_t)s * kAccSlots * 2))) return rc;
+      if ((rc = launch_fused(h, acc + (size_t)s * kAccRow))) return rc;
   for (; s < n; s++)
     for (int c = 0; c < h->g.K; c++, (*nlaunch)++)
-      if ((rc = launch_sweep(h, c, acc + (size_t)s * kAccSlots * 2))) return rc;
+      if ((rc = launch_sweep(h, c, acc + (size_t)s * kAccRow))) return rc;
   return SPK_HIP_OK;
 }
 
+// folds `rows` accumulator rows at acc into compact (Naccept, dE) pairs at out
+int launch_fold(spk_hip_lattice *h, const unsigned long long *acc, int rows, unsigned long long *out) {
+  hipLaunchKernelGGL(k_fold_rows, dim3(grid_for(rows, 64)), dim3(64), 0, h->stream, acc, rows, kAccSlots, kAccRow,
+                     out);
+  return check_hip(hipGetLastError(), "fold launch");
+}
+
 // the graph of one chunk starting from the current state buffer: zero the
-// chunk's accumulator rows, then its sweeps.  A chunk holds an even number
+// chunk's accumulator rows, then its sweeps, then fold the rows.  A chunk holds an even number
 // of fused launches, so it ends in the buffer it started from.
 int chunk_graph(spk_hip_lattice *h, hipGraphExec_t *out) {
   for (auto &c : h->graphs)
@@ -367,11 +347,12 @@ int chunk_graph(spk_hip_lattice *h, hipGraphExec_t *out) {
   void *start = h->spin;
   if ((rc = check_hip(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal), "begin capture")))
     return rc;
-  const int64_t n = (int64_t)h->chunk * kAccSlots * 2;
+  const int64_t n = (int64_t)h->chunk * kAccRow;
   hipLaunchKernelGGL(k_zero_u64, dim3(grid_for(n, 256)), dim3(256), 0, h->stream, h->acc, n);
   int64_t nl = 0;
   rc = check_hip(hipGetLastError(), "zero launch");
   if (!rc) rc = launch_sweeps(h, h->chunk, h->acc, &nl);
+  if (!rc) rc = launch_fold(h, h->acc, h->chunk, h->accc);
   hipError_t e = hipStreamEndCapture(h->stream, &graph);
   if (rc) return rc;
   if ((rc = check_hip(e, "end capture"))) return rc;
@@ -410,39 +391,40 @@ int ensure_acc(spk_hip_lattice *h, int nsweeps) {
   if (nsweeps <= h->acc_cap) return SPK_HIP_OK;
   int cap = h->acc_cap ? h->acc_cap : kInitialAccRows;
   while (cap < nsweeps) cap *= 2;
-  unsigned long long *p = nullptr;
-  if (hipMalloc(&p, (size_t)cap * kAccSlots * 2 * sizeof(unsigned long long)) != hipSuccess)
+  unsigned long long *p = nullptr, *q = nullptr;
+  if (hipMalloc(&p, (size_t)cap * kAccRow * sizeof(unsigned long long)) != hipSuccess)
     return set_error(SPK_HIP_ENOMEM, "device allocation of %d accumulator rows failed", cap);
+  if (hipMalloc(&q, (size_t)cap * 2 * sizeof(unsigned long long)) != hipSuccess) {
+    (void)hipFree(p);
+    return set_error(SPK_HIP_ENOMEM, "device allocation of %d accumulator rows failed", cap);
+  }
   if (h->acc) {
     (void)hipStreamSynchronize(h->stream);
     (void)hipFree(h->acc);
+    (void)hipFree(h->accc);
   }
-  drop_graphs(h);   // the accumulator address is baked into captured launches
+  drop_graphs(h);   // the accumulator addresses are baked into captured launches
   h->acc = p;
+  h->accc = q;
   h->acc_cap = cap;
-  h->hacc.resize((size_t)cap * kAccSlots * 2);
+  h->hacc.resize((size_t)cap * 2);
   return SPK_HIP_OK;
 }
 
-// sums the slots of accumulator row `row` (already copied to h->hacc)
+// (Naccept, dE) of accumulator row `row`, already folded and copied to h->hacc
 void row_sum(const spk_hip_lattice *h, int row, int64_t *x, int64_t *y) {
-  unsigned long long sx = 0, sy = 0;
-  const unsigned long long *p = h->hacc.data() + (size_t)row * kAccSlots * 2;
-  for (int s = 0; s < kAccSlots; s++) {
-    sx += p[2 * s];
-    sy += p[2 * s + 1];
-  }
-  *x = (int64_t)sx;
-  *y = (int64_t)sy;
+  *x = (int64_t)h->hacc[2 * (size_t)row];
+  *y = (int64_t)h->hacc[2 * (size_t)row + 1];
 }
 
 int compute_energy(spk_hip_lattice *h, int64_t *e) {
   int rc = ensure_acc(h, 1);
   if (rc) return rc;
-  const size_t bytes = kAccSlots * 2 * sizeof(unsigned long long);
-  if ((rc = check_hip(hipMemsetAsync(h->acc, 0, bytes, h->stream), "memset"))) return rc;
-  if ((rc = launch_energy(h, h->acc))) return rc;
-  if ((rc = check_hip(hipMemcpyAsync(h->hacc.data(), h->acc, bytes, hipMemcpyDeviceToHost, h->stream),
+  if ((rc = check_hip(hipMemsetAsync(h->acc, 0, kAccRow * sizeof(unsigned long long), h->stream), "memset")))
+    return rc;
+  if ((rc = launch_energy(h, h->acc)) || (rc = launch_fold(h, h->acc, 1, h->accc))) return rc;
+  if ((rc = check_hip(hipMemcpyAsync(h->hacc.data(), h->accc, 2 * sizeof(unsigned long long),
+                                     hipMemcpyDeviceToHost, h->stream),
                       "energy readback")))
     return rc;
   if ((rc = check_hip(hipStreamSynchronize(h->stream), "energy sync"))) return rc;
@@ -535,6 +517,7 @@ void spk_hip_destroy(spk_hip_lattice *h) {
   if (h->spin_alt) (void)hipFree(h->spin_alt);
   if (h->seed_alt) (void)hipFree(h->seed_alt);
   if (h->acc) (void)hipFree(h->acc);
+  if (h->accc) (void)hipFree(h->accc);
   if (h->ev0) (void)hipEventDestroy(h->ev0);
   if (h->ev1) (void)hipEventDestroy(h->ev1);
   if (h->stream) (void)hipStreamDestroy(h->stream);
@@ -668,35 +651,39 @@ int spk_hip_sweeps(spk_hip_lattice *h, int nsweeps, int64_t *naccept_total, int6
   const int nchunks = C > 0 ? nsweeps / C : 0;
   const int rem = nsweeps - nchunks * C;
   if ((rc = ensure_acc(h, C + rem)) || (rc = ensure_pinned(h, nsweeps))) return rc;
-  const size_t row = kAccSlots * 2 * sizeof(unsigned long long);
-  unsigned long long *rem_acc = h->acc + (size_t)(nchunks ? C : 0) * kAccSlots * 2;
+  const size_t row = 2 * sizeof(unsigned long long);   // a folded row
+  const size_t first_rem = nchunks ? C : 0;             // eager rows follow the chunk's rows
+  unsigned long long *rem_acc = h->acc + first_rem * kAccRow;
   if ((rc = check_hip(hipEventRecord(h->ev0, h->stream), "event"))) return rc;
   int64_t nlaunch = 0;
   for (int k = 0; k < nchunks; k++) {
     hipGraphExec_t exec;
     if ((rc = chunk_graph(h, &exec))) return rc;
     if ((rc = check_hip(hipGraphLaunch(exec, h->stream), "graph launch"))) return rc;
-    if ((rc = check_hip(hipMemcpyAsync(h->pinned + (size_t)k * C * kAccSlots * 2, h->acc, row * C,
-                                       hipMemcpyDeviceToHost, h->stream),
+    if ((rc = check_hip(hipMemcpyAsync(h->pinned + (size_t)k * C * 2, h->accc, row * C, hipMemcpyDeviceToHost,
+                                       h->stream),
                         "accumulator readback")))
       return rc;
   }
   if (nchunks) nlaunch += (int64_t)nchunks * (h->fuse ? C / h->fuse : (int64_t)C * h->g.K);
   if (rem) {
-    if ((rc = check_hip(hipMemsetAsync(rem_acc, 0, row * rem, h->stream), "memset"))) return rc;
+    if ((rc = check_hip(hipMemsetAsync(rem_acc, 0, (size_t)rem * kAccRow * sizeof(unsigned long long), h->stream),
+                        "memset")))
+      return rc;
     if ((rc = launch_sweeps(h, rem, rem_acc, &nlaunch))) return rc;
   }
   if ((rc = check_hip(hipEventRecord(h->ev1, h->stream), "event"))) return rc;
-  if (rem && (rc = check_hip(hipMemcpyAsync(h->pinned + (size_t)nchunks * C * kAccSlots * 2, rem_acc, row * rem,
-                                            hipMemcpyDeviceToHost, h->stream),
-                             "accumulator readback")))
+  if (rem && ((rc = launch_fold(h, rem_acc, rem, h->accc + first_rem * 2)) ||
+              (rc = check_hip(hipMemcpyAsync(h->pinned + (size_t)nchunks * C * 2, h->accc + first_rem * 2, row * rem,
+                                             hipMemcpyDeviceToHost, h->stream),
+                              "accumulator readback"))))
     return rc;
   if ((rc = check_hip(hipStreamSynchronize(h->stream), "sweeps sync"))) return rc;
   float ms = 0.f;
   if ((rc = check_hip(hipEventElapsedTime(&ms, h->ev0, h->ev1), "event time"))) return rc;
   h->last_ms = ms;
   h->last_nlaunch = nlaunch;
-  if (h->hacc.size() < (size_t)nsweeps * kAccSlots * 2) h->hacc.resize((size_t)nsweeps * kAccSlots * 2);
+  if (h->hacc.size() < (size_t)nsweeps * 2) h->hacc.resize((size_t)nsweeps * 2);
   std::memcpy(h->hacc.data(), h->pinned, row * nsweeps);
 
   int64_t nacc = 0, eng = e0;
@@ -750,17 +737,9 @@ int spk_hip_sync(spk_hip_lattice *h) {
 
 int spk_hip_set_launch(spk_hip_lattice *h, int block) {
   if (!h) return set_error(SPK_HIP_EINVAL, "null handle");
-  if (block < 64 || block > 1024 || block % 64)
-    return set_error(SPK_HIP_EINVAL, "block must be a multiple of 64 in [64, 1024]");
+  if (block < 64 || block > 1024 || (block & (block - 1)))
+    return set_error(SPK_HIP_EINVAL, "block must be a power of two in [64, 1024]");
   h->block = block;
-  drop_graphs(h);
-  return SPK_HIP_OK;
-}
-
-int spk_hip_set_packs(spk_hip_lattice *h, int packs) {
-  if (!h) return set_error(SPK_HIP_EINVAL, "null handle");
-  if (packs != 1 && packs != 2) return set_error(SPK_HIP_EINVAL, "packs per thread must be 1 or 2");
-  h->packs = packs;
   drop_graphs(h);
   return SPK_HIP_OK;
 }

@@ -31,7 +31,7 @@ ABI_SYMBOLS = (
     "spk_hip_batch_download", "spk_hip_batch_run", "spk_hip_batch_status",
     "spk_hip_batch_trace", "spk_hip_batch_set_launch", "spk_hip_batch_last_timing",
     "spk_hip_batch_pool", "spk_hip_checksum", "spk_hip_batch_checksum", "spk_hip_serve",
-    "spk_hip_set_fusion", "spk_hip_get_fusion", "spk_hip_set_graph", "spk_hip_set_packs",
+    "spk_hip_set_fusion", "spk_hip_get_fusion", "spk_hip_set_graph",
 )
 
 
@@ -80,7 +80,6 @@ def load_library(path=None):
     L.spk_hip_set_launch.argtypes = [vp, C.c_int]
     L.spk_hip_checksum.argtypes = [vp, C.POINTER(C.c_uint64)]
     L.spk_hip_set_fusion.argtypes = [vp, C.c_int]
-    L.spk_hip_set_packs.argtypes = [vp, C.c_int]
     L.spk_hip_set_graph.argtypes = [vp, C.c_int]
     L.spk_hip_get_fusion.argtypes = [vp, C.POINTER(C.c_int)]
     L.spk_hip_last_timing.argtypes = [vp, C.POINTER(C.c_double), C.POINTER(C.c_int64)]
@@ -234,10 +233,6 @@ class Lattice:
 
     def set_launch(self, block):
         _check(self.L.spk_hip_set_launch(self.h, int(block)))
-
-    def set_packs(self, packs):
-        """4-site packs per thread in the per-colour kernel (1 or 2)."""
-        _check(self.L.spk_hip_set_packs(self.h, int(packs)))
 
     def set_fusion(self, enable):
         _check(self.L.spk_hip_set_fusion(self.h, 1 if enable else 0))

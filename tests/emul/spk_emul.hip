@@ -121,6 +121,21 @@ static int run(int app, int nx, int ny, int nz, int q, double kt, int nsw, const
     s[p] = (T)sn[t];
     k[p] = kn[t];
   }
+  // the per-colour kernel's workgroup tiles cover every pack of a colour
+  // exactly once, for every workgroup size
+  for (int block = 64; block <= 1024; block *= 2) {
+    const TileMap m = make_tile_map(g, block);
+    std::vector<int> seen(g.Mp / 4, 0);
+    for (int tile = 0; tile < m.nxc * m.nyg * m.nzg; tile++)
+      for (int tid = 0; tid < block; tid++) {
+        int a0, b, e;
+        if (!tile_site(g, m, (uint32_t)tile, (uint32_t)tid, a0, b, e)) continue;
+        if (a0 % 4 || a0 >= g.px || b >= g.hy || e >= g.hz) { fprintf(stderr, "tile site out of range\n"); return 11; }
+        seen[(a0 + (long long)g.px * (b + (long long)g.hy * e)) / 4]++;
+      }
+    for (int v : seen)
+      if (v != 1) { fprintf(stderr, "tiles do not cover each pack once (block %d)\n", block); return 12; }
+  }
   std::vector<long long> nacc(nsw), eng(nsw);
   long long na = 0, e = energy<LAT, T>(s, g);
   int w = 0;

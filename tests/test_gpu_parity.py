@@ -61,17 +61,17 @@ CASES = [
 ]
 
 
-@pytest.mark.parametrize("packs", [1, 2])
+@pytest.mark.parametrize("block", [512, 64])
 @pytest.mark.parametrize("case", CASES, ids=lambda c: "%s-%s-%s" % (c[0], c[1], "x".join(map(str, c[2]))))
-def test_small_lattice_bit_exact(orc, case, packs):
-    """Every lattice x app x storage type, with one or two 4-site packs per
-    thread (odd pack counts leave the last thread a single pack)."""
+def test_small_lattice_bit_exact(orc, case, block):
+    """Every lattice x app x storage type, with workgroup tiles of two shapes
+    (partial tiles at the sub-grid's edges)."""
     import spparks_amd as S
     app, lat, shape, q, kt, seed, n = case
-    say("case", case, "packs", packs)
+    say("case", case, "block", block)
     spin0, seeds0, spin1, seeds1, nacc, eng = oracle_run(orc, app, lat, shape, q, kt, seed, n)
     with gpu_init(S, app, lat, shape, q, kt, seed) as L:
-        L.set_packs(packs)
+        L.set_launch(block)
         gs0, gk0 = L.download()
         assert np.array_equal(gs0, spin0), "set site range on device"
         assert np.array_equal(gk0, seeds0), "tagreset seeds on device"
@@ -164,7 +164,7 @@ def test_invalid_values_rejected():
             L.upload(None, np.zeros(64, np.int32))
 
 
-@pytest.mark.parametrize("block", [64, 128, 512, 1024])
+@pytest.mark.parametrize("block", [64, 128, 256, 1024])
 def test_launch_shape_independence(block):
     """colour/strict is decomposition independent: any workgroup size gives
     the same trajectory."""
