@@ -516,7 +516,9 @@ int spk_hip_batch_add(spk_hip_batch *b, const spk_hip_replica_desc *r, int *id) 
   if (variant_app(r->lattice.app))
     return set_error(SPK_HIP_EINVAL, "batched replicas run app styles potts and ising (got app %d)", r->lattice.app);
   if (!offsets_fit_32(R.g))
-    return set_error(SPK_HIP_EINVAL, "a batched replica holds fewer than 2^30 sites (got %lld); run it alone",
+    return set_error(SPK_HIP_EINVAL,
+                     "a batched replica holds fewer than 2^30 sites and rows shorter than 2^25 sites "
+                     "(got %lld sites); run it alone",
                      (long long)R.g.K * R.g.M);
   R.stype = spin_type_for(r->lattice.app, r->lattice.nspins);
   R.variant = R.g.lat * 3 + R.stype;

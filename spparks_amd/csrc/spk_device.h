@@ -299,6 +299,21 @@ __host__ __device__ __forceinline__ int32_t propose(const Rule &r, int32_t s) {
   return s < r.half ? 1 : 2;
 }
 
+// propose() for a kernel that knows the state count at compile time.  With
+// two states Potts' (int)(2u)+1 is 1 exactly when u < 0.5 (2u is exact), so
+// both apps reduce to Ising's test against the integer threshold `half`;
+// otherwise the app is Potts.
+template <bool TWO>
+__host__ __device__ __forceinline__ int32_t propose_as(const Rule &r, int32_t s) {
+  if constexpr (TWO) {
+    return s < r.half ? 1 : 2;
+  } else {
+    double u = kAM * (double)s;
+    int32_t p = (int32_t)((double)r.nspins * u) + 1;
+    return p > r.nspins ? r.nspins : p;
+  }
+}
+
 // The accept/reject tail of site_event_rejection (src/app_potts.cpp:125-133):
 // ef <= ei accepts; T == 0 rejects an uphill move without a second draw;
 // otherwise a second draw s2 rejects when AM*s2 > exp(-(ef-ei)/kT), i.e.
@@ -309,6 +324,21 @@ __host__ __device__ __forceinline__ bool metropolis(const Rule &r, const int *th
   if (r.zero_t) return false;
   s = pm_step(s);
   return s < thr[ef - ei];
+}
+
+// metropolis() without branches, for the packed kernels: the second draw is
+// always taken and kept only for an uphill move at T > 0 (a wavefront nearly
+// always holds an uphill lane, so a branch around the draw saves nothing and
+// costs scalar instructions per site).  Same result and seed as metropolis().
+__host__ __device__ __forceinline__ bool metropolis_select(const Rule &r, const int *thr, int ei, int ef,
+                                                           int32_t &s) {
+  const int d = ef - ei;
+  const bool up = d > 0;
+  const bool draw = up && !r.zero_t;
+  const int32_t s2 = pm_step(s);
+  const bool pass = s2 < thr[up ? d : 0];
+  s = draw ? s2 : s;
+  return !up || (draw && pass);
 }
 
 struct SiteOutcome {
@@ -515,9 +545,10 @@ __host__ __device__ __forceinline__ Counts variant_site(T *__restrict__ spin, in
 // lane (see Tally).
 
 // True when every element offset of the lattice's spin and seed arrays
-// (K*Mp elements, at most 4 bytes each) fits a 32-bit byte offset.
+// (K*Mp elements, at most 4 bytes each) fits a 32-bit byte offset, and the
+// row pitch and row count are below 2^24 (24-bit row products, RowOffset).
 __host__ __device__ inline bool offsets_fit_32(const Geom &g) {
-  return (uint64_t)g.K * (uint64_t)g.Mp * 4u < (1ull << 32);
+  return (uint64_t)g.K * (uint64_t)g.Mp * 4u < (1ull << 32) && g.px < (1 << 24) && g.hy < (1 << 24);
 }
 
 // 32-bit or 64-bit element offsets.  With 32-bit offsets the byte offset is
@@ -793,55 +824,84 @@ struct PackWork {
   Tally<T, TWO> tally;
 };
 
+// The row pitch px times a row index.  For 32-bit offsets the product is a
+// 24-bit multiply (v_mul_u32_u24): exact while both factors are below 2^24,
+// which offsets_fit_32 requires of px and of the row count hy.
+template <typename I>
+struct RowOffset {
+  I pitch;
+  __host__ __device__ __forceinline__ explicit RowOffset(I p) : pitch(p) {}
+  __host__ __device__ __forceinline__ I operator*(I idx) const {
+#ifdef __HIP_DEVICE_COMPILE__
+    if constexpr (sizeof(I) == 4) return (I)__umul24((unsigned)pitch, (unsigned)idx);
+#endif
+    return pitch * idx;
+  }
+};
+
+// The neighbour words of a pack: NSEG row segments (a pack plus the element
+// beyond it on the side `off` names) and NPACK plain packs.
+template <int LAT>
+struct StencilWords {
+  static constexpr int NSEG = LAT == SPK_HIP_SQ_8N ? 3 : LAT == SPK_HIP_SC_26N ? 9 : 1;
+  static constexpr int NPACK = LAT == SPK_HIP_SC_6N ? 4 : LAT == SPK_HIP_SC_26N ? 8 : 2;
+};
+
+template <typename T, int NSEG, int NPACK>
+struct RawWords {
+  Lanes<T> seg[NSEG];
+  uint32_t extra[NSEG];
+  int off[NSEG];
+  Lanes<T> pk[NPACK];
+};
+
 template <int LAT, typename T, bool TWO, typename I>
 __host__ __device__ __forceinline__ PackWork<T, TWO, I> pack_count(const T *__restrict__ spin,
                                                                    const int32_t *__restrict__ seed,
                                                                    const Geom &g, const Rule &r, int c, int a0,
                                                                    int b, int e) {
+  using SW = StencilWords<LAT>;
   PackWork<T, TWO, I> w;
-  const I px = (I)g.px, pxy = (I)g.px * (I)g.hy, Mp = (I)g.Mp;
+  const I Mp = (I)g.Mp;
+  // row offsets px*y + pxy*z; px*y as a 24-bit multiply (see RowOffset)
+  const RowOffset<I> px((I)g.px);
+  const I pxy = (I)g.px * (I)g.hy;
   // colour q's sub-grid: a base that is the same for the whole launch, so
   // loads of one row from several colours share one per-thread offset
   auto colour = [&](int q) { return elem<I>(spin, (I)q * Mp); };
   w.self = (I)a0 + px * (I)b + pxy * (I)e;
   w.k = g.hx - a0;
-  w.old = load_lanes<T, I>(colour(c), w.self);
-  {
-    const Words<4> sd = *reinterpret_cast<const Words<4> *>(elem<I>(elem<I>(seed, (I)c * Mp), w.self));
-#pragma unroll
-    for (int v = 0; v < 4; v++) {
-      w.s[v] = pm_step((int32_t)sd.w[v]);
-      w.pr[v] = propose(r, w.s[v]);
-    }
-  }
-  const Lanes<T> &old = w.old;
-  const Lanes<T> prop = Lanes<T>::pack(w.pr);
-  Tally<T, TWO> &tally = w.tally;
-  const bool ragged = (g.hx & 3) != 0;   // same for the whole launch
-  const RowEnd<T> rowend(w.k);
 
-  auto count_pack = [&](const T *base, I row) { tally.add(load_lanes<T, I>(base, row + (I)a0), old, prop); };
-  auto count_segment = [&](const T *base, I row, int off) {
-    const Lanes<T> P = load_lanes<T, I>(base, row + (I)a0);
-    const int xe = off ? (a0 + 4 >= g.hx ? 0 : a0 + 4) : (a0 == 0 ? g.hx - 1 : a0 - 1);
-    const uint32_t E = (uint32_t)(*elem<I>(base, row + (I)xe)) & Lanes<T>::MASK;
-    Lanes<T> S = shifted<T>(P, E, off);
-    if (ragged && off) rowend.apply(S, E);
-    tally.add2(P, S, old, prop);
+  // ---- load phase: the offsets of every word the pack needs first, then
+  // every load, before any word is used: one memory round trip per pack
+  RawWords<T, SW::NSEG, SW::NPACK> raw;
+  const T *sbase[SW::NSEG], *pbase[SW::NPACK];
+  I srow[SW::NSEG], prow[SW::NPACK];
+  int ns = 0, np = 0;
+  const int right = a0 + 4 >= g.hx ? 0 : a0 + 4, left = a0 == 0 ? g.hx - 1 : a0 - 1;
+  auto seg = [&](const T *base, I row, int off) {
+    sbase[ns] = base;
+    srow[ns] = row;
+    raw.off[ns] = off;
+    ns++;
   };
-
+  auto pack = [&](const T *base, I row) {
+    pbase[np] = base;
+    prow[np] = row;
+    np++;
+  };
   if constexpr (LAT == SPK_HIP_SQ_4N) {
     const T *o = colour(1 - c);
-    count_segment(o, px * (I)b, (b + c) & 1);
-    count_pack(o, px * (I)wrap_dec(b, g.hy));
-    count_pack(o, px * (I)wrap_inc(b, g.hy));
+    seg(o, px * (I)b, (b + c) & 1);
+    pack(o, px * (I)wrap_dec(b, g.hy));
+    pack(o, px * (I)wrap_inc(b, g.hy));
   } else if constexpr (LAT == SPK_HIP_SC_6N) {
     const T *o = colour(1 - c);
-    count_segment(o, px * (I)b + pxy * (I)e, (b + e + c) & 1);
-    count_pack(o, px * (I)wrap_dec(b, g.hy) + pxy * (I)e);
-    count_pack(o, px * (I)wrap_inc(b, g.hy) + pxy * (I)e);
-    count_pack(o, px * (I)b + pxy * (I)wrap_dec(e, g.hz));
-    count_pack(o, px * (I)b + pxy * (I)wrap_inc(e, g.hz));
+    seg(o, px * (I)b + pxy * (I)e, (b + e + c) & 1);
+    pack(o, px * (I)wrap_dec(b, g.hy) + pxy * (I)e);
+    pack(o, px * (I)wrap_inc(b, g.hy) + pxy * (I)e);
+    pack(o, px * (I)b + pxy * (I)wrap_dec(e, g.hz));
+    pack(o, px * (I)b + pxy * (I)wrap_inc(e, g.hz));
   } else if constexpr (LAT == SPK_HIP_SQ_8N) {
     const int pi = c & 1, pj = c >> 1;
     const int ys[3] = {pj ? b : wrap_dec(b, g.hy), b, pj ? wrap_inc(b, g.hy) : b};
@@ -849,8 +909,8 @@ __host__ __device__ __forceinline__ PackWork<T, TWO, I> pack_count(const T *__re
     for (int dy = 0; dy < 3; dy++) {
       const int qy = pj ^ (dy != 1);
       const I row = px * (I)ys[dy];
-      if (dy != 1) count_pack(colour(pi + 2 * qy), row);
-      count_segment(colour((1 - pi) + 2 * qy), row, pi);
+      if (dy != 1) pack(colour(pi + 2 * qy), row);
+      seg(colour((1 - pi) + 2 * qy), row, pi);
     }
   } else {
     const int pi = c & 1, pj = (c >> 1) & 1, pk = c >> 2;
@@ -862,13 +922,41 @@ __host__ __device__ __forceinline__ PackWork<T, TWO, I> pack_count(const T *__re
       for (int dy = 0; dy < 3; dy++) {
         const int qyz = 2 * (pj ^ (dy != 1)) + 4 * (pk ^ (dz != 1));
         const I row = px * (I)ys[dy] + pxy * (I)zs[dz];
-        if (dy != 1 || dz != 1) count_pack(colour(pi + qyz), row);
-        count_segment(colour((1 - pi) + qyz), row, pi);
-        // one z-plane's loads in flight at a time: bounds the live words
-        if (dy == 2) fence_schedule();
+        if (dy != 1 || dz != 1) pack(colour(pi + qyz), row);
+        seg(colour((1 - pi) + qyz), row, pi);
       }
   }
+  w.old = load_lanes<T, I>(colour(c), w.self);
+  const Words<4> sd = *reinterpret_cast<const Words<4> *>(elem<I>(elem<I>(seed, (I)c * Mp), w.self));
+#pragma unroll
+  for (int m = 0; m < SW::NSEG; m++) {
+    raw.seg[m] = load_lanes<T, I>(sbase[m], srow[m] + (I)a0);
+    raw.extra[m] = (uint32_t)(*elem<I>(sbase[m], srow[m] + (I)(raw.off[m] ? right : left))) & Lanes<T>::MASK;
+  }
+#pragma unroll
+  for (int m = 0; m < SW::NPACK; m++) raw.pk[m] = load_lanes<T, I>(pbase[m], prow[m] + (I)a0);
+  fence_schedule();   // no counting work may be scheduled between the loads
 
+  // ---- count phase: first draws and proposals, then every word compared
+  // with the old and the proposed lanes
+#pragma unroll
+  for (int v = 0; v < 4; v++) {
+    w.s[v] = pm_step((int32_t)sd.w[v]);
+    w.pr[v] = propose_as<TWO>(r, w.s[v]);
+  }
+  const Lanes<T> prop = Lanes<T>::pack(w.pr);
+  const bool ragged = (g.hx & 3) != 0;   // same for the whole launch
+#pragma unroll
+  for (int m = 0; m < SW::NSEG; m++) {
+    Lanes<T> S = shifted<T>(raw.seg[m], raw.extra[m], raw.off[m]);
+    if (ragged) {
+      const RowEnd<T> rowend(raw.off[m] ? w.k : 5);   // only the +x side wraps inside a pack
+      rowend.apply(S, raw.extra[m]);
+    }
+    w.tally.add2(raw.seg[m], S, w.old, prop);
+  }
+#pragma unroll
+  for (int m = 0; m < SW::NPACK; m++) w.tally.add(raw.pk[m], w.old, prop);
   return w;
 }
 
@@ -895,7 +983,7 @@ __host__ __device__ __forceinline__ Counts pack_decide(PackWork<T, TWO, I> &w, T
     const int o_v = old.get(v);
     const int ei = tally.ei(v);
     const int ef = TWO ? (pr[v] == o_v ? ei : NN - ei) : tally.ef(v);
-    const bool accept = metropolis(r, thr, ei, ef, s[v]);
+    const bool accept = metropolis_select(r, thr, ei, ef, s[v]);
     const bool changed = v < k && accept && pr[v] != o_v;   // pad lanes count nothing
     if (changed) nsp.set(v, (uint32_t)pr[v]);
     nsd.w[v] = (uint32_t)s[v];

@@ -31,16 +31,21 @@ template <int LAT, typename T, int V, bool TWO, typename I>
 __global__ __launch_bounds__(1024) void k_sweep_color(T *__restrict__ spin, int32_t *__restrict__ seed,
                                                       Geom g, Rule r, TileMap m, int color,
                                                       unsigned long long *__restrict__ acc) {
-  __shared__ int s_thr[32];
-  if (threadIdx.x < 28) s_thr[threadIdx.x] = r.thr[threadIdx.x];
-  __syncthreads();
-
   static_assert(V == 4, "tiles map 4 sites per thread");
+  __shared__ int s_thr[32];
   const int tile = xcd_tile(blockIdx.x, gridDim.x);
-  Counts cnt{0, 0};
   int a0, b, e;
-  if (tile < m.nxc * m.nyg * m.nzg && tile_site(g, m, (uint32_t)tile, threadIdx.x, a0, b, e))
-    cnt = sweep_sites<LAT, T, V, TWO, I>(spin, seed, g, r, s_thr, color, a0, b, e);
+  const bool live = tile < m.nxc * m.nyg * m.nzg && tile_site(g, m, (uint32_t)tile, threadIdx.x, a0, b, e);
+  // the thresholds are staged in LDS between counting and deciding, so the
+  // pack's loads are not held behind a barrier
+  const int thr_v = threadIdx.x < 28 ? r.thr[threadIdx.x] : 0;
+  PackWork<T, TWO, I> w;
+  if (live) w = pack_count<LAT, T, TWO, I>(spin, seed, g, r, color, a0, b, e);
+  if (threadIdx.x < 28) s_thr[threadIdx.x] = thr_v;
+  __syncthreads();
+  Counts cnt{0, 0};
+  if (live) cnt = pack_decide<LAT, T, TWO, I>(w, spin, seed, g, r, s_thr, color);
+  __syncthreads();   // block_accumulate reuses LDS after the decide step's reads
   block_accumulate(cnt.nacc, cnt.de, acc, blockIdx.x & (kAccSlots - 1));
 }
 
