@@ -483,7 +483,8 @@ int spk_hip_create(const spk_hip_desc *d, spk_hip_lattice **out) {
   h->N = (int64_t)g.M * g.K;
   h->slots = (int64_t)g.Mp * g.K;
   h->stype = spin_type_for(d->app, d->nspins);
-  h->block = 512;   // measured best on MI355X for 4096^2 sq/4n and 256^3 sc/26n
+  // measured best on MI355X: 512 for 4096^2 sq/4n, 256 for 256^3 sc/26n
+  h->block = (g.lat == SPK_HIP_SQ_4N || g.lat == SPK_HIP_SQ_8N) ? 512 : 256;
   if ((rc = check_hip(hipGetDevice(&h->device), "hipGetDevice")) ||
       (rc = check_hip(hipStreamCreate(&h->stream), "hipStreamCreate")) ||
       (rc = check_hip(hipEventCreate(&h->ev0), "hipEventCreate")) ||
