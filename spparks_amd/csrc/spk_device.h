@@ -134,6 +134,7 @@ struct TileMap {
   int lx, ly, lz;      // log2 extents of a tile in threads / rows / planes
   int nxc, nyg, nzg;   // tiles along x, y, z
   FastDiv fxc, fyg;    // division by nxc, nyg
+  int linear;          // 1: tile t is packs t*block .. t*block+block-1 in pack order
 };
 
 __host__ __device__ inline int ceil_log2(int v) {
@@ -157,6 +158,18 @@ __host__ __device__ inline TileMap make_tile_map(const Geom &g, int block) {
   m.nzg = (g.hz + (1 << m.lz) - 1) >> m.lz;
   m.fxc = fastdiv_make((uint32_t)m.nxc);
   m.fyg = fastdiv_make((uint32_t)m.nyg);
+  // box tiles round the row to a power of two of threads and cut partial
+  // tiles at the edges; where that idles more than a tenth of the launched
+  // threads (short or odd rows), consecutive packs waste less
+  const int64_t packs = (int64_t)g.Mp / 4;
+  const int64_t launched = (int64_t)m.nxc * m.nyg * m.nzg * block;
+  m.linear = launched * 9 > packs * 10;
+  if (m.linear) {
+    m.lx = lb;   // tile_site: tile t starts at pack t << lb
+    m.ly = m.lz = 0;
+    m.nxc = (int)((packs + block - 1) / block);
+    m.nyg = m.nzg = 1;
+  }
   return m;
 }
 
@@ -164,6 +177,12 @@ __host__ __device__ inline TileMap make_tile_map(const Geom &g, int block) {
 // slot outside the sub-grid (or a thread beyond the tile's extent)
 __host__ __device__ __forceinline__ bool tile_site(const Geom &g, const TileMap &m, uint32_t tile, uint32_t tid,
                                                    int &a0, int &b, int &e) {
+  if (m.linear) {
+    const uint32_t t = (tile << m.lx) + tid;
+    if (t >= (uint32_t)(g.Mp / 4)) return false;
+    pack_site(g, t, a0, b, e);
+    return true;
+  }
   uint32_t rest, xc, zg, yg;
   fastdiv(m.fxc, tile, rest, xc);
   fastdiv(m.fyg, rest, zg, yg);
