@@ -267,7 +267,7 @@ int replica_energy(spk_hip_batch *b, const Replica &R, int64_t *e) {
     });
   });
   if ((rc = check_hip(hipGetLastError(), "energy launch"))) return rc;
-  hipLaunchKernelGGL(k_fold_rows, dim3(1), dim3(64), 0, b->stream, b->scratch, 1, kAccSlots, kAccRow,
+  hipLaunchKernelGGL(k_fold_rows, dim3(1), dim3(64), 0, b->stream, b->scratch, kAccSlots, kAccRow,
                      b->scratch + kAccRow);
   if ((rc = check_hip(hipGetLastError(), "fold launch"))) return rc;
   unsigned long long h[2];
@@ -744,7 +744,7 @@ int spk_hip_batch_run(spk_hip_batch *b, int max_sweeps, int use_graph) {
 
   // per-replica Naccept / energy per sweep of the block
   b->h_acc.resize(2 * (size_t)rows);
-  hipLaunchKernelGGL(k_fold_rows, dim3(grid_for(rows, 64)), dim3(64), 0, b->stream, b->d_acc, rows, kBatchSlots,
+  hipLaunchKernelGGL(k_fold_rows, dim3(rows), dim3(64), 0, b->stream, b->d_acc, kBatchSlots,
                      kBatchRow, b->d_accc);
   if ((rc = check_hip(hipGetLastError(), "fold launch"))) return rc;
   if ((rc = check_hip(hipMemcpyAsync(b->h_acc.data(), b->d_accc, 2 * (size_t)rows * sizeof(unsigned long long),

@@ -58,18 +58,22 @@ __global__ void k_to_natural(const T *__restrict__ blk, int32_t *__restrict__ na
 
 // Compacts accumulator rows: out[2r], out[2r+1] = the sums of row r's
 // `slots` slots (row r at acc + r*row_words, slot k at +k*kSlotStride).
-__global__ void k_fold_rows(const unsigned long long *__restrict__ acc, int rows, int slots, int row_words,
-                            unsigned long long *__restrict__ out) {
-  const int r = blockIdx.x * blockDim.x + threadIdx.x;
-  if (r >= rows) return;
-  const unsigned long long *p = acc + (size_t)r * row_words;
+__global__ __launch_bounds__(64) void k_fold_rows(const unsigned long long *__restrict__ acc, int slots,
+                                                  int row_words, unsigned long long *__restrict__ out) {
+  // one wave per row: lane k holds slot k (and k+64, ...), so a row costs
+  // one round trip to L2 instead of `slots` dependent ones
+  const unsigned long long *p = acc + (size_t)blockIdx.x * row_words;
   unsigned long long x = 0, y = 0;
-  for (int k = 0; k < slots; k++) {
+  for (int k = threadIdx.x; k < slots; k += 64) {
     x += p[(size_t)k * kSlotStride];
     y += p[(size_t)k * kSlotStride + 1];
   }
-  out[2 * (size_t)r] = x;
-  out[2 * (size_t)r + 1] = y;
+  x = wave_sum64(x);
+  y = wave_sum64(y);
+  if (threadIdx.x == 0) {
+    out[2 * (size_t)blockIdx.x] = x;
+    out[2 * (size_t)blockIdx.x + 1] = y;
+  }
 }
 
 // counts values outside [lo, hi]

@@ -333,7 +333,7 @@ int launch_sweeps(spk_hip_lattice *h, int n, unsigned long long *acc, int64_t *n
 
 // folds `rows` accumulator rows at acc into compact (Naccept, dE) pairs at out
 int launch_fold(spk_hip_lattice *h, const unsigned long long *acc, int rows, unsigned long long *out) {
-  hipLaunchKernelGGL(k_fold_rows, dim3(grid_for(rows, 64)), dim3(64), 0, h->stream, acc, rows, kAccSlots, kAccRow,
+  hipLaunchKernelGGL(k_fold_rows, dim3(rows), dim3(64), 0, h->stream, acc, kAccSlots, kAccRow,
                      out);
   return check_hip(hipGetLastError(), "fold launch");
 }
