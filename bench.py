@@ -121,7 +121,7 @@ def run_ensemble(args):
         with bmod.Batch(pool, len(reps)) as B:
             fill(B)
             B.set_launch(0, 0)
-            B.run(4, graph=False)
+            B.run(24, graph=False)   # about 20 colour steps per candidate shape
             t = B.last_timing()
             shape = (t["block"], t["grid"])
 

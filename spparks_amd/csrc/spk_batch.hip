@@ -95,7 +95,7 @@ __global__ __launch_bounds__(1024) void k_batch_step(char *__restrict__ pool, co
         cnt = sweep_sites<LAT, T, kVec, false, uint32_t>(sp, sdp, g, rule, s_thr, p % K, a0, b, e);
     }
     // tiles of one replica pass spread over the row's slots (one line each)
-    block_accumulate(cnt.nacc, cnt.de, acc + (int64_t)(rep.acc_row + p / K) * kBatchRow, t & (kBatchSlots - 1));
+    block_accumulate_pack(cnt.nacc, cnt.de, acc + (int64_t)(rep.acc_row + p / K) * kBatchRow, t & (kBatchSlots - 1));
   }
 }
 

@@ -644,18 +644,35 @@ __host__ __device__ __forceinline__ void store_lanes(T *base, I idx, const Lanes
   *reinterpret_cast<Words<Lanes<T>::NW> *>(elem<I>(base, idx)) = x;
 }
 
+// Byte i of the result is byte sel.byte[i] (0..7) of the 8 bytes hi:lo,
+// bytes 0-3 being lo's: the v_perm_b32 instruction.
+__host__ __device__ __forceinline__ uint32_t byte_perm(uint32_t hi, uint32_t lo, uint32_t sel) {
+#ifdef __HIP_DEVICE_COMPILE__
+  return __builtin_amdgcn_perm(hi, lo, sel);
+#else
+  const uint64_t v = (uint64_t)hi << 32 | lo;
+  uint32_t r = 0;
+  for (int i = 0; i < 4; i++) r |= (uint32_t)(v >> (8 * ((sel >> (8 * i)) & 7)) & 0xffu) << (8 * i);
+  return r;
+#endif
+}
+
 // Lanes of positions p-1 (off 0) or p+1 (off 1), p = a0+v, from the lanes P
 // of positions a0..a0+3 and the element E one beyond them on that side (the
-// periodic image: a0-1 or hx-1; a0+4 or 0).
+// periodic image: a0-1 or hx-1; a0+4 or 0), held in E's low bits (higher
+// bits are ignored).  Narrow lanes take byte permutes whose selector is the
+// only thing that depends on the side, so a side that is the same for the
+// whole launch costs one permute per word.
 template <typename T>
 __host__ __device__ __forceinline__ Lanes<T> shifted(const Lanes<T> &P, uint32_t E, int off) {
   Lanes<T> S;
   if constexpr (Lanes<T>::NW == 1) {
-    S.w[0] = off ? (P.w[0] >> 8) | (E << 24) : (P.w[0] << 8) | E;
+    // off 1: P bytes 1, 2, 3 then E's byte 0; off 0: E's byte 0 then P bytes 0, 1, 2
+    S.w[0] = byte_perm(E, P.w[0], off ? 0x04030201u : 0x02010004u);
   } else if constexpr (Lanes<T>::NW == 2) {
-    const uint32_t mid = (P.w[0] >> 16) | (P.w[1] << 16);   // lanes 1, 2
-    S.w[0] = off ? mid : (P.w[0] << 16) | E;
-    S.w[1] = off ? (P.w[1] >> 16) | (E << 16) : mid;
+    // off 1: (P0 hi, P1 lo) and (P1 hi, E); off 0: (E, P0 lo) and (P0 hi, P1 lo)
+    S.w[0] = byte_perm(off ? P.w[1] : E, P.w[0], off ? 0x05040302u : 0x01000504u);
+    S.w[1] = byte_perm(off ? E : P.w[0], P.w[1], off ? 0x05040302u : 0x01000706u);
   } else {
 #pragma unroll
     for (int v = 0; v < 4; v++) {
@@ -681,6 +698,7 @@ struct RowEnd {
     }
   }
   __host__ __device__ __forceinline__ void apply(Lanes<T> &S, uint32_t E) const {
+    E &= Lanes<T>::MASK;
     uint32_t Ew = E;
     if constexpr (Lanes<T>::NW == 1) Ew = E * 0x01010101u;
     else if constexpr (Lanes<T>::NW == 2) Ew = E | (E << 16);
@@ -807,9 +825,16 @@ struct Tally {
         add_ne_words2(o[i], x.w[i] ^ old.w[i], y.w[i] ^ old.w[i]);
         if constexpr (!TWO) add_ne_words2(p[i], x.w[i] ^ prop.w[i], y.w[i] ^ prop.w[i]);
       }
+    } else if constexpr (kXorSum) {
+      o[0] += (x.w[0] ^ old.w[0]) + (y.w[0] ^ old.w[0]);
+      keep_sum(o[0]);
     } else {
-      add(x, old, prop);
-      add(y, old, prop);
+      o[0] += ne_bytes(x.w[0] ^ old.w[0]) + ne_bytes(y.w[0] ^ old.w[0]);
+      keep_sum(o[0]);
+      if constexpr (!TWO) {
+        p[0] += ne_bytes(x.w[0] ^ prop.w[0]) + ne_bytes(y.w[0] ^ prop.w[0]);
+        keep_sum(p[0]);
+      }
     }
   }
   __host__ __device__ __forceinline__ int ei(int v) const {
@@ -874,7 +899,10 @@ struct RawWords {
   Lanes<T> pk[NPACK];
 };
 
-template <int LAT, typename T, bool TWO, typename I>
+// XDIR: the side (0, 1) every segment shifts towards, when the caller knows
+// it for the whole launch (sq/8n and sc/26n: the colour's x parity); -1
+// takes each segment's own.
+template <int LAT, typename T, bool TWO, typename I, int XDIR = -1>
 __host__ __device__ __forceinline__ PackWork<T, TWO, I> pack_count(const T *__restrict__ spin,
                                                                    const int32_t *__restrict__ seed,
                                                                    const Geom &g, const Rule &r, int c, int a0,
@@ -950,7 +978,8 @@ __host__ __device__ __forceinline__ PackWork<T, TWO, I> pack_count(const T *__re
 #pragma unroll
   for (int m = 0; m < SW::NSEG; m++) {
     raw.seg[m] = load_lanes<T, I>(sbase[m], srow[m] + (I)a0);
-    raw.extra[m] = (uint32_t)(*elem<I>(sbase[m], srow[m] + (I)(raw.off[m] ? right : left))) & Lanes<T>::MASK;
+    const int off = XDIR >= 0 ? XDIR : raw.off[m];
+    raw.extra[m] = (uint32_t)(*elem<I>(sbase[m], srow[m] + (I)(off ? right : left)));
   }
 #pragma unroll
   for (int m = 0; m < SW::NPACK; m++) raw.pk[m] = load_lanes<T, I>(pbase[m], prow[m] + (I)a0);
@@ -965,17 +994,20 @@ __host__ __device__ __forceinline__ PackWork<T, TWO, I> pack_count(const T *__re
   }
   const Lanes<T> prop = Lanes<T>::pack(w.pr);
   const bool ragged = (g.hx & 3) != 0;   // same for the whole launch
-#pragma unroll
-  for (int m = 0; m < SW::NSEG; m++) {
-    Lanes<T> S = shifted<T>(raw.seg[m], raw.extra[m], raw.off[m]);
+  // segment m: its pack and the pack shifted one site towards side `off`
+  auto segment = [&](int m, int off) {
+    Lanes<T> S = shifted<T>(raw.seg[m], raw.extra[m], off);
     if (ragged) {
-      const RowEnd<T> rowend(raw.off[m] ? w.k : 5);   // only the +x side wraps inside a pack
+      const RowEnd<T> rowend(off ? w.k : 5);   // only the +x side wraps inside a pack
       rowend.apply(S, raw.extra[m]);
     }
     w.tally.add2(raw.seg[m], S, w.old, prop);
-  }
+  };
 #pragma unroll
-  for (int m = 0; m < SW::NPACK; m++) w.tally.add(raw.pk[m], w.old, prop);
+  for (int m = 0; m < SW::NSEG; m++) segment(m, XDIR >= 0 ? XDIR : raw.off[m]);
+  static_assert(SW::NPACK % 2 == 0, "plain packs are counted in pairs");
+#pragma unroll
+  for (int m = 0; m < SW::NPACK; m += 2) w.tally.add2(raw.pk[m], raw.pk[m + 1], w.old, prop);
   return w;
 }
 
@@ -1214,6 +1246,32 @@ __device__ __forceinline__ void block_accumulate(int x, int y, unsigned long lon
   __syncthreads();
   if (threadIdx.x == 0) {
     int sx = 0, sy = 0;   // a block's totals stay far below 2^31 as well
+    const int nw = blockDim.x >> 6;
+    for (int w = 0; w < nw; w++) {
+      sx += red[0][w];
+      sy += red[1][w];
+    }
+    unsigned long long *a = acc + (size_t)slot * kSlotStride;
+    if (sx) atomicAdd(&a[0], (unsigned long long)(long long)sx);
+    if (sy) atomicAdd(&a[1], (unsigned long long)(long long)sy);
+  }
+}
+
+// block_accumulate for small per-thread counts: 0 <= x, 64 * x < 2^16 and
+// 64 * |y| < 2^15 (one pack of 4 sites: x <= 4, |y| <= 4 * 26).  A wave's
+// two sums then fit one 32-bit word, x in the low half and y in the high
+// half, so one wave reduction carries both.
+__device__ __forceinline__ void block_accumulate_pack(int x, int y, unsigned long long *acc, int slot) {
+  __shared__ int red[2][16];
+  const int v = wave_sum((int)((uint32_t)x + ((uint32_t)y << 16)));
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  if (lane == 0) {
+    red[0][wid] = v & 0xffff;
+    red[1][wid] = v >> 16;   // arithmetic: the high half is the signed sum of y
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int sx = 0, sy = 0;
     const int nw = blockDim.x >> 6;
     for (int w = 0; w < nw; w++) {
       sx += red[0][w];

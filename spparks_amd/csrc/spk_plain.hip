@@ -27,7 +27,7 @@ using namespace spk;
 // unit-stride.  Each workgroup covers one TileMap tile (rows of its own
 // waves supply most of the y/z neighbour rows); tiles are remapped so that
 // neighbouring tiles share an XCD.
-template <int LAT, typename T, int V, bool TWO, typename I>
+template <int LAT, typename T, int V, bool TWO, typename I, int XDIR>
 __global__ __launch_bounds__(1024) void k_sweep_color(T *__restrict__ spin, int32_t *__restrict__ seed,
                                                       Geom g, Rule r, TileMap m, int color,
                                                       unsigned long long *__restrict__ acc) {
@@ -40,13 +40,13 @@ __global__ __launch_bounds__(1024) void k_sweep_color(T *__restrict__ spin, int3
   // pack's loads are not held behind a barrier
   const int thr_v = threadIdx.x < 28 ? r.thr[threadIdx.x] : 0;
   PackWork<T, TWO, I> w;
-  if (live) w = pack_count<LAT, T, TWO, I>(spin, seed, g, r, color, a0, b, e);
+  if (live) w = pack_count<LAT, T, TWO, I, XDIR>(spin, seed, g, r, color, a0, b, e);
   if (threadIdx.x < 28) s_thr[threadIdx.x] = thr_v;
   __syncthreads();
   Counts cnt{0, 0};
   if (live) cnt = pack_decide<LAT, T, TWO, I>(w, spin, seed, g, r, s_thr, color);
   __syncthreads();   // block_accumulate reuses LDS after the decide step's reads
-  block_accumulate(cnt.nacc, cnt.de, acc, blockIdx.x & (kAccSlots - 1));
+  block_accumulate_pack(cnt.nacc, cnt.de, acc, blockIdx.x & (kAccSlots - 1));
 }
 
 // One colour pass of a proposal variant (potts/neigh, potts/neighonly,
@@ -63,7 +63,7 @@ __global__ __launch_bounds__(1024) void k_sweep_variant(T *__restrict__ spin, in
   const int64_t t = (int64_t)tile * blockDim.x + threadIdx.x;
   Counts cnt{0, 0};
   if (t < (int64_t)g.Mp) cnt = variant_site<LAT, T, APP>(spin, seed, g, r, s_thr, color, (uint32_t)t);
-  block_accumulate(cnt.nacc, cnt.de, acc, blockIdx.x & (kAccSlots - 1));
+  block_accumulate_pack(cnt.nacc, cnt.de, acc, blockIdx.x & (kAccSlots - 1));
 }
 
 // S whole sweeps of a 2d lattice per launch, one extended tile per
@@ -225,15 +225,25 @@ int launch_sweep(spk_hip_lattice *h, int color, unsigned long long *acc) {
         hipLaunchKernelGGL(kern, dim3(grid), dim3(bs), 0, h->stream, static_cast<T *>(h->spin), h->seed, h->g,
                            h->rule, m, color, acc);
       };
+      // sq/8n and sc/26n: the x-shift side is the colour's x parity, a
+      // constant of the launch, so it is a template argument
+      auto with_x = [&]<bool TWO, typename I>() {
+        if constexpr (LAT == SPK_HIP_SQ_8N || LAT == SPK_HIP_SC_26N) {
+          if (color & 1) go(k_sweep_color<LAT, T, kVec, TWO, I, 1>);
+          else go(k_sweep_color<LAT, T, kVec, TWO, I, 0>);
+        } else {
+          go(k_sweep_color<LAT, T, kVec, TWO, I, -1>);
+        }
+      };
       if constexpr (sizeof(T) == 1) {
         if (two) {
-          if (off32) go(k_sweep_color<LAT, T, kVec, true, uint32_t>);
-          else go(k_sweep_color<LAT, T, kVec, true, int64_t>);
+          if (off32) with_x.template operator()<true, uint32_t>();
+          else with_x.template operator()<true, int64_t>();
           return;
         }
       }
-      if (off32) go(k_sweep_color<LAT, T, kVec, false, uint32_t>);
-      else go(k_sweep_color<LAT, T, kVec, false, int64_t>);
+      if (off32) with_x.template operator()<false, uint32_t>();
+      else with_x.template operator()<false, int64_t>();
     });
   });
   return check_hip(hipGetLastError(), "sweep kernel launch");
