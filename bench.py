@@ -278,6 +278,10 @@ def run(args):
         L.init_seeds(rseed)
         if args.block:
             L.set_launch(args.block)
+        if args.no_bitplanes:
+            L.set_bitplanes(0)
+        elif args.bit_sites:
+            L.set_bitplanes(args.bit_sites)
         if args.fusion:
             L.set_fusion(True)
         L.set_graph(args.chunk)
@@ -299,6 +303,7 @@ def run(args):
     B = bytes_per_update(lattice, sb)
     B_survey = bytes_per_update(lattice, 4)
     tname = {1: "u8", 2: "u16", 4: "i32"}[sb]
+    planes = lattice == "sq/4n" and (app == "ising" or q == 2) and not fused and not args.no_bitplanes
     if fused:
         # a fused launch makes `fused` whole sweeps (K * fused colour passes)
         # of the lattice in LDS tiles: compulsory traffic is each site's
@@ -307,6 +312,14 @@ def run(args):
         kernel = "k_sweep_fused2d<%s, %s, S=%d>" % (lattice, tname, fused)
         bytes_per_launch = n * (2 * sb + 8)
         B_kernel = (2 * sb + 8) / fused
+    elif planes:
+        # two-state sq/4n on bit planes: a colour pass reads and writes the
+        # int32 seed of each of its sites and moves one bit per site of its
+        # own colour (read, write) and of the other (read)
+        kernel = "k_sweep_bits (two-state sq/4n, 32 sites per word)"
+        B = 8 + 3 / 8
+        bytes_per_launch = int(B * n // K)
+        B_kernel = B
     else:
         kernel = "k_sweep_color<%s, %s, 4>" % (lattice, tname)
         bytes_per_launch = B * n // K
@@ -331,7 +344,8 @@ def run(args):
                    "naccept": int(nacc), "final_energy": energy},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": load_traffic(args.workload + ("-fused" if fused else "")),
+                     "traffic": load_traffic(args.workload + ("-fused" if fused else "-bytes" if
+                                                              lattice == "sq/4n" and not planes and q <= 2 else "")),
                      "kernel": kernel,
                      "bytes_per_launch": bytes_per_launch,
                      "avg_launch_us": round(per_launch_s * 1e6, 3),
@@ -363,6 +377,10 @@ def main():
     ap.add_argument("--block-sweeps", type=int, default=25, help="cfg4: sweeps per graph block")
     ap.add_argument("--grid", type=int, default=0, help="cfg4: persistent workgroups (with --block)")
     ap.add_argument("--block", type=int, default=0, help="sweep-kernel workgroup size (0 = default)")
+    ap.add_argument("--no-bitplanes", action="store_true",
+                    help="two-state sq/4n lattices on the byte-per-site kernel instead of bit planes")
+    ap.add_argument("--bit-sites", type=int, default=0, choices=[0, 8, 16, 32],
+                    help="bit planes: sites per thread (0 = default)")
     ap.add_argument("--fusion", action="store_true", help="2d lattices: several sweeps per launch in LDS tiles")
     ap.add_argument("--chunk", type=int, default=32, help="sweeps per replayed graph (0 = eager launches)")
     ap.add_argument("--no-cpu-baseline", action="store_true")

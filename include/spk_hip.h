@@ -149,6 +149,16 @@ int spk_hip_sync(spk_hip_lattice *h);
    covers a tile of sub-grid rows (x, y and z), rows in y and z. */
 int spk_hip_set_launch(spk_hip_lattice *h, int block);
 
+/* Two-state sq/4n lattices (Ising, or Potts with Q = 2) sweep on bit planes
+   by default: for the duration of each spk_hip_sweeps call the spins are
+   held one bit per site, 32 sites of a sub-grid row per word, and the seeds
+   in a wavefront-interleaved order; the state is packed on entry and
+   unpacked on exit.  enable = 0 keeps such lattices on the byte-per-site
+   kernel; 1 turns the planes back on; 8, 16 or 32 also sets the sites each
+   thread takes.  Results are identical either way.  No counterpart in the
+   reference (a storage choice of this implementation). */
+int spk_hip_set_bitplanes(spk_hip_lattice *h, int enable);
+
 /* Sweep-fused launches for 2d lattices (sq/4n, sq/8n): each workgroup holds
    a tile plus a halo in LDS and runs several whole sweeps there (4 for
    sq/4n, 2 for sq/8n) before writing back.  Off by default; enable takes

@@ -1,0 +1,277 @@
+// Two-state sq/4n lattices (Ising, Potts Q=2) as bit planes: the sweep path
+// for the 2d Ising workloads.
+//
+// A colour pass of sweep color/strict (src/app_lattice.cpp:1023-1061,
+// sweep_nomask_strict) on a checkerboard updates every site of one colour
+// from its four neighbours, all of the other colour.  With two states a site
+// is one bit (spin - 1), so a 32-bit word holds 32 sites of one sub-grid row
+// and the four neighbour counts of all 32 come out of a bit-sliced adder over
+// four "differs" words; only the Park-Miller draws and the accept test stay
+// per site.  The per-site seeds (the reference's ranstrict[] generators,
+// src/app_lattice.cpp:346-351) keep their 4 bytes and dominate the traffic.
+//
+// The planes are a shadow of the u8 colour-blocked state that lives for one
+// spk_hip_sweeps call: packed on entry, unpacked on exit, so every other
+// kernel (energy, transfers, checksum, dumps) sees the usual layout.
+//
+// Layout.  Colour c, row b (0..hy-1), word w (0..pw-1, pw = ceil(hx/32)):
+//   bits[(c*hy + b)*pw + w], bit i = site a = 32*w + i of that row
+//   (bits of a >= hx are padding).
+// A thread owns SPT (8, 16 or 32) consecutive sites of one word: thread t of
+// a colour takes part q = t % (32/SPT) of word t / (32/SPT).  Its local site
+// i (0..SPT-1) has its seed at
+//   c*seedc + ((((t>>6)*(SPT/4) + (i>>2))*64 + (t&63))*4 + (i&3)
+// so a wavefront's j-th 16-byte seed load (local sites 4j..4j+3 of every
+// lane) covers 1 KiB of consecutive bytes.
+#pragma once
+
+#include "spk_device.h"
+
+namespace spk {
+
+struct BitGeom {
+  int pw;           // words per sub-grid row
+  int spt;          // sites per thread (8, 16, 32)
+  int tpw;          // threads per word (32 / spt)
+  int64_t threads;  // threads per colour with a real word (hy*pw*tpw)
+  int64_t nt;       // threads per colour rounded up to 64
+  int64_t seedc;    // shadow seed slots per colour (nt*spt)
+  int r;            // bit of site hx-1 in a row's last word
+  uint32_t lastmask;   // real-site bits of a row's last word
+  FastDiv fpw;      // division by pw
+};
+
+__host__ __device__ inline BitGeom bit_geom(const Geom &g, int spt) {
+  BitGeom bg;
+  bg.pw = (g.hx + 31) / 32;
+  bg.spt = spt;
+  bg.tpw = 32 / spt;
+  bg.threads = (int64_t)g.hy * bg.pw * bg.tpw;
+  bg.nt = (bg.threads + 63) / 64 * 64;
+  bg.seedc = bg.nt * spt;
+  bg.r = (g.hx - 1) & 31;
+  bg.lastmask = bg.r == 31 ? 0xffffffffu : (2u << bg.r) - 1u;
+  bg.fpw = fastdiv_make((uint32_t)bg.pw);
+  return bg;
+}
+
+__host__ __device__ __forceinline__ int64_t bit_seed_index(const BitGeom &bg, int c, uint32_t t, int i) {
+  return (int64_t)c * bg.seedc + ((((int64_t)(t >> 6) * (bg.spt / 4) + (i >> 2)) * 64 + (t & 63)) * 4 + (i & 3));
+}
+
+// --------------------------------------------------------------- conversions
+// Word (c, b, w) of the planes from the u8 state: bit i = spin - 1 of site
+// a = 32w+i (a < hx).  Spins are read a 4-byte group at a time (px is a
+// multiple of 4, so a group never leaves its row).
+__host__ __device__ __forceinline__ uint32_t bits_pack_word(const uint8_t *__restrict__ spin, const Geom &g, int c,
+                                                            int b, int w) {
+  const uint8_t *row = spin + (int64_t)c * g.Mp + (int64_t)b * g.px;
+  uint32_t out = 0;
+#pragma unroll
+  for (int j = 0; j < 8; j++) {
+    const int a = 32 * w + 4 * j;
+    if (a >= g.hx) break;
+    const uint32_t q = *reinterpret_cast<const uint32_t *>(row + a);
+#pragma unroll
+    for (int v = 0; v < 4; v++)
+      if (a + v < g.hx) out |= (((q >> (8 * v)) - 1u) & 1u) << (4 * j + v);
+  }
+  return out;
+}
+
+// The inverse: spins 1 + bit for the real sites of word (c, b, w); the pad
+// bytes of a row keep their values.
+__host__ __device__ __forceinline__ void bits_unpack_word(uint8_t *__restrict__ spin, const Geom &g, int c, int b,
+                                                          int w, uint32_t word) {
+  uint8_t *row = spin + (int64_t)c * g.Mp + (int64_t)b * g.px;
+#pragma unroll
+  for (int j = 0; j < 8; j++) {
+    const int a = 32 * w + 4 * j;
+    if (a >= g.hx) break;
+    uint32_t q = *reinterpret_cast<const uint32_t *>(row + a);
+#pragma unroll
+    for (int v = 0; v < 4; v++)
+      if (a + v < g.hx) {
+        const uint32_t s = 1u + ((word >> (4 * j + v)) & 1u);
+        q = (q & ~(0xffu << (8 * v))) | (s << (8 * v));
+      }
+    *reinterpret_cast<uint32_t *>(row + a) = q;
+  }
+}
+
+// Shadow slot s (0 .. 2*seedc-1) -> colour, thread, local site.
+__host__ __device__ __forceinline__ void bit_seed_slot(const BitGeom &bg, int64_t s, int &c, uint32_t &t, int &i) {
+  c = s >= bg.seedc;
+  const int64_t r = s - (c ? bg.seedc : 0);
+  const int64_t per = 64 * (int64_t)bg.spt;   // slots of 64 threads
+  const int64_t grp = r / per;
+  const int within = (int)(r - grp * per);
+  const int j = within >> 8;                  // 16-byte load index
+  const int l = (within >> 2) & 63;           // lane
+  i = 4 * j + (within & 3);
+  t = (uint32_t)(grp * 64 + l);
+}
+
+// Canonical seed offset of shadow slot (c, t, i), or -1 for a padding slot.
+__host__ __device__ __forceinline__ int64_t bit_seed_home(const Geom &g, const BitGeom &bg, int c, uint32_t t, int i) {
+  if ((int64_t)t >= bg.threads) return -1;
+  const uint32_t wd = t / (uint32_t)bg.tpw, q = t % (uint32_t)bg.tpw;
+  uint32_t b, w;
+  fastdiv(bg.fpw, wd, b, w);
+  const int a = 32 * (int)w + (int)q * bg.spt + i;
+  if (a >= g.hx) return -1;
+  return (int64_t)c * g.Mp + (int64_t)b * g.px + a;
+}
+
+// ------------------------------------------------------ bitwise primitives
+// Truth table of f(x, y, z) over bits in the order of v_bitop3_b32's
+// immediate: bit (x<<2 | y<<1 | z).
+template <class F>
+constexpr uint32_t truth3(F f) {
+  uint32_t m = 0;
+  for (int k = 0; k < 8; k++)
+    if (f((k >> 2) & 1, (k >> 1) & 1, k & 1)) m |= 1u << k;
+  return m;
+}
+
+// f applied bitwise to (x, y, z), f given by its truth table
+template <uint32_t TT>
+__host__ __device__ __forceinline__ uint32_t bop3(uint32_t x, uint32_t y, uint32_t z) {
+#ifdef __HIP_DEVICE_COMPILE__
+  return __builtin_amdgcn_bitop3_b32(x, y, z, TT);
+#else
+  uint32_t r = 0;
+  for (int k = 0; k < 8; k++)
+    if ((TT >> k) & 1u) r |= ((k & 4) ? x : ~x) & ((k & 2) ? y : ~y) & ((k & 1) ? z : ~z);
+  return r;
+#endif
+}
+
+// ~0 when bit i of x is set, else 0
+__host__ __device__ __forceinline__ uint32_t bit_mask(uint32_t x, int i) {
+#ifdef __HIP_DEVICE_COMPILE__
+  return (uint32_t)__builtin_amdgcn_sbfe((int)x, (unsigned)i, 1u);
+#else
+  return 0u - ((x >> i) & 1u);
+#endif
+}
+
+// ~0 when a < b (both below 2^31, so a - b cannot overflow), else 0
+__host__ __device__ __forceinline__ uint32_t less_mask(int32_t a, int32_t b) {
+  return (uint32_t)((int32_t)((uint32_t)a - (uint32_t)b) >> 31);
+}
+
+constexpr uint32_t kSel = truth3([](int x, int y, int z) { return x ? y : z; });            // x ? y : z
+constexpr uint32_t kAnd3 = truth3([](int x, int y, int z) { return x & y & z; });           // x & y & z
+constexpr uint32_t kKeep = truth3([](int x, int y, int z) { return x & (!y | z); });        // x & (~y | z)
+constexpr uint32_t kOrAnd = truth3([](int x, int y, int z) { return (x & y) | z; });        // (x & y) | z
+
+// -------------------------------------------------------------- the events
+// The SPT site events of thread t of colour c: the events of
+// AppIsing::site_event_rejection (src/app_ising.cpp:104-136) and, for Q=2,
+// AppPotts::site_event_rejection (src/app_potts.cpp:112-146), which draw
+// the same way: the first draw proposes 1 (below half) or 2; a change
+// (the proposal differs from the spin) with dE = ef - ei = 4 - 2*ei > 0
+// at T > 0 takes a second draw and is kept when it is below thr[dE]; a
+// change with dE <= 0 is kept without one; T = 0 rejects uphill changes
+// without one.  Returns (changes, their dE sum).
+template <int SPT>
+__host__ __device__ __forceinline__ Counts bits_thread(uint32_t *__restrict__ bits, int32_t *__restrict__ sseed,
+                                                       const Geom &g, const BitGeom &bg, const Rule &r, int c,
+                                                       uint32_t t) {
+  static_assert(SPT == 8 || SPT == 16 || SPT == 32, "8, 16 or 32 sites per thread");
+  constexpr int TPW = 32 / SPT;
+  const uint32_t wd = t / TPW;
+  const int q = (int)(t % TPW);
+  uint32_t b, wu;
+  fastdiv(bg.fpw, wd, b, wu);
+  const int w = (int)wu;
+  const int last = bg.pw - 1;
+  const int o = 1 - c;
+  const int p = ((int)b + c) & 1;   // x-neighbours at a and a+1 (p = 1) or a-1 (p = 0)
+  const int64_t pw = bg.pw;
+  const uint32_t *ob = bits + ((int64_t)o * g.hy) * pw;
+  const int64_t self = ((int64_t)c * g.hy + b) * pw + w;
+  const int wn = p ? (w == last ? 0 : w + 1) : (w == 0 ? last : w - 1);
+
+  // ---- loads: five plane words and the thread's seeds, before any is used
+  const uint32_t own = bits[self];
+  const uint32_t up = ob[(int64_t)wrap_dec((int)b, g.hy) * pw + w];
+  const uint32_t down = ob[(int64_t)wrap_inc((int)b, g.hy) * pw + w];
+  const uint32_t x = ob[(int64_t)b * pw + w];
+  const uint32_t xe = ob[(int64_t)b * pw + wn];
+  Words<4> sq[SPT / 4];
+#pragma unroll
+  for (int j = 0; j < SPT / 4; j++) sq[j] = *reinterpret_cast<const Words<4> *>(sseed + bit_seed_index(bg, c, t, 4 * j));
+  fence_schedule();
+
+  // ---- the shifted x-neighbour word: site i's neighbour at a+1 (p = 1) or
+  // a-1 (p = 0); across a row's ends the periodic image (a = 0 or hx-1)
+  uint32_t xs;
+  if (p) {
+    const uint32_t e = xe & 1u;   // site 0 of word w+1, or of word 0 after the last word
+    const uint32_t at = w == last ? 1u << bg.r : 0x80000000u;
+    xs = ((x >> 1) & ~at) | (e ? at : 0u);
+  } else {
+    const uint32_t e = (xe >> (w == 0 ? bg.r : 31)) & 1u;   // site 31 of word w-1, or site hx-1
+    xs = (x << 1) | e;
+  }
+  // ---- unlike-neighbour counts ei = s0 + 2*s1 + 4*s2 of all 32 sites
+  const uint32_t d1 = own ^ up, d2 = own ^ down, d3 = own ^ x, d4 = own ^ xs;
+  const uint32_t t1 = d1 ^ d2, c1 = d1 & d2, t2 = d3 ^ d4, c2 = d3 & d4;
+  const uint32_t s0 = t1 ^ t2, c3 = t1 & t2;   // c3 excludes c1 and c2
+  const uint32_t s1 = c1 ^ c2 ^ c3, s2 = c1 & c2;
+  const uint32_t upw = ~(s1 | s2);   // ei <= 1: a change is uphill (dE = 4 or 2)
+
+  // ---- per site: proposal draw, second draw, accept; masks are 0 or ~0.
+  // At T = 0 both thresholds are 0, so no second draw passes; the draw
+  // itself is not taken (nzt clears it).
+  const uint32_t nzt = r.zero_t ? 0u : 0xffffffffu;
+  const int32_t thr2 = r.zero_t ? 0 : r.thr[2], thr4 = r.zero_t ? 0 : r.thr[4];
+  const int32_t half_m1 = r.half - 1;
+  // this thread's part of the word, shifted down so site i is bit i
+  const int i0 = q * SPT;
+  const uint32_t own_q = own >> i0, upw_q = upw >> i0, s0_q = s0 >> i0;
+  uint32_t ch_q = 0;
+#pragma unroll
+  for (int j = 0; j < SPT / 4; j++) {
+    Words<4> nq;
+#pragma unroll
+    for (int v = 0; v < 4; v++) {
+      const int i = 4 * j + v;
+      const int32_t u1 = pm_step((int32_t)sq[j].w[v]);
+      const uint32_t prm = less_mask(half_m1, u1);   // proposal 2 (u1 >= half)
+      const uint32_t flipm = prm ^ bit_mask(own_q, i);
+      const uint32_t upm = bit_mask(upw_q, i);
+      const int32_t thr = (int32_t)bop3<kSel>(bit_mask(s0_q, i), (uint32_t)thr2, (uint32_t)thr4);   // ei 1: dE 2
+      const int32_t u2 = pm_step(u1);
+      const uint32_t passm = less_mask(u2, thr);
+      const uint32_t drawm = bop3<kAnd3>(flipm, upm, nzt);
+      nq.w[v] = bop3<kSel>(drawm, (uint32_t)u2, (uint32_t)u1);
+      ch_q = bop3<kOrAnd>(bop3<kKeep>(flipm, upm, passm), 1u << i, ch_q);
+    }
+    *reinterpret_cast<Words<4> *>(sseed + bit_seed_index(bg, c, t, 4 * j)) = nq;
+  }
+  uint32_t ch = SPT == 32 ? ch_q : (ch_q & ((1u << (SPT & 31)) - 1u)) << i0;
+  if (w == last) ch &= bg.lastmask;
+
+  // ---- this thread's part of the word, and the counts
+  if (ch) {
+    const uint32_t nw = own ^ ch;
+    if constexpr (SPT == 32) {
+      bits[self] = nw;
+    } else if constexpr (SPT == 16) {
+      reinterpret_cast<uint16_t *>(bits + self)[q] = (uint16_t)(nw >> (16 * q));
+    } else {
+      reinterpret_cast<uint8_t *>(bits + self)[q] = (uint8_t)(nw >> (8 * q));
+    }
+  }
+  Counts out;
+  out.nacc = __builtin_popcount(ch);
+  // sum over changes of dE = 4 - 2*ei
+  const int sum_ei = __builtin_popcount(ch & s0) + 2 * __builtin_popcount(ch & s1) + 4 * __builtin_popcount(ch & s2);
+  out.de = 4 * out.nacc - 2 * sum_ei;
+  return out;
+}
+
+}  // namespace spk

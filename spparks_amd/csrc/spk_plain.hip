@@ -13,6 +13,7 @@
 #include <utility>
 #include <vector>
 
+#include "spk_bits.h"
 #include "spk_device.h"
 #include "spk_host.h"
 #include "spk_kernels.h"
@@ -20,6 +21,60 @@
 using namespace spk;
 
 // ==================================================================== kernels
+
+// One colour pass of a two-state sq/4n lattice on bit planes: thread t owns
+// SPT sites of one plane word (see spk_bits.h).
+template <int SPT>
+__global__ __launch_bounds__(256) void k_sweep_bits(uint32_t *__restrict__ bits, int32_t *__restrict__ sseed, Geom g,
+                                                    BitGeom bg, Rule r, int color,
+                                                    unsigned long long *__restrict__ acc) {
+  const uint32_t t = (uint32_t)xcd_tile(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+  Counts cnt{0, 0};
+  if ((int64_t)t < bg.threads) cnt = bits_thread<SPT>(bits, sseed, g, bg, r, color, t);
+  block_accumulate_pack(cnt.nacc, cnt.de, acc, blockIdx.x & (kAccSlots - 1));
+}
+
+// u8 state -> planes, one thread per plane word of both colours
+__global__ void k_bits_pack(const uint8_t *__restrict__ spin, Geom g, BitGeom bg, uint32_t *__restrict__ bits) {
+  const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t words = (int64_t)g.hy * bg.pw;
+  if (q >= 2 * words) return;
+  const int c = q >= words;
+  const int64_t t = q - (c ? words : 0);
+  const int b = (int)(t / bg.pw), w = (int)(t % bg.pw);
+  bits[q] = bits_pack_word(spin, g, c, b, w);
+}
+
+__global__ void k_bits_unpack(const uint32_t *__restrict__ bits, Geom g, BitGeom bg, uint8_t *__restrict__ spin) {
+  const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t words = (int64_t)g.hy * bg.pw;
+  if (q >= 2 * words) return;
+  const int c = q >= words;
+  const int64_t t = q - (c ? words : 0);
+  const int b = (int)(t / bg.pw), w = (int)(t % bg.pw);
+  bits_unpack_word(spin, g, c, b, w, bits[q]);
+}
+
+// seeds <-> shadow seeds, one thread per shadow slot; padding slots hold 1
+__global__ void k_seeds_to_shadow(const int32_t *__restrict__ seed, Geom g, BitGeom bg, int32_t *__restrict__ sseed) {
+  const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= 2 * bg.seedc) return;
+  int c, i;
+  uint32_t t;
+  bit_seed_slot(bg, s, c, t, i);
+  const int64_t home = bit_seed_home(g, bg, c, t, i);
+  sseed[s] = home < 0 ? 1 : seed[home];
+}
+
+__global__ void k_seeds_from_shadow(const int32_t *__restrict__ sseed, Geom g, BitGeom bg, int32_t *__restrict__ seed) {
+  const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= 2 * bg.seedc) return;
+  int c, i;
+  uint32_t t;
+  bit_seed_slot(bg, s, c, t, i);
+  const int64_t home = bit_seed_home(g, bg, c, t, i);
+  if (home >= 0) seed[home] = sseed[s];
+}
 
 // One colour pass: every site of colour `color` performs one strict event.
 // A pack is 4 consecutive sites of one sub-grid row (x fastest), so its own
@@ -148,6 +203,8 @@ __global__ __launch_bounds__(512) void k_sweep_fused2d(const T *__restrict__ spi
 
 // ===================================================================== host
 
+constexpr int kBitSites = 32;   // default sites per thread on bit planes
+
 struct spk_hip_lattice {
   spk_hip_desc d;
   Geom g;
@@ -162,6 +219,13 @@ struct spk_hip_lattice {
   void *spin_alt = nullptr;            // second state buffer for sweep-fused launches
   int32_t *seed_alt = nullptr;
   int fuse = 0;                        // sweeps per fused launch (0: per-colour kernel only)
+  // bit planes of a two-state sq/4n lattice (spk_bits.h), live during a sweeps call
+  int bitplanes = kBitSites;           // sites per thread on the planes (0: planes off)
+  bool bits_active = false;
+  BitGeom bg{};
+  uint32_t *bits = nullptr;
+  int32_t *sseed = nullptr;
+  int64_t sseed_cap = 0;               // shadow seed slots allocated
   int tile_x = 0, tile_y = 0;          // fused core tile
   int32_t *stage = nullptr;            // N int32, natural order (transfers)
   unsigned long long *acc = nullptr;   // [cap][kAccRow]: per-sweep slots
@@ -213,6 +277,18 @@ int launch_variant(spk_hip_lattice *h, int color, unsigned long long *acc) {
 
 int launch_sweep(spk_hip_lattice *h, int color, unsigned long long *acc) {
   if (variant_app(h->d.app)) return launch_variant(h, color, acc);
+  if (h->bits_active) {
+    unsigned grid = grid_for(h->bg.threads, 256);
+    grid = (grid + kNumXcd - 1) / kNumXcd * kNumXcd;   // xcd_tile needs a multiple of 8
+    auto go = [&](auto kern) {
+      hipLaunchKernelGGL(kern, dim3(grid), dim3(256), 0, h->stream, h->bits, h->sseed, h->g, h->bg, h->rule, color,
+                         acc);
+    };
+    if (h->bg.spt == 8) go(k_sweep_bits<8>);
+    else if (h->bg.spt == 16) go(k_sweep_bits<16>);
+    else go(k_sweep_bits<32>);
+    return check_hip(hipGetLastError(), "bit-plane sweep launch");
+  }
   const int bs = h->block;
   const TileMap m = make_tile_map(h->g, bs);
   unsigned grid = (unsigned)m.nxc * m.nyg * m.nzg;
@@ -305,6 +381,54 @@ int launch_fused(spk_hip_lattice *h, unsigned long long *acc) {
 void drop_graphs(spk_hip_lattice *h) {
   for (auto &c : h->graphs) (void)hipGraphExecDestroy(c.exec);
   h->graphs.clear();
+}
+
+// Two-state sq/4n lattices on the per-colour path sweep on bit planes.
+bool bits_eligible(const spk_hip_lattice *h) {
+  return h->bitplanes && h->g.lat == SPK_HIP_SQ_4N && h->rule.nspins == 2 && h->stype == kU8 &&
+         !variant_app(h->d.app) && !h->fuse;
+}
+
+// Packs the state into the planes (allocated on first use) and routes the
+// colour passes there until bits_leave.
+int bits_enter(spk_hip_lattice *h) {
+  h->bg = bit_geom(h->g, h->bitplanes);
+  if (!h->bits) {
+    const size_t words = 2 * (size_t)h->g.hy * h->bg.pw;
+    if (hipMalloc(&h->bits, words * sizeof(uint32_t)) != hipSuccess) {
+      h->bits = nullptr;
+      return set_error(SPK_HIP_ENOMEM, "bit-plane shadow state not available");
+    }
+  }
+  if (2 * h->bg.seedc > h->sseed_cap) {
+    (void)hipStreamSynchronize(h->stream);
+    if (h->sseed) (void)hipFree(h->sseed);
+    h->sseed_cap = 0;
+    if (hipMalloc(&h->sseed, 2 * (size_t)h->bg.seedc * sizeof(int32_t)) != hipSuccess) {
+      h->sseed = nullptr;
+      return set_error(SPK_HIP_ENOMEM, "bit-plane shadow seeds not available");
+    }
+    h->sseed_cap = 2 * h->bg.seedc;
+  }
+  const int64_t words = 2 * (int64_t)h->g.hy * h->bg.pw;
+  hipLaunchKernelGGL(k_bits_pack, dim3(grid_for(words, 256)), dim3(256), 0, h->stream,
+                     static_cast<const uint8_t *>(h->spin), h->g, h->bg, h->bits);
+  hipLaunchKernelGGL(k_seeds_to_shadow, dim3(grid_for(2 * h->bg.seedc, 256)), dim3(256), 0, h->stream, h->seed, h->g,
+                     h->bg, h->sseed);
+  int rc = check_hip(hipGetLastError(), "bit-plane pack");
+  if (!rc) h->bits_active = true;
+  return rc;
+}
+
+int bits_leave(spk_hip_lattice *h) {
+  if (!h->bits_active) return SPK_HIP_OK;
+  h->bits_active = false;
+  const int64_t words = 2 * (int64_t)h->g.hy * h->bg.pw;
+  hipLaunchKernelGGL(k_bits_unpack, dim3(grid_for(words, 256)), dim3(256), 0, h->stream, h->bits, h->g, h->bg,
+                     static_cast<uint8_t *>(h->spin));
+  hipLaunchKernelGGL(k_seeds_from_shadow, dim3(grid_for(2 * h->bg.seedc, 256)), dim3(256), 0, h->stream, h->sseed,
+                     h->g, h->bg, h->seed);
+  return check_hip(hipGetLastError(), "bit-plane unpack");
 }
 
 __global__ void k_zero_u64(unsigned long long *__restrict__ v, int64_t n) {
@@ -532,6 +656,8 @@ void spk_hip_destroy(spk_hip_lattice *h) {
   if (h->stage) (void)hipFree(h->stage);
   if (h->spin_alt) (void)hipFree(h->spin_alt);
   if (h->seed_alt) (void)hipFree(h->seed_alt);
+  if (h->bits) (void)hipFree(h->bits);
+  if (h->sseed) (void)hipFree(h->sseed);
   if (h->acc) (void)hipFree(h->acc);
   if (h->accc) (void)hipFree(h->accc);
   if (h->ev0) (void)hipEventDestroy(h->ev0);
@@ -670,25 +796,35 @@ int spk_hip_sweeps(spk_hip_lattice *h, int nsweeps, int64_t *naccept_total, int6
   const size_t row = 2 * sizeof(unsigned long long);   // a folded row
   const size_t first_rem = nchunks ? C : 0;             // eager rows follow the chunk's rows
   unsigned long long *rem_acc = h->acc + first_rem * kAccRow;
-  if ((rc = check_hip(hipEventRecord(h->ev0, h->stream), "event"))) return rc;
+  // two-state sq/4n: the state moves to bit planes for the call (packing
+  // and unpacking lie outside the timed events)
+  if (bits_eligible(h) && (rc = bits_enter(h))) return rc;
   int64_t nlaunch = 0;
-  for (int k = 0; k < nchunks; k++) {
-    hipGraphExec_t exec;
-    if ((rc = chunk_graph(h, &exec))) return rc;
-    if ((rc = check_hip(hipGraphLaunch(exec, h->stream), "graph launch"))) return rc;
-    if ((rc = check_hip(hipMemcpyAsync(h->pinned + (size_t)k * C * 2, h->accc, row * C, hipMemcpyDeviceToHost,
-                                       h->stream),
-                        "accumulator readback")))
-      return rc;
-  }
-  if (nchunks) nlaunch += (int64_t)nchunks * (h->fuse ? C / h->fuse : (int64_t)C * h->g.K);
-  if (rem) {
-    if ((rc = check_hip(hipMemsetAsync(rem_acc, 0, (size_t)rem * kAccRow * sizeof(unsigned long long), h->stream),
-                        "memset")))
-      return rc;
-    if ((rc = launch_sweeps(h, rem, rem_acc, &nlaunch))) return rc;
-  }
-  if ((rc = check_hip(hipEventRecord(h->ev1, h->stream), "event"))) return rc;
+  auto launch_all = [&]() -> int {
+    int rc;
+    if ((rc = check_hip(hipEventRecord(h->ev0, h->stream), "event"))) return rc;
+    for (int k = 0; k < nchunks; k++) {
+      hipGraphExec_t exec;
+      if ((rc = chunk_graph(h, &exec))) return rc;
+      if ((rc = check_hip(hipGraphLaunch(exec, h->stream), "graph launch"))) return rc;
+      if ((rc = check_hip(hipMemcpyAsync(h->pinned + (size_t)k * C * 2, h->accc, row * C, hipMemcpyDeviceToHost,
+                                         h->stream),
+                          "accumulator readback")))
+        return rc;
+    }
+    if (nchunks) nlaunch += (int64_t)nchunks * (h->fuse ? C / h->fuse : (int64_t)C * h->g.K);
+    if (rem) {
+      if ((rc = check_hip(hipMemsetAsync(rem_acc, 0, (size_t)rem * kAccRow * sizeof(unsigned long long), h->stream),
+                          "memset")))
+        return rc;
+      if ((rc = launch_sweeps(h, rem, rem_acc, &nlaunch))) return rc;
+    }
+    return check_hip(hipEventRecord(h->ev1, h->stream), "event");
+  };
+  rc = launch_all();
+  const int rc_leave = bits_leave(h);   // the state returns to the u8 layout on every path
+  if (rc) return rc;
+  if ((rc = rc_leave)) return rc;
   if (rem && ((rc = launch_fold(h, rem_acc, rem, h->accc + first_rem * 2)) ||
               (rc = check_hip(hipMemcpyAsync(h->pinned + (size_t)nchunks * C * 2, h->accc + first_rem * 2, row * rem,
                                              hipMemcpyDeviceToHost, h->stream),
@@ -757,6 +893,15 @@ int spk_hip_set_launch(spk_hip_lattice *h, int block) {
     return set_error(SPK_HIP_EINVAL, "block must be a power of two in [64, 1024]");
   h->block = block;
   drop_graphs(h);
+  return SPK_HIP_OK;
+}
+
+int spk_hip_set_bitplanes(spk_hip_lattice *h, int enable) {
+  if (!h) return set_error(SPK_HIP_EINVAL, "null handle");
+  if (enable != 0 && enable != 1 && enable != 8 && enable != 16 && enable != 32)
+    return set_error(SPK_HIP_EINVAL, "bit planes: 0 (off), 1 (on), or 8, 16, 32 sites per thread");
+  h->bitplanes = enable == 1 ? kBitSites : enable;
+  drop_graphs(h);   // captured launches name one kernel or the other
   return SPK_HIP_OK;
 }
 

@@ -5,6 +5,8 @@
 //
 // usage: spk_emul app lat nx ny nz nspins kT nsweeps stype vec in.bin out.bin
 //   stype: 0 uint8, 1 uint16, 2 int32 spin storage; vec: sites per thread (4)
+//   SPK_EMUL_BITS=8|16|32 in the environment runs two-state sq/4n lattices
+//   on the bit-plane path (spk_bits.h), that many sites per thread.
 //   SPK_EMUL_FUSE=TXxTY in the environment (2d lattices) runs the
 //   sweep-fused tile algorithm instead, S sweeps per launch, the rest of
 //   the sweeps per colour
@@ -17,6 +19,7 @@
 #include <utility>
 #include <vector>
 
+#include "spk_bits.h"
 #include "spk_device.h"
 
 using namespace spk;
@@ -159,6 +162,73 @@ static int run(int app, int nx, int ny, int nz, int q, double kt, int nsw, const
         eng[w + q] = e;
       }
       if (e != energy<LAT, T>(s, g)) { fprintf(stderr, "incremental energy drifted (fused)\n"); return 4; }
+    }
+  }
+  const char *bz = getenv("SPK_EMUL_BITS");
+  if (bz && LAT == SPK_HIP_SQ_4N && r.nspins == 2 && sizeof(T) == 1 && app < SPK_HIP_POTTS_NEIGH) {
+    // bit-plane path (spk_bits.h): pack, every pass thread by thread, unpack
+    const int spt = atoi(bz);
+    if (spt != 8 && spt != 16 && spt != 32) { fprintf(stderr, "SPK_EMUL_BITS must be 8, 16 or 32\n"); return 16; }
+    const BitGeom bg = bit_geom(g, spt);
+    const int64_t words = (int64_t)g.hy * bg.pw;
+    // every real site has exactly one shadow seed slot
+    std::vector<int> home_hits(S_slots(g), 0);
+    for (int64_t sl = 0; sl < 2 * bg.seedc; sl++) {
+      int c, i;
+      uint32_t t;
+      bit_seed_slot(bg, sl, c, t, i);
+      if (bit_seed_index(bg, c, t, i) != sl) { fprintf(stderr, "bit seed slot does not invert\n"); return 13; }
+      const int64_t h = bit_seed_home(g, bg, c, t, i);
+      if (h >= (int64_t)S_slots(g)) { fprintf(stderr, "bit seed home out of range\n"); return 14; }
+      if (h >= 0) home_hits[h]++;
+    }
+    for (int c = 0; c < 2; c++)
+      for (int b = 0; b < g.hy; b++)
+        for (int a = 0; a < g.px; a++)
+          if (home_hits[(int64_t)c * g.Mp + (int64_t)b * g.px + a] != (a < g.hx ? 1 : 0)) {
+            fprintf(stderr, "bit seed slots do not cover each site once\n");
+            return 15;
+          }
+    std::vector<uint32_t> bits(2 * words);
+    std::vector<int32_t> sseed(2 * bg.seedc);
+    for (int64_t q = 0; q < 2 * words; q++) {
+      const int c = q >= words;
+      const int64_t t = q - (c ? words : 0);
+      bits[q] = bits_pack_word(reinterpret_cast<const uint8_t *>(s.data()), g, c, (int)(t / bg.pw), (int)(t % bg.pw));
+    }
+    for (int64_t sl = 0; sl < 2 * bg.seedc; sl++) {
+      int c, i;
+      uint32_t t;
+      bit_seed_slot(bg, sl, c, t, i);
+      const int64_t h = bit_seed_home(g, bg, c, t, i);
+      sseed[sl] = h < 0 ? 1 : k[h];
+    }
+    for (; w < nsw; w++) {
+      long long de = 0;
+      for (int c = 0; c < 2; c++)
+        for (uint32_t t = 0; t < (uint32_t)bg.threads; t++) {
+          const Counts cnt = spt == 8    ? bits_thread<8>(bits.data(), sseed.data(), g, bg, r, c, t)
+                             : spt == 16 ? bits_thread<16>(bits.data(), sseed.data(), g, bg, r, c, t)
+                                         : bits_thread<32>(bits.data(), sseed.data(), g, bg, r, c, t);
+          na += cnt.nacc;
+          de += cnt.de;
+        }
+      for (int64_t q = 0; q < 2 * words; q++) {
+        const int c = q >= words;
+        const int64_t t = q - (c ? words : 0);
+        bits_unpack_word(reinterpret_cast<uint8_t *>(s.data()), g, c, (int)(t / bg.pw), (int)(t % bg.pw), bits[q]);
+      }
+      for (int64_t sl = 0; sl < 2 * bg.seedc; sl++) {
+        int c, i;
+        uint32_t t;
+        bit_seed_slot(bg, sl, c, t, i);
+        const int64_t h = bit_seed_home(g, bg, c, t, i);
+        if (h >= 0) k[h] = sseed[sl];
+      }
+      e += 2 * de;
+      if (e != energy<LAT, T>(s, g)) { fprintf(stderr, "incremental energy drifted (bit planes)\n"); return 4; }
+      nacc[w] = na;
+      eng[w] = e;
     }
   }
   if (app >= SPK_HIP_POTTS_NEIGH) {

@@ -110,6 +110,32 @@ FUSED_CASES = [
 ]
 
 
+BIT_CASES = [
+    # two-state sq/4n on bit planes: sub-grid rows of 1, 9, 32 (one full
+    # word), 33 (a 1-bit last word), 63 and 65 sites; Potts Q=2; T = 0
+    ("ising", "sq/4n", (2, 2, 1), 2, 2.0, 3, 6),
+    ("ising", "sq/4n", (18, 10, 1), 2, 2.269, 5, 7),
+    ("ising", "sq/4n", (64, 48, 1), 2, 2.269, 777, 9),
+    ("ising", "sq/4n", (66, 10, 1), 2, 1.5, 9, 6),
+    ("potts", "sq/4n", (126, 6, 1), 2, 0.9, 10, 5),
+    ("ising", "sq/4n", (130, 4, 1), 2, 3.5, 11, 5),
+    ("ising", "sq/4n", (40, 34, 1), 2, 0.0, 12, 5),
+    ("potts", "sq/4n", (64, 2, 1), 2, 0.0, 13, 4),
+]
+
+
+@pytest.mark.parametrize("spt", [8, 16, 32])
+@pytest.mark.parametrize("case", BIT_CASES, ids=lambda c: "%s-%s-q%d-T%g" % (c[0], "x".join(map(str, c[2])), c[3], c[4]))
+def test_emulated_bit_planes_match_oracle(orc, tmp_path, case, spt):
+    """Two-state sq/4n lattices on bit planes (32 sites per word, shadow
+    seeds interleaved by wavefront): row ends inside a word, single-word
+    rows, one-bit last words, the seed shadow's one-to-one map, and the
+    incremental energy after every sweep."""
+    if not os.path.exists(EMUL):
+        pytest.skip("tests/emul/spk_emul not built")
+    run_emul(orc, tmp_path, case, {"SPK_EMUL_BITS": str(spt)}, variants=[(0, 4)])
+
+
 @pytest.mark.parametrize("tiles", ["8x4", "16x8", "128x64"])
 @pytest.mark.parametrize("case", FUSED_CASES, ids=lambda c: "%s-%s-%s-q%d" % (c[0], c[1], "x".join(map(str, c[2])), c[3]))
 def test_emulated_fused_tiles_match_oracle(orc, tmp_path, case, tiles):

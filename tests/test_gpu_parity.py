@@ -289,3 +289,63 @@ def test_full_size_first_sweeps_bit_exact(orc, case):
         assert L.energy() == eng[-1]
     assert np.array_equal(gn, nacc) and np.array_equal(ge, eng)
     assert np.array_equal(gs1, spin1) and np.array_equal(gk1, seeds1)
+
+
+BIT_CASES = [
+    # two-state sq/4n lattices, which sweep on bit planes by default:
+    # sub-grid rows of 1, 9, 32, 33, 63 and 65 sites, Potts Q=2, T = 0
+    ("ising", "sq/4n", (2, 2, 1), 2, 2.0, 3, 6),
+    ("ising", "sq/4n", (18, 10, 1), 2, 2.269, 5, 7),
+    ("ising", "sq/4n", (64, 48, 1), 2, 2.269, 777, 9),
+    ("ising", "sq/4n", (66, 10, 1), 2, 1.5, 9, 6),
+    ("potts", "sq/4n", (126, 6, 1), 2, 0.9, 10, 5),
+    ("ising", "sq/4n", (130, 4, 1), 2, 3.5, 11, 5),
+    ("ising", "sq/4n", (40, 34, 1), 2, 0.0, 12, 5),
+    ("potts", "sq/4n", (64, 2, 1), 2, 0.0, 13, 4),
+    ("ising", "sq/4n", (256, 130, 1), 2, 2.269, 14, 6),
+]
+
+
+@pytest.mark.parametrize("planes", [8, 16, 32, 0], ids=["bits8", "bits16", "bits32", "bytes"])
+@pytest.mark.parametrize("case", BIT_CASES, ids=lambda c: "%s-%s-q%d-T%g" % (c[0], "x".join(map(str, c[2])), c[3], c[4]))
+def test_two_state_bit_planes_bit_exact(orc, case, planes):
+    """Two-state sq/4n lattices on bit planes (and with them switched off)
+    against the oracle: spins, seeds, per-sweep Naccept and energy."""
+    import spparks_amd as S
+    app, lat, shape, q, kt, seed, n = case
+    say("case", case, "bit planes", planes)
+    _, _, spin1, seeds1, nacc, eng = oracle_run(orc, app, lat, shape, q, kt, seed, n)
+    with gpu_init(S, app, lat, shape, q, kt, seed) as L:
+        L.set_bitplanes(planes)
+        tot, gn, ge = L.sweeps(n)
+        gs1, gk1 = L.download()
+    assert np.array_equal(gs1, spin1)
+    assert np.array_equal(gk1, seeds1)
+    assert np.array_equal(gn, nacc)
+    assert np.array_equal(ge, eng)
+
+
+def test_bit_planes_equal_bytes_on_a_large_lattice():
+    """1026 x 512 Ising (sub-grid rows of 513 sites: a one-bit last word),
+    45 sweeps in calls of 13, 32 and 0: bit planes and the byte kernel give
+    the same spins, seeds, Naccept and energies (graph chunks and eager
+    remainders, packing and unpacking on every call)."""
+    import spparks_amd as S
+    out = []
+    for planes in (1, 8, 32, 0):
+        say("bit planes", planes)
+        with gpu_init(S, "ising", "sq/4n", (1026, 512, 1), 2, 2.269, 4321) as L:
+            L.set_bitplanes(planes)
+            rows = []
+            for n in (13, 32, 0):
+                rows.append(L.sweeps(n) if n else None)
+            spin, seeds = L.download()
+            out.append((spin, seeds, rows, L.energy()))
+    s2, k2, r2, e2 = out[-1]   # the byte kernel
+    for s1, k1, r1, e1 in out[:-1]:
+        assert np.array_equal(s1, s2) and np.array_equal(k1, k2)
+        assert e1 == e2
+        for a, b in zip(r1, r2):
+            if a is None:
+                continue
+            assert a[0] == b[0] and np.array_equal(a[1], b[1]) and np.array_equal(a[2], b[2])
