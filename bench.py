@@ -29,6 +29,8 @@ WORKLOADS = {
              "Potts Q=100 sq/8n 128x128 periodic, kT=0.5, seed 56789"),
     "cfg2": ("ising", "sq/4n", (4096, 4096, 1), 2, 2.269, 12345, 1000,
              "Ising sq/4n 4096x4096 periodic, kT=2.269, seed 12345"),
+    "ising8k": ("ising", "sq/4n", (8192, 8192, 1), 2, 2.269, 12345, 250,
+                "Ising sq/4n 8192x8192 periodic, kT=2.269, seed 12345 (a larger lattice of the cfg2 kind)"),
     "cfg3": ("potts", "sc/26n", (256, 256, 256), 10000, 0.5, 24680, 200,
              "Potts Q=10000 sc/26n 256^3 periodic, kT=0.5, mask yes, seed 24680"),
 }

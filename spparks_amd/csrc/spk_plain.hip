@@ -25,7 +25,7 @@ using namespace spk;
 // One colour pass of a two-state sq/4n lattice on bit planes: thread t owns
 // SPT sites of one plane word (see spk_bits.h).
 template <int SPT>
-__global__ __launch_bounds__(256) void k_sweep_bits(uint32_t *__restrict__ bits, int32_t *__restrict__ sseed, Geom g,
+__global__ __launch_bounds__(1024) void k_sweep_bits(uint32_t *__restrict__ bits, int32_t *__restrict__ sseed, Geom g,
                                                     BitGeom bg, Rule r, int color,
                                                     unsigned long long *__restrict__ acc) {
   const uint32_t t = (uint32_t)xcd_tile(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
@@ -278,10 +278,11 @@ int launch_variant(spk_hip_lattice *h, int color, unsigned long long *acc) {
 int launch_sweep(spk_hip_lattice *h, int color, unsigned long long *acc) {
   if (variant_app(h->d.app)) return launch_variant(h, color, acc);
   if (h->bits_active) {
-    unsigned grid = grid_for(h->bg.threads, 256);
+    const int bs = h->block;
+    unsigned grid = grid_for(h->bg.threads, bs);
     grid = (grid + kNumXcd - 1) / kNumXcd * kNumXcd;   // xcd_tile needs a multiple of 8
     auto go = [&](auto kern) {
-      hipLaunchKernelGGL(kern, dim3(grid), dim3(256), 0, h->stream, h->bits, h->sseed, h->g, h->bg, h->rule, color,
+      hipLaunchKernelGGL(kern, dim3(grid), dim3(bs), 0, h->stream, h->bits, h->sseed, h->g, h->bg, h->rule, color,
                          acc);
     };
     if (h->bg.spt == 8) go(k_sweep_bits<8>);
