@@ -668,9 +668,62 @@ int spk_hip_batch_run(spk_hip_batch *b, int max_sweeps, int use_graph) {
   }
   if ((rc = ensure_events(b, 2 * (size_t)L + 2)) || (rc = ensure_streams(b))) return rc;
 
-  if (tune) {
-    // step j runs with candidate j % C; every candidate's block size has its
-    // own tile lists, uploaded before the step that uses it
+  // sites updated in step j, for a candidate's rate
+  auto step_work = [&](int j) {
+    double w = 0.0;
+    for (size_t k = 0; k < b->variants.size(); k++) {
+      const StepDesc &sd = b->h_steps[k * L + j];
+      for (int a = 0; a < sd.count; a++) w += (double)b->h_reps[b->h_act[sd.first + a]].g.M;
+    }
+    return w;
+  };
+  const int C = (int)cand.size();
+  const int per = tune && L / C >= 8 ? L / C / 8 * 8 : 0;   // whole sweeps of an 8-colour replica
+  if (tune && per) {
+    // each candidate runs `per` consecutive steps as a replay does (no wait
+    // between steps) under one event pair; the schedule repeats every sweep,
+    // so every chunk holds the same mix of replicas.  The steps after the
+    // last chunk run with the winner.
+    std::vector<double> rate(C, 0.0);
+    double tot = 0.0;
+    for (int c = 0; c < C; c++) {
+      const int j0 = c * per;
+      if ((rc = build_schedule(b, max_sweeps, cand[c].block, &L, &rows, &updates))) return rc;
+      if ((rc = upload_schedule(b, rows, c == 0))) return rc;   // one clear per block
+      if ((rc = check_hip(hipEventRecord(b->events[0], b->stream), "event")) ||
+          (rc = launch_steps(b, j0, j0 + per, L, cand[c].block, cand[c].grid)) ||
+          (rc = check_hip(hipEventRecord(b->events[1], b->stream), "event")) ||
+          (rc = check_hip(hipStreamSynchronize(b->stream), "tune sync")))
+        return rc;
+      float ms = 0.f;
+      if ((rc = check_hip(hipEventElapsedTime(&ms, b->events[0], b->events[1]), "event time"))) return rc;
+      double w = 0.0;
+      for (int j = j0; j < j0 + per; j++) w += step_work(j);
+      rate[c] = w / std::max((double)ms, 1e-9);
+      tot += ms;
+    }
+    size_t best = 0;
+    for (int c = 1; c < C; c++)
+      if (rate[c] > rate[best]) best = c;
+    b->block = cand[best].block;
+    b->grid = cand[best].grid;
+    if (C * per < L) {
+      if ((rc = build_schedule(b, max_sweeps, b->block, &L, &rows, &updates)) ||
+          (rc = upload_schedule(b, rows, false)) ||
+          (rc = check_hip(hipEventRecord(b->events[0], b->stream), "event")) ||
+          (rc = launch_steps(b, C * per, L, L, b->block, b->grid)) ||
+          (rc = check_hip(hipEventRecord(b->events[1], b->stream), "event")) ||
+          (rc = check_hip(hipStreamSynchronize(b->stream), "run sync")))
+        return rc;
+      float ms = 0.f;
+      if ((rc = check_hip(hipEventElapsedTime(&ms, b->events[0], b->events[1]), "event time"))) return rc;
+      tot += ms;
+    }
+    b->last_ms = tot;
+  } else if (tune) {
+    // too few steps for whole-sweep chunks: step j runs with candidate j % C;
+    // every candidate's block size has its own tile lists, uploaded before
+    // the step that uses it
     std::vector<double> t_ms(cand.size(), 0.0), work(cand.size(), 0.0);
     int j = 0;
     double tot = 0.0;
@@ -686,13 +739,7 @@ int spk_hip_batch_run(spk_hip_batch *b, int max_sweeps, int use_graph) {
         float ms = 0.f;
         if ((rc = check_hip(hipEventElapsedTime(&ms, b->events[2 * j], b->events[2 * j + 1]), "event time")))
           return rc;
-        // sites updated in this step, for the candidate's rate
-        double w = 0.0;
-        for (size_t k = 0; k < b->variants.size(); k++) {
-          const StepDesc &sd = b->h_steps[k * L + j];
-          for (int a = 0; a < sd.count; a++) w += (double)b->h_reps[b->h_act[sd.first + a]].g.M;
-        }
-        work[c] += w;
+        work[c] += step_work(j);
         t_ms[c] += ms;
         tot += ms;
       }
