@@ -280,6 +280,8 @@ def run(args):
         L.init_seeds(rseed)
         if args.block:
             L.set_launch(args.block)
+        if args.no_resident:
+            L.set_resident(False)
         if args.no_bitplanes:
             L.set_bitplanes(0)
         elif args.bit_sites:
@@ -305,8 +307,17 @@ def run(args):
     B = bytes_per_update(lattice, sb)
     B_survey = bytes_per_update(lattice, 4)
     tname = {1: "u8", 2: "u16", 4: "i32"}[sb]
-    planes = lattice == "sq/4n" and (app == "ising" or q == 2) and not fused and not args.no_bitplanes
-    if fused:
+    resident = nlaunch < steps   # the whole state in one workgroup's LDS: one launch per call
+    planes = (lattice == "sq/4n" and (app == "ising" or q == 2) and not fused and not args.no_bitplanes
+              and not resident)
+    if resident:
+        # all sweeps of the call in one single-workgroup launch: the state
+        # crosses HBM once each way; the kernel is bound by the latency of
+        # its colour passes in LDS, not by HBM
+        kernel = "k_sweep_resident<%s, %s> (one workgroup, state in LDS)" % (lattice, tname)
+        bytes_per_launch = 2 * n * (sb + 4)
+        B_kernel = bytes_per_launch / (n * steps / nlaunch)
+    elif fused:
         # a fused launch makes `fused` whole sweeps (K * fused colour passes)
         # of the lattice in LDS tiles: compulsory traffic is each site's
         # spin and seed read once and written once per launch (halo re-reads
@@ -381,6 +392,8 @@ def main():
     ap.add_argument("--block", type=int, default=0, help="sweep-kernel workgroup size (0 = default)")
     ap.add_argument("--no-bitplanes", action="store_true",
                     help="two-state sq/4n lattices on the byte-per-site kernel instead of bit planes")
+    ap.add_argument("--no-resident", action="store_true",
+                    help="small lattices through the per-colour kernels instead of one LDS-resident workgroup")
     ap.add_argument("--bit-sites", type=int, default=0, choices=[0, 8, 16, 32],
                     help="bit planes: sites per thread (0 = default)")
     ap.add_argument("--fusion", action="store_true", help="2d lattices: several sweeps per launch in LDS tiles")

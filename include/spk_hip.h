@@ -159,6 +159,15 @@ int spk_hip_set_launch(spk_hip_lattice *h, int block);
    reference (a storage choice of this implementation). */
 int spk_hip_set_bitplanes(spk_hip_lattice *h, int enable);
 
+/* Lattices whose spins and seeds fit one workgroup's LDS (up to about 30 k
+   sites at one byte per spin on MI355X: the reference's CPU-sized cases)
+   run every sweep of a spk_hip_sweeps call in one launch of a single
+   workgroup, the state read and written once.  On by default; enable = 0
+   sends them through the per-colour kernels.  Results are identical either
+   way.  No counterpart in the reference (a launch choice of this
+   implementation). */
+int spk_hip_set_resident(spk_hip_lattice *h, int enable);
+
 /* Sweep-fused launches for 2d lattices (sq/4n, sq/8n): each workgroup holds
    a tile plus a halo in LDS and runs several whole sweeps there (4 for
    sq/4n, 2 for sq/8n) before writing back.  Off by default; enable takes

@@ -104,6 +104,46 @@ __global__ __launch_bounds__(1024) void k_sweep_color(T *__restrict__ spin, int3
   block_accumulate_pack(cnt.nacc, cnt.de, acc, blockIdx.x & (kAccSlots - 1));
 }
 
+// Every sweep of a lattice small enough to live in one workgroup's LDS
+// (the reference's CPU-sized cases): spins and seeds are read once, all
+// colour passes run on the LDS copy with a barrier between passes, and the
+// state is written back once.  The same per-pack events as k_sweep_color,
+// on LDS addresses; sweep s's (Naccept, dE) go to slot 0 of acc row s.
+template <int LAT, typename T, bool TWO>
+__global__ __launch_bounds__(1024) void k_sweep_resident(T *__restrict__ spin, int32_t *__restrict__ seed, Geom g,
+                                                         Rule r, int nsweeps, unsigned long long *__restrict__ acc) {
+  extern __shared__ __align__(16) unsigned char lds[];
+  __shared__ int s_thr[32];
+  const uint32_t slots = (uint32_t)g.K * (uint32_t)g.Mp;
+  const uint32_t spin_words = slots * (uint32_t)sizeof(T) / 4;   // Mp is a multiple of 4
+  T *ls = reinterpret_cast<T *>(lds);
+  int32_t *lk = reinterpret_cast<int32_t *>(lds + (spin_words * 4 + 15) / 16 * 16);
+  for (uint32_t i = threadIdx.x; i < spin_words; i += blockDim.x)
+    reinterpret_cast<uint32_t *>(ls)[i] = reinterpret_cast<const uint32_t *>(spin)[i];
+  for (uint32_t i = threadIdx.x; i < slots; i += blockDim.x) lk[i] = seed[i];
+  if (threadIdx.x < 28) s_thr[threadIdx.x] = r.thr[threadIdx.x];
+  __syncthreads();
+  const uint32_t npack = (uint32_t)g.Mp / 4;
+  for (int sw = 0; sw < nsweeps; sw++) {
+    int na = 0, de = 0;
+    for (int c = 0; c < g.K; c++) {
+      for (uint32_t p = threadIdx.x; p < npack; p += blockDim.x) {
+        int a0, b, e;
+        pack_site(g, p, a0, b, e);
+        const Counts cnt = sweep_sites<LAT, T, 4, TWO, uint32_t>(ls, lk, g, r, s_thr, c, a0, b, e);
+        na += cnt.nacc;
+        de += cnt.de;
+      }
+      __syncthreads();   // the next colour reads this colour's new spins
+    }
+    block_accumulate(na, de, acc + (size_t)sw * kAccRow, 0);
+    __syncthreads();
+  }
+  for (uint32_t i = threadIdx.x; i < spin_words; i += blockDim.x)
+    reinterpret_cast<uint32_t *>(spin)[i] = reinterpret_cast<const uint32_t *>(ls)[i];
+  for (uint32_t i = threadIdx.x; i < slots; i += blockDim.x) seed[i] = lk[i];
+}
+
 // One colour pass of a proposal variant (potts/neigh, potts/neighonly,
 // ising/single): one slot per thread (4 consecutive slots per pack, so a
 // wavefront's own-site accesses stay unit-stride), neighbours read in the
@@ -226,6 +266,9 @@ struct spk_hip_lattice {
   uint32_t *bits = nullptr;
   int32_t *sseed = nullptr;
   int64_t sseed_cap = 0;               // shadow seed slots allocated
+  // lattices that fit one workgroup's LDS run every sweep in one launch
+  int resident = 1;
+  size_t max_lds = 0;                  // LDS bytes a workgroup may use
   int tile_x = 0, tile_y = 0;          // fused core tile
   int32_t *stage = nullptr;            // N int32, natural order (transfers)
   unsigned long long *acc = nullptr;   // [cap][kAccRow]: per-sweep slots
@@ -473,6 +516,89 @@ int launch_fold(spk_hip_lattice *h, const unsigned long long *acc, int rows, uns
   return check_hip(hipGetLastError(), "fold launch");
 }
 
+// LDS bytes of the resident kernel's copy of the state (spins, then seeds
+// at a 16-byte boundary), or 0 when the lattice does not fit one workgroup.
+size_t resident_bytes(const spk_hip_lattice *h) {
+  if (variant_app(h->d.app) || h->fuse || !h->resident) return 0;
+  const size_t slots = (size_t)h->g.K * h->g.Mp;
+  const size_t bytes = (slots * spin_bytes(h->stype) + 15) / 16 * 16 + slots * sizeof(int32_t);
+  return bytes <= h->max_lds ? bytes : 0;
+}
+
+// nsweeps sweeps through the resident kernel, in pieces of at most acc_cap
+// sweeps; each piece's rows are folded and copied to pinned rows [done, ..).
+int run_resident(spk_hip_lattice *h, int nsweeps, size_t lds, int64_t *nlaunch) {
+  const size_t row = 2 * sizeof(unsigned long long);
+  int rc;
+  if ((rc = check_hip(hipEventRecord(h->ev0, h->stream), "event"))) return rc;
+  for (int done = 0; done < nsweeps;) {
+    const int n = std::min(nsweeps - done, h->acc_cap);
+    hipLaunchKernelGGL(k_zero_u64, dim3(grid_for((int64_t)n * kAccRow, 256)), dim3(256), 0, h->stream, h->acc,
+                       (int64_t)n * kAccRow);
+    const bool two = h->rule.nspins == 2;
+    with_lattice(h->g.lat, [&]<int LAT>() {
+      with_type(h->stype, [&]<typename T>() {
+        auto go = [&](auto kern) {
+          (void)hipFuncSetAttribute(reinterpret_cast<const void *>(kern),
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+          hipLaunchKernelGGL(kern, dim3(1), dim3(1024), lds, h->stream, static_cast<T *>(h->spin), h->seed, h->g,
+                             h->rule, n, h->acc);
+        };
+        if constexpr (sizeof(T) == 1) {
+          if (two) {
+            go(k_sweep_resident<LAT, T, true>);
+            return;
+          }
+        }
+        go(k_sweep_resident<LAT, T, false>);
+      });
+    });
+    if ((rc = check_hip(hipGetLastError(), "resident sweep launch"))) return rc;
+    (*nlaunch)++;
+    if ((rc = launch_fold(h, h->acc, n, h->accc)) ||
+        (rc = check_hip(hipMemcpyAsync(h->pinned + (size_t)done * 2, h->accc, row * n, hipMemcpyDeviceToHost,
+                                       h->stream),
+                        "accumulator readback")))
+      return rc;
+    done += n;
+  }
+  return check_hip(hipEventRecord(h->ev1, h->stream), "event");
+}
+
+void row_sum(const spk_hip_lattice *h, int row, int64_t *x, int64_t *y);
+
+// After the launches of a sweeps call: waits, takes the timing, and turns
+// the folded rows in pinned memory into cumulative Naccept and energies.
+int finish_sweeps(spk_hip_lattice *h, int nsweeps, int64_t nlaunch, int64_t e0, int64_t *naccept_total,
+                  int64_t *naccept_per_sweep, double *energy_per_sweep) {
+  const size_t row = 2 * sizeof(unsigned long long);
+  int rc;
+  if ((rc = check_hip(hipStreamSynchronize(h->stream), "sweeps sync"))) return rc;
+  float ms = 0.f;
+  if ((rc = check_hip(hipEventElapsedTime(&ms, h->ev0, h->ev1), "event time"))) return rc;
+  h->last_ms = ms;
+  h->last_nlaunch = nlaunch;
+  if (h->hacc.size() < (size_t)nsweeps * 2) h->hacc.resize((size_t)nsweeps * 2);
+  std::memcpy(h->hacc.data(), h->pinned, row * nsweeps);
+
+  int64_t nacc = 0, eng = e0;
+  const bool track = h->energy_valid || energy_per_sweep;
+  if (track && !energy_per_sweep) eng = h->energy;
+  for (int s = 0; s < nsweeps; s++) {
+    int64_t a, de;
+    row_sum(h, s, &a, &de);
+    nacc += a;
+    eng += 2 * de;   // a flip changes site i's and its neighbours' counts
+    if (naccept_per_sweep) naccept_per_sweep[s] = nacc;
+    if (energy_per_sweep) energy_per_sweep[s] = (double)eng;
+  }
+  if (naccept_total) *naccept_total += nacc;
+  if (track) h->energy = eng;
+  h->energy_valid = track;
+  return SPK_HIP_OK;
+}
+
+
 // the graph of one chunk starting from the current state buffer: zero the
 // chunk's accumulator rows, then its sweeps, then fold the rows.  A chunk holds an even number
 // of fused launches, so it ends in the buffer it started from.
@@ -626,6 +752,14 @@ int spk_hip_create(const spk_hip_desc *d, spk_hip_lattice **out) {
       (rc = check_hip(hipEventCreate(&h->ev1), "hipEventCreate"))) {
     spk_hip_destroy(h);
     return rc;
+  }
+  {
+    // LDS a workgroup may take, less room for the resident kernel's static arrays
+    int lds = 0, optin = 0;
+    (void)hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, h->device);
+    (void)hipDeviceGetAttribute(&optin, hipDeviceAttributeSharedMemPerBlockOptin, h->device);
+    const int most = std::max(lds, optin);
+    h->max_lds = most > 1024 ? (size_t)most - 1024 : 0;
   }
   if (hipMalloc(&h->spin, (size_t)h->slots * spin_bytes(h->stype)) != hipSuccess ||
       hipMalloc(&h->seed, (size_t)h->slots * sizeof(int32_t)) != hipSuccess ||
@@ -787,6 +921,14 @@ int spk_hip_sweeps(spk_hip_lattice *h, int nsweeps, int64_t *naccept_total, int6
     if (h->energy_valid) e0 = h->energy;
     else if ((rc = compute_energy(h, &e0))) return rc;
   }
+  const size_t row = 2 * sizeof(unsigned long long);   // a folded row
+  int64_t nlaunch = 0;
+  if (const size_t lds = resident_bytes(h)) {
+    if ((rc = ensure_acc(h, std::min(nsweeps, kInitialAccRows))) || (rc = ensure_pinned(h, nsweeps)) ||
+        (rc = run_resident(h, nsweeps, lds, &nlaunch)))
+      return rc;
+    return finish_sweeps(h, nsweeps, nlaunch, e0, naccept_total, naccept_per_sweep, energy_per_sweep);
+  }
   // whole chunks replay a captured graph; the rest launch eagerly.  Chunk
   // rows live in device rows [0, chunk), eager rows after them; every row
   // reaches the pinned host copy in stream order.
@@ -794,13 +936,11 @@ int spk_hip_sweeps(spk_hip_lattice *h, int nsweeps, int64_t *naccept_total, int6
   const int nchunks = C > 0 ? nsweeps / C : 0;
   const int rem = nsweeps - nchunks * C;
   if ((rc = ensure_acc(h, C + rem)) || (rc = ensure_pinned(h, nsweeps))) return rc;
-  const size_t row = 2 * sizeof(unsigned long long);   // a folded row
   const size_t first_rem = nchunks ? C : 0;             // eager rows follow the chunk's rows
   unsigned long long *rem_acc = h->acc + first_rem * kAccRow;
   // two-state sq/4n: the state moves to bit planes for the call (packing
   // and unpacking lie outside the timed events)
   if (bits_eligible(h) && (rc = bits_enter(h))) return rc;
-  int64_t nlaunch = 0;
   auto launch_all = [&]() -> int {
     int rc;
     if ((rc = check_hip(hipEventRecord(h->ev0, h->stream), "event"))) return rc;
@@ -831,29 +971,7 @@ int spk_hip_sweeps(spk_hip_lattice *h, int nsweeps, int64_t *naccept_total, int6
                                              hipMemcpyDeviceToHost, h->stream),
                               "accumulator readback"))))
     return rc;
-  if ((rc = check_hip(hipStreamSynchronize(h->stream), "sweeps sync"))) return rc;
-  float ms = 0.f;
-  if ((rc = check_hip(hipEventElapsedTime(&ms, h->ev0, h->ev1), "event time"))) return rc;
-  h->last_ms = ms;
-  h->last_nlaunch = nlaunch;
-  if (h->hacc.size() < (size_t)nsweeps * 2) h->hacc.resize((size_t)nsweeps * 2);
-  std::memcpy(h->hacc.data(), h->pinned, row * nsweeps);
-
-  int64_t nacc = 0, eng = e0;
-  const bool track = h->energy_valid || energy_per_sweep;
-  if (track && !energy_per_sweep) eng = h->energy;
-  for (int s = 0; s < nsweeps; s++) {
-    int64_t a, de;
-    row_sum(h, s, &a, &de);
-    nacc += a;
-    eng += 2 * de;   // a flip changes site i's and its neighbours' counts
-    if (naccept_per_sweep) naccept_per_sweep[s] = nacc;
-    if (energy_per_sweep) energy_per_sweep[s] = (double)eng;
-  }
-  if (naccept_total) *naccept_total += nacc;
-  if (track) h->energy = eng;
-  h->energy_valid = track;
-  return SPK_HIP_OK;
+  return finish_sweeps(h, nsweeps, nlaunch, e0, naccept_total, naccept_per_sweep, energy_per_sweep);
 }
 
 int spk_hip_energy(spk_hip_lattice *h, double *energy) {
@@ -894,6 +1012,12 @@ int spk_hip_set_launch(spk_hip_lattice *h, int block) {
     return set_error(SPK_HIP_EINVAL, "block must be a power of two in [64, 1024]");
   h->block = block;
   drop_graphs(h);
+  return SPK_HIP_OK;
+}
+
+int spk_hip_set_resident(spk_hip_lattice *h, int enable) {
+  if (!h) return set_error(SPK_HIP_EINVAL, "null handle");
+  h->resident = enable ? 1 : 0;
   return SPK_HIP_OK;
 }
 

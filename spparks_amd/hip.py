@@ -24,7 +24,7 @@ ABI_SYMBOLS = (
     "spk_hip_create", "spk_hip_destroy", "spk_hip_set_temperature",
     "spk_hip_upload", "spk_hip_download", "spk_hip_set_range",
     "spk_hip_set_value", "spk_hip_init_seeds", "spk_hip_sweeps",
-    "spk_hip_energy", "spk_hip_sync", "spk_hip_set_launch", "spk_hip_set_bitplanes",
+    "spk_hip_energy", "spk_hip_sync", "spk_hip_set_launch", "spk_hip_set_bitplanes", "spk_hip_set_resident",
     "spk_hip_last_timing",
     "spk_hip_batch_create", "spk_hip_batch_destroy", "spk_hip_replica_bytes",
     "spk_hip_batch_add", "spk_hip_batch_remove", "spk_hip_batch_upload",
@@ -79,6 +79,7 @@ def load_library(path=None):
     L.spk_hip_sync.argtypes = [vp]
     L.spk_hip_set_launch.argtypes = [vp, C.c_int]
     L.spk_hip_set_bitplanes.argtypes = [vp, C.c_int]
+    L.spk_hip_set_resident.argtypes = [vp, C.c_int]
     L.spk_hip_checksum.argtypes = [vp, C.POINTER(C.c_uint64)]
     L.spk_hip_set_fusion.argtypes = [vp, C.c_int]
     L.spk_hip_set_graph.argtypes = [vp, C.c_int]
@@ -234,6 +235,9 @@ class Lattice:
 
     def set_launch(self, block):
         _check(self.L.spk_hip_set_launch(self.h, int(block)))
+
+    def set_resident(self, enable):
+        _check(self.L.spk_hip_set_resident(self.h, int(bool(enable))))
 
     def set_bitplanes(self, enable):
         """0/False: off; 1/True: on; 8, 16 or 32: on with that many sites per thread."""

@@ -61,17 +61,20 @@ CASES = [
 ]
 
 
-@pytest.mark.parametrize("block", [512, 64])
+@pytest.mark.parametrize("block,resident", [(512, 0), (64, 0), (0, 1)], ids=["b512", "b64", "resident"])
 @pytest.mark.parametrize("case", CASES, ids=lambda c: "%s-%s-%s" % (c[0], c[1], "x".join(map(str, c[2]))))
-def test_small_lattice_bit_exact(orc, case, block):
-    """Every lattice x app x storage type, with workgroup tiles of two shapes
-    (partial tiles at the sub-grid's edges)."""
+def test_small_lattice_bit_exact(orc, case, block, resident):
+    """Every lattice x app x storage type: per-colour kernels with workgroup
+    tiles of two shapes (partial tiles at the sub-grid's edges), and the
+    one-workgroup kernel that keeps the whole state in LDS."""
     import spparks_amd as S
     app, lat, shape, q, kt, seed, n = case
-    say("case", case, "block", block)
+    say("case", case, "block", block, "resident", resident)
     spin0, seeds0, spin1, seeds1, nacc, eng = oracle_run(orc, app, lat, shape, q, kt, seed, n)
     with gpu_init(S, app, lat, shape, q, kt, seed) as L:
-        L.set_launch(block)
+        L.set_resident(resident)
+        if block:
+            L.set_launch(block)
         gs0, gk0 = L.download()
         assert np.array_equal(gs0, spin0), "set site range on device"
         assert np.array_equal(gk0, seeds0), "tagreset seeds on device"
@@ -171,6 +174,7 @@ def test_launch_shape_independence(block):
     import spparks_amd as S
     ref = None
     with gpu_init(S, "potts", "sc/26n", (16, 12, 10), 50, 0.7, 99) as L:
+        L.set_resident(False)
         L.set_launch(block)
         _, nacc, eng = L.sweeps(5)
         spin, seeds = L.download()
@@ -217,6 +221,7 @@ def test_graph_replays_equal_eager_launches(case):
     runs = []
     for chunk in (32, 0):
         with gpu_init(S, app, lat, shape, q, kt, seed) as L:
+            L.set_resident(False)
             L.set_graph(chunk)
             out = []
             for n in (70, 33, 64):
@@ -316,6 +321,7 @@ def test_two_state_bit_planes_bit_exact(orc, case, planes):
     say("case", case, "bit planes", planes)
     _, _, spin1, seeds1, nacc, eng = oracle_run(orc, app, lat, shape, q, kt, seed, n)
     with gpu_init(S, app, lat, shape, q, kt, seed) as L:
+        L.set_resident(False)
         L.set_bitplanes(planes)
         tot, gn, ge = L.sweeps(n)
         gs1, gk1 = L.download()
@@ -349,3 +355,27 @@ def test_bit_planes_equal_bytes_on_a_large_lattice():
             if a is None:
                 continue
             assert a[0] == b[0] and np.array_equal(a[1], b[1]) and np.array_equal(a[2], b[2])
+
+
+@pytest.mark.parametrize("case", [("potts", "sq/8n", (16, 12, 1), 100, 0.5, 56789),
+                                  ("ising", "sq/4n", (66, 10, 1), 2, 2.269, 9),
+                                  ("potts", "sc/26n", (10, 8, 6), 70000, 0.6, 3)],
+                         ids=["potts-sq8n-u8", "ising-sq4n-ragged", "potts-sc26n-i32"])
+def test_resident_kernel_equals_per_colour_kernels(case):
+    """1100 sweeps in one call (more than one launch's 1024 accumulator
+    rows), then 7 more: the one-workgroup LDS kernel and the per-colour
+    kernels give the same spins, seeds, Naccept and energies."""
+    import spparks_amd as S
+    app, lat, shape, q, kt, seed = case
+    out = []
+    for resident in (1, 0):
+        say("case", case, "resident", resident)
+        with gpu_init(S, app, lat, shape, q, kt, seed) as L:
+            L.set_resident(resident)
+            a = L.sweeps(1100)
+            b = L.sweeps(7)
+            out.append((a, b) + L.download())
+    (a1, b1, s1, k1), (a2, b2, s2, k2) = out
+    assert np.array_equal(s1, s2) and np.array_equal(k1, k2)
+    for x, y in ((a1, a2), (b1, b2)):
+        assert x[0] == y[0] and np.array_equal(x[1], y[1]) and np.array_equal(x[2], y[2])
