@@ -379,3 +379,43 @@ def test_resident_kernel_equals_per_colour_kernels(case):
     assert np.array_equal(s1, s2) and np.array_equal(k1, k2)
     for x, y in ((a1, a2), (b1, b2)):
         assert x[0] == y[0] and np.array_equal(x[1], y[1]) and np.array_equal(x[2], y[2])
+
+
+def _random_cases(n=16, seed=20261202):
+    """Seeded random lattices: style, app, Q (all three storage types), even
+    edges, kT (including 0), sweep counts."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for k in range(n):
+        lat = ["sq/4n", "sq/8n", "sc/6n", "sc/26n"][k % 4]
+        app = "ising" if rng.random() < 0.3 else "potts"
+        q = 2 if app == "ising" else int(rng.choice([2, 3, 17, 255, 300, 70000]))
+        if lat.startswith("sq"):
+            shape = (2 * int(rng.integers(1, 40)), 2 * int(rng.integers(1, 24)), 1)
+        else:
+            shape = (2 * int(rng.integers(1, 9)), 2 * int(rng.integers(1, 7)), 2 * int(rng.integers(1, 6)))
+        kt = 0.0 if rng.random() < 0.15 else round(float(rng.uniform(0.2, 3.5)), 3)
+        out.append((app, lat, shape, q, kt, int(rng.integers(1, 10 ** 6)), int(rng.integers(2, 6))))
+    return out
+
+
+@pytest.mark.parametrize("case", _random_cases(), ids=lambda c: "%s-%s-%s-q%d-T%g" % (c[0], c[1], "x".join(map(str, c[2])), c[3], c[4]))
+def test_random_lattices_every_path_bit_exact(orc, case):
+    """Seeded random cases through every path that applies to them (the
+    one-workgroup LDS kernel, the per-colour kernels, and for two-state sq/4n
+    the bit planes) against the oracle."""
+    import spparks_amd as S
+    app, lat, shape, q, kt, seed, n = case
+    _, _, spin1, seeds1, nacc, eng = oracle_run(orc, app, lat, shape, q, kt, seed, n)
+    paths = [(1, 0), (0, 0)]
+    if lat == "sq/4n" and q == 2:
+        paths.append((0, 1))
+    for resident, planes in paths:
+        say("case", case, "resident", resident, "bit planes", planes)
+        with gpu_init(S, app, lat, shape, q, kt, seed) as L:
+            L.set_resident(resident)
+            L.set_bitplanes(planes)
+            _, gn, ge = L.sweeps(n)
+            gs, gk = L.download()
+        assert np.array_equal(gs, spin1) and np.array_equal(gk, seeds1), (resident, planes)
+        assert np.array_equal(gn, nacc) and np.array_equal(ge, eng), (resident, planes)
