@@ -1,7 +1,7 @@
 #!/usr/bin/env python3
 """bench.py -- strict colour sweep throughput on one MI355X.
 
-    python bench.py [--steps K] [--warmup W] [--workload cfg2]
+    python bench.py [--steps K] [--warmup W] [--workload ising-sq4n-4096]
 
 A step is one strict sweep (every colour once) over the whole lattice, i.e.
 nx*ny*nz site updates.  The default workload is the 4096x4096 sq/4n Ising
@@ -25,13 +25,13 @@ HBM_PEAK_GBS = 8000.0
 
 # name -> (app, lattice, shape, Q, kT, seed, sweeps, description)
 WORKLOADS = {
-    "cfg1": ("potts", "sq/8n", (128, 128, 1), 100, 0.5, 56789, 100,
+    "potts-sq8n-128": ("potts", "sq/8n", (128, 128, 1), 100, 0.5, 56789, 100,
              "Potts Q=100 sq/8n 128x128 periodic, kT=0.5, seed 56789"),
-    "cfg2": ("ising", "sq/4n", (4096, 4096, 1), 2, 2.269, 12345, 1000,
+    "ising-sq4n-4096": ("ising", "sq/4n", (4096, 4096, 1), 2, 2.269, 12345, 1000,
              "Ising sq/4n 4096x4096 periodic, kT=2.269, seed 12345"),
-    "ising8k": ("ising", "sq/4n", (8192, 8192, 1), 2, 2.269, 12345, 250,
-                "Ising sq/4n 8192x8192 periodic, kT=2.269, seed 12345 (a larger lattice of the cfg2 kind)"),
-    "cfg3": ("potts", "sc/26n", (256, 256, 256), 10000, 0.5, 24680, 200,
+    "ising-sq4n-8192": ("ising", "sq/4n", (8192, 8192, 1), 2, 2.269, 12345, 250,
+                "Ising sq/4n 8192x8192 periodic, kT=2.269, seed 12345 (four times the 4096x4096 lattice)"),
+    "potts-sc26n-256": ("potts", "sc/26n", (256, 256, 256), 10000, 0.5, 24680, 200,
              "Potts Q=10000 sc/26n 256^3 periodic, kT=0.5, mask yes, seed 24680"),
 }
 
@@ -93,7 +93,7 @@ def _hash(a, b):
 
 
 def run_ensemble(args):
-    """cfg4: 512 independent replicas (256 Ising sq/4n, 256 Potts sc/26n),
+    """Ensemble of 512 independent replicas (256 Ising sq/4n, 256 Potts sc/26n),
     100 strict sweeps each.  Batched: one device pool, one persistent kernel
     per (variant, colour step), 25-sweep blocks replayed as HIP graphs, launch
     shape from the autotuner.  Plain: the same replicas one at a time through
@@ -176,7 +176,7 @@ def run_ensemble(args):
         "ms_per_step": round(wall_b * 1e3 / nsw, 4), "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "int32", "data": "synthetic: 512 replicas from "
         "spparks_amd.workloads.ensemble_replicas(), each initialised from its own seed",
-        "config": {"workload": "cfg4: batched ensemble of 512 replicas (256 Ising sq/4n L 64-1024, 256 Potts "
+        "config": {"workload": "ensemble-512: batched ensemble of 512 replicas (256 Ising sq/4n L 64-1024, 256 Potts "
                                "sc/26n L 32-128, Q 100-1e5), %d sweeps each in %d-sweep graph blocks" % (nsw, block),
                    "replicas": len(reps), "sites": sum(sites), "site_updates": updates,
                    "launch_shape": {"block": shape[0], "grid": shape[1]}},
@@ -185,7 +185,7 @@ def run_ensemble(args):
         "parity": {"replicas_compared": len(reps), "mismatches": mismatch,
                    "compared": "final spins+seeds hash, Naccept, energy: batched vs plain"},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": load_traffic("cfg4"),
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": load_traffic("ensemble-512"),
                      "kernel": "k_batch_step (all variants of the block)",
                      "device_ms": round(dev_ms, 3), "step_launches": nl},
         "cpu_baseline": None,
@@ -193,7 +193,7 @@ def run_ensemble(args):
 
 
 def run_serving(args):
-    """cfg5: Poisson streams of 3d Potts grain-growth solves (sc/26n, edges
+    """Serving: Poisson streams of 3d Potts grain-growth solves (sc/26n, edges
     32-192, Q 100-1e6, kT 0.1-1.0, 50-500 sweeps) at offered loads of 10, 50
     and 200 requests/s, continuously batched through spk_hip_serve against a
     200 GB HBM budget.  Reports p50/p99 completion latency and aggregate
@@ -243,7 +243,7 @@ def run_serving(args):
         "steps": sum(l["requests"] for l in levels), "warmup": 0, "ms_per_step": None,
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "int32",
         "data": "synthetic: Poisson request streams from spparks_amd.workloads.serving_requests()",
-        "config": {"workload": "cfg5: serving 3d Potts sc/26n solves at 10/50/200 req/s, %g GB HBM budget, "
+        "config": {"workload": "serving-sc26n: serving 3d Potts sc/26n solves at 10/50/200 req/s, %g GB HBM budget, "
                                "%d-sweep blocks" % (args.serve_budget_gb, args.serve_block)},
         "levels": levels,
         "p99_latency_s_at_200": top["p99_latency_s"],
@@ -381,14 +381,14 @@ def main():
     ap.add_argument("--steps", type=int, default=None, help="timed sweeps (default: the workload's count)")
     ap.add_argument("--warmup", type=int, default=64,
                     help="untimed sweeps first (captures the sweep graphs)")
-    ap.add_argument("--workload", default="cfg2", choices=sorted(WORKLOADS) + ["cfg4", "cfg5"])
+    ap.add_argument("--workload", default="ising-sq4n-4096", choices=sorted(WORKLOADS) + ["ensemble-512", "serving-sc26n"])
     ap.add_argument("--serve-n", type=int, nargs=3, default=[150, 500, 1000], metavar=("N10", "N50", "N200"),
-                    help="cfg5: requests per offered load")
-    ap.add_argument("--serve-block", type=int, default=10, help="cfg5: sweeps per batch block")
+                    help="serving: requests per offered load")
+    ap.add_argument("--serve-block", type=int, default=10, help="serving: sweeps per batch block")
     ap.add_argument("--serve-budget-gb", type=float, default=200.0)
-    ap.add_argument("--serve-check", type=int, default=8, help="cfg5: requests per load re-solved alone")
-    ap.add_argument("--block-sweeps", type=int, default=25, help="cfg4: sweeps per graph block")
-    ap.add_argument("--grid", type=int, default=0, help="cfg4: persistent workgroups (with --block)")
+    ap.add_argument("--serve-check", type=int, default=8, help="serving: requests per load re-solved alone")
+    ap.add_argument("--block-sweeps", type=int, default=25, help="ensemble: sweeps per graph block")
+    ap.add_argument("--grid", type=int, default=0, help="ensemble: persistent workgroups (with --block)")
     ap.add_argument("--block", type=int, default=0, help="sweep-kernel workgroup size (0 = default)")
     ap.add_argument("--no-bitplanes", action="store_true",
                     help="two-state sq/4n lattices on the byte-per-site kernel instead of bit planes")
@@ -412,9 +412,9 @@ def main():
               "(a single lattice would shard by slabs with a per-colour halo; replicas by "
               "replica).  Run with --gpus 1.", file=sys.stderr)
         sys.exit(2)
-    if args.workload == "cfg4":
+    if args.workload == "ensemble-512":
         print(json.dumps(run_ensemble(args)), flush=True)
-    elif args.workload == "cfg5":
+    elif args.workload == "serving-sc26n":
         print(json.dumps(run_serving(args)), flush=True)
     else:
         print(json.dumps(run(args)), flush=True)

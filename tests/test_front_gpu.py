@@ -49,10 +49,10 @@ def stats_rows(out):
     return rows
 
 
-def test_cfg1_script_known_answers(tmp_path):
-    """in.cfg1 unchanged: the stats lines carry the reference run's numbers."""
-    g = json.load(open(os.path.join(GOLD, "cfg1_known_answers.json")))
-    out = run_exe(tmp_path, os.path.join(SCRIPTS, "in.cfg1"), "-sf", "hip")
+def test_potts_sq8n_128_script_known_answers(tmp_path):
+    """in.potts_sq8n_128 unchanged: the stats lines carry the reference run's numbers."""
+    g = json.load(open(os.path.join(GOLD, "potts_sq8n_128_known_answers.json")))
+    out = run_exe(tmp_path, os.path.join(SCRIPTS, "in.potts_sq8n_128"), "-sf", "hip")
     rows = stats_rows(out)
     assert rows[0] == (0.0, 0, 0, 0, float(g["initial_energy"]))
     assert rows[-1][3] == 100
@@ -68,7 +68,7 @@ def test_library_api_matches_oracle(orc, tmp_path, monkeypatch):
     from spparks_amd.spparks import spparks
     monkeypatch.chdir(tmp_path)
     spk = spparks(cmdargs=["-screen", "none", "-log", "none"])
-    lines = open(os.path.join(SCRIPTS, "in.cfg1")).read().replace("run             1.0", "run 0.1")
+    lines = open(os.path.join(SCRIPTS, "in.potts_sq8n_128")).read().replace("run             1.0", "run 0.1")
     for line in lines.splitlines():
         spk.command(line)
     # time accumulates 0.01 per sweep in floating point: `run 0.1` is 11 sweeps
@@ -111,10 +111,10 @@ def test_sweep_count_from_time_accumulation(tmp_path):
 
 
 def test_full_size_scripts_first_sweeps(orc, tmp_path):
-    """in.cfg2 / in.cfg3 unchanged except the run length (-var T): the first
+    """in.ising_sq4n_4096 / in.potts_sc26n_256 unchanged except the run length (-var T): the first
     two sweeps of the 4096^2 Ising and 256^3 Potts runs against the oracle."""
-    cases = [("in.cfg2", "ising", "sq/4n", (4096, 4096, 1), 2, 2.269, 12345, "1.0", "0.5"),
-             ("in.cfg3", "potts", "sc/26n", (256, 256, 256), 10000, 0.5, 24680, "0.00015", "1.0e-4")]
+    cases = [("in.ising_sq4n_4096", "ising", "sq/4n", (4096, 4096, 1), 2, 2.269, 12345, "1.0", "0.5"),
+             ("in.potts_sc26n_256", "potts", "sc/26n", (256, 256, 256), 10000, 0.5, 24680, "0.00015", "1.0e-4")]
     for script, app, lat, shape, q, kt, seed, T, DT in cases:
         out = run_exe(tmp_path, os.path.join(SCRIPTS, script), "-var", "T", T, "-var", "DT", DT)
         rows = stats_rows(out)

@@ -126,9 +126,9 @@ def test_variant_bit_exact(orc, case):
     assert np.array_equal(ge, eng)
 
 
-def test_cfg1_known_answers_on_gpu():
+def test_potts_sq8n_128_known_answers_on_gpu():
     import spparks_amd as S
-    g = json.load(open(os.path.join(GOLD, "cfg1_known_answers.json")))
+    g = json.load(open(os.path.join(GOLD, "potts_sq8n_128_known_answers.json")))
     with gpu_init(S, g["app"], g["lattice"], (g["nx"], g["ny"], g["nz"]), g["nspins"],
                   g["temperature"], g["seed"]) as L:
         assert L.energy() == g["initial_energy"]
@@ -279,7 +279,7 @@ FULL = [
 ]
 
 
-@pytest.mark.parametrize("case", FULL, ids=["cfg2-ising-4096sq", "cfg3-potts-256cube"])
+@pytest.mark.parametrize("case", FULL, ids=["ising-sq4n-4096", "potts-sc26n-256"])
 def test_full_size_first_sweeps_bit_exact(orc, case):
     import spparks_amd as S
     app, lat, shape, q, kt, seed, n = case

@@ -133,8 +133,8 @@ def test_odd_edge_rejected(orc):
 
 # ------------------------------------------------------------- known answers
 
-def test_cfg1_known_answers(orc):
-    g = json.load(open(os.path.join(GOLD, "cfg1_known_answers.json")))
+def test_potts_sq8n_128_known_answers(orc):
+    g = json.load(open(os.path.join(GOLD, "potts_sq8n_128_known_answers.json")))
     d = orc.make_desc(g["app"], g["lattice"], g["nx"], g["ny"], g["nz"], g["nspins"], g["temperature"])
     spin, seeds = orc.script_init(d, g["seed"], g["lo"], g["hi"])
     assert orc.energy(d, spin) == g["initial_energy"]
