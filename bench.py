@@ -150,7 +150,7 @@ def run_ensemble(args):
     # the same replicas one at a time through the plain single-lattice sweep
     wall_p = 0.0
     mismatch = 0
-    for r in reps:
+    for r in ([] if args.skip_plain else reps):
         with S.Lattice(r["app"], r["lattice"], *r["shape"], nspins=r["nspins"], temperature=r["kT"]) as L:
             s0, rseed = S.strict_init_params(r["seed"])
             L.set_range(s0, 1, r["nspins"])
@@ -166,7 +166,7 @@ def run_ensemble(args):
             mismatch += 1
 
     value = updates / wall_b / 1e9
-    plain = updates / wall_p / 1e9
+    plain = updates / wall_p / 1e9 if wall_p > 0 else None
     bytes_alg = sum(n * nsw * bytes_per_update(r["lattice"], spin_bytes(r["app"], r["nspins"]))
                     for n, r in zip(sites, reps))
     achieved = bytes_alg / (dev_ms / 1e3) / 1e9
@@ -180,9 +180,10 @@ def run_ensemble(args):
                                "sc/26n L 32-128, Q 100-1e5), %d sweeps each in %d-sweep graph blocks" % (nsw, block),
                    "replicas": len(reps), "sites": sum(sites), "site_updates": updates,
                    "launch_shape": {"block": shape[0], "grid": shape[1]}},
-        "batched_vs_plain_speedup": round(value / plain, 3),
-        "plain_one_at_a_time": {"value": round(plain, 4), "unit": "G site-updates/s", "seconds": round(wall_p, 4)},
-        "parity": {"replicas_compared": len(reps), "mismatches": mismatch,
+        "batched_vs_plain_speedup": round(value / plain, 3) if plain else None,
+        "plain_one_at_a_time": ({"value": round(plain, 4), "unit": "G site-updates/s", "seconds": round(wall_p, 4)}
+                                if plain else None),
+        "parity": {"replicas_compared": 0 if args.skip_plain else len(reps), "mismatches": mismatch,
                    "compared": "final spins+seeds hash, Naccept, energy: batched vs plain"},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": load_traffic("ensemble-512"),
@@ -387,6 +388,8 @@ def main():
     ap.add_argument("--serve-block", type=int, default=10, help="serving: sweeps per batch block")
     ap.add_argument("--serve-budget-gb", type=float, default=200.0)
     ap.add_argument("--serve-check", type=int, default=8, help="serving: requests per load re-solved alone")
+    ap.add_argument("--skip-plain", action="store_true",
+                    help="ensemble: time the batched run only (no one-at-a-time comparison)")
     ap.add_argument("--block-sweeps", type=int, default=25, help="ensemble: sweeps per graph block")
     ap.add_argument("--grid", type=int, default=0, help="ensemble: persistent workgroups (with --block)")
     ap.add_argument("--block", type=int, default=0, help="sweep-kernel workgroup size (0 = default)")
