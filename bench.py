@@ -54,13 +54,15 @@ def bytes_per_update(lattice, sbytes):
 
 
 def load_traffic(workload):
+    """rocprofv3-measured HBM traffic of the workload's dominant kernel:
+    (bytes per launch or None, the measurement's details or None)."""
     p = os.path.join(ROOT, "profiles", "traffic.json")
     try:
         with open(p) as f:
-            t = json.load(f)
-        return t.get(workload)
+            t = json.load(f).get(workload)
     except (OSError, ValueError):
-        return None
+        t = None
+    return (t["bytes_per_launch"] if t else None), t
 
 
 def cpu_baseline(app, lattice, shape, q, kt, seed, budget_s, threads):
@@ -186,7 +188,7 @@ def run_ensemble(args):
         "parity": {"replicas_compared": 0 if args.skip_plain else len(reps), "mismatches": mismatch,
                    "compared": "final spins+seeds hash, Naccept, energy: batched vs plain"},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": load_traffic("ensemble-512"),
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": load_traffic("ensemble-512")[0],
                      "kernel": "k_batch_step (all variants of the block)",
                      "device_ms": round(dev_ms, 3), "step_launches": nl},
         "cpu_baseline": None,
@@ -306,7 +308,6 @@ def run(args):
     per_launch_s = dev_ms / 1e3 / nlaunch
     sb = spin_bytes(app, q)
     B = bytes_per_update(lattice, sb)
-    B_survey = bytes_per_update(lattice, 4)
     tname = {1: "u8", 2: "u16", 4: "i32"}[sb]
     resident = nlaunch < steps   # the whole state in one workgroup's LDS: one launch per call
     planes = (lattice == "sq/4n" and (app == "ising" or q == 2) and not fused and not args.no_bitplanes
@@ -339,6 +340,10 @@ def run(args):
         bytes_per_launch = B * n // K
         B_kernel = B
     achieved = bytes_per_launch / per_launch_s / 1e9
+    traffic, traffic_detail = load_traffic(args.workload + (
+        "-fused" if fused else "-bytes" if lattice == "sq/4n" and not planes and not resident and q <= 2 else ""))
+    if resident:
+        traffic, traffic_detail = None, None   # measured for the per-colour kernels only
     out = {
         "metric": "G site-updates/s & % HBM peak; batched vs plain speedup; p99 solve latency",
         "value": round(value, 4),
@@ -358,14 +363,13 @@ def run(args):
                    "naccept": int(nacc), "final_energy": energy},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": load_traffic(args.workload + ("-fused" if fused else "-bytes" if
-                                                              lattice == "sq/4n" and not planes and q <= 2 else "")),
+                     "traffic": traffic,
+                     "traffic_measurement": traffic_detail,
                      "kernel": kernel,
                      "bytes_per_launch": bytes_per_launch,
                      "avg_launch_us": round(per_launch_s * 1e6, 3),
                      "bytes_per_update": round(B_kernel, 3)},
         "pct_hbm_peak": round(100 * value * 1e9 * B / (HBM_PEAK_GBS * 1e9), 2),
-        "survey_model_pct_hbm": round(100 * value * 1e9 * B_survey / (HBM_PEAK_GBS * 1e9), 2),
         "spin_bytes": spin_bytes(app, q),
     }
     if not args.no_cpu_baseline:

@@ -78,3 +78,12 @@ def test_table_rejects_bad_temperature():
         S.metropolis_table(0.0, 8)
     with pytest.raises(S.HipError):
         S.metropolis_table(-1.0, 8)
+
+
+def test_launch_settings_reject_null_handles():
+    """The launch-setting calls check their handle before touching the GPU."""
+    from spparks_amd.hip import load_library
+    L = load_library()
+    for fn, arg in (("spk_hip_set_bitplanes", 1), ("spk_hip_set_resident", 1), ("spk_hip_set_launch", 256),
+                    ("spk_hip_set_graph", 32), ("spk_hip_set_fusion", 1)):
+        assert getattr(L, fn)(None, arg) == -1, fn   # SPK_HIP_EINVAL

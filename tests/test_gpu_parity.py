@@ -419,3 +419,14 @@ def test_random_lattices_every_path_bit_exact(orc, case):
             gs, gk = L.download()
         assert np.array_equal(gs, spin1) and np.array_equal(gk, seeds1), (resident, planes)
         assert np.array_equal(gn, nacc) and np.array_equal(ge, eng), (resident, planes)
+
+
+def test_bitplane_setting_rejects_bad_values():
+    import spparks_amd as S
+    with gpu_init(S, "ising", "sq/4n", (64, 48, 1), 2, 2.269, 1) as L:
+        for bad in (3, 64, -1):
+            with pytest.raises(S.HipError, match="bit planes"):
+                L.set_bitplanes(bad)
+        L.set_bitplanes(16)
+        L.set_resident(False)
+        L.sweeps(2)
