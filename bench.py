@@ -309,7 +309,9 @@ def run(args):
     sb = spin_bytes(app, q)
     B = bytes_per_update(lattice, sb)
     tname = {1: "u8", 2: "u16", 4: "i32"}[sb]
-    resident = nlaunch < steps   # the whole state in one workgroup's LDS: one launch per call
+    # the whole state in one workgroup's LDS: one launch per call (fused
+    # launches are also fewer than sweeps, but report their sweeps per launch)
+    resident = not fused and nlaunch < steps
     planes = (lattice == "sq/4n" and (app == "ising" or q == 2) and not fused and not args.no_bitplanes
               and not resident)
     if resident:
