@@ -1,0 +1,95 @@
+"""Committed fixtures against the library's host entry points and the
+workload generator (no GPU needed).
+
+metropolis_table.json holds the acceptance table bits from glibc's exp
+(src/app_potts.cpp:130, src/app_ising.cpp:120) and the integer thresholds
+found by bisection; ensemble_replicas.json is the 512-replica table the
+batched workload runs."""
+import decimal
+import json
+import math
+import os
+
+import pytest
+
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def load(name):
+    with open(os.path.join(GOLD, name)) as f:
+        return json.load(f)
+
+
+TABLES = load("metropolis_table.json")
+
+
+@pytest.mark.parametrize("kt", sorted(TABLES["tables"]))
+def test_library_table_matches_fixture(kt):
+    """spk_hip_metropolis_table gives the fixture's exp bits and thresholds."""
+    import spparks_amd as S
+    want = TABLES["tables"][kt]
+    tab, thr = S.metropolis_table(float(kt), TABLES["maxd"])
+    assert [float(x).hex() for x in tab[1:]] == want["exp_bits"][1:]
+    assert [int(t) for t in thr] == want["thr"]
+
+
+@pytest.mark.parametrize("kt", sorted(TABLES["tables"]))
+def test_fixture_is_exp_of_the_reference_argument(kt):
+    """Every stored value is within 0.51 ulp of exp((-d)*(1.0/kT)) evaluated
+    to 60 digits, and exactly the listed entries are off by more than half
+    an ulp.  A wrong sign, a dropped 1/kT or an off-by-one d is many ulps off."""
+    decimal.getcontext().prec = 60
+    want = TABLES["tables"][kt]
+    off = []
+    for d in range(1, TABLES["maxd"] + 1):
+        e = float.fromhex(want["exp_bits"][d])
+        x = (-d) * (1.0 / float(kt))
+        if e == 0.0:
+            assert x < -745.0
+            continue
+        err = abs(decimal.Decimal(e) - decimal.Decimal(x).exp()) / decimal.Decimal(math.ulp(e))
+        assert err < decimal.Decimal("0.51"), (kt, d, float(err))
+        if err > decimal.Decimal("0.5"):
+            off.append(d)
+    assert off == want["not_correctly_rounded"]
+
+
+def test_table_is_glibc_not_correctly_rounded_exp():
+    """At kT 2.269, d 8 glibc's exp is not the correctly rounded double; the
+    strict sweep must carry glibc's value, as the reference does."""
+    want = TABLES["tables"]["2.269"]
+    assert want["not_correctly_rounded"] == [8]
+    e = float.fromhex(want["exp_bits"][8])
+    assert e == math.exp((-8) * (1.0 / 2.269))
+    decimal.getcontext().prec = 60
+    exact = decimal.Decimal((-8) * (1.0 / 2.269)).exp()
+    nearest = min((e, math.nextafter(e, 0.0), math.nextafter(e, 1.0)),
+                  key=lambda v: abs(decimal.Decimal(v) - exact))
+    assert nearest != e
+
+
+@pytest.mark.parametrize("kt", sorted(TABLES["tables"]))
+def test_fixture_thresholds_are_the_comparison(kt):
+    """thr[d] is where (1.0/2147483647)*s first exceeds tab[d]; thr[0] is
+    where it first reaches 0.5."""
+    am = 1.0 / 2147483647
+    want = TABLES["tables"][kt]
+    t0 = want["thr"][0]
+    assert am * t0 >= 0.5 and not (am * (t0 - 1) >= 0.5)
+    for d in range(1, TABLES["maxd"] + 1):
+        e, t = float.fromhex(want["exp_bits"][d]), want["thr"][d]
+        assert 1 <= t <= 2147483647
+        if t < 2147483647:
+            assert am * t > e
+        if t > 1:
+            assert not (am * (t - 1) > e)
+
+
+def test_ensemble_generator_matches_fixture():
+    """The batched workload's replica table is the committed one."""
+    from spparks_amd.workloads import ensemble_replicas
+    want = load("ensemble_replicas.json")
+    got = ensemble_replicas()
+    assert got == want["replicas"]
+    assert len(got) == 512
+    assert sum(r["shape"][0] * r["shape"][1] * r["shape"][2] for r in got) > 150_000_000
