@@ -93,3 +93,22 @@ def test_ensemble_generator_matches_fixture():
     assert got == want["replicas"]
     assert len(got) == 512
     assert sum(r["shape"][0] * r["shape"][1] * r["shape"][2] for r in got) > 150_000_000
+
+
+def test_full_run_fixture_starts_as_the_oracle(orc):
+    """full_runs.json (written by tests/golden/make_full_runs.py) begins
+    where the oracle does: the initial state checksum and the first three
+    sweeps' Naccept and energy of both full-size runs."""
+    from spparks_amd.hip import checksum
+    runs = load("full_runs.json")["runs"]
+    assert [r["name"] for r in runs] == ["ising-sq4n-4096", "potts-sc26n-256"]
+    for r in runs:
+        assert len(r["naccept"]) == len(r["energy"]) == r["sweeps"]
+        assert len(r["checksum"]) == r["sweeps"] // r["every"] + 1
+        assert all(b > a for a, b in zip(r["naccept"], r["naccept"][1:]))
+        d = orc.make_desc(r["app"], r["lattice"], *r["shape"], r["nspins"], r["kT"])
+        spin, seeds = orc.script_init(d, r["seed"], 1, r["nspins"])
+        assert str(checksum(spin, seeds)) == r["checksum"][0]
+        nacc, eng = orc.sweeps(d, spin, seeds, 3)
+        assert [int(x) for x in nacc] == r["naccept"][:3]
+        assert [int(e) for e in eng] == r["energy"][:3]

@@ -296,6 +296,40 @@ def test_full_size_first_sweeps_bit_exact(orc, case):
     assert np.array_equal(gs1, spin1) and np.array_equal(gk1, seeds1)
 
 
+with open(os.path.join(GOLD, "full_runs.json")) as _f:
+    FULL_RUNS = {r["name"]: r for r in json.load(_f)["runs"]}
+
+
+@pytest.mark.parametrize("name,mode", [("ising-sq4n-4096", "bits"), ("ising-sq4n-4096", "bytes"),
+                                       ("ising-sq4n-4096", "fused"), ("potts-sc26n-256", "default")],
+                         ids=["ising-sq4n-4096-bits", "ising-sq4n-4096-bytes",
+                              "ising-sq4n-4096-fused", "potts-sc26n-256"])
+def test_full_length_runs_match_oracle_fixture(name, mode):
+    """The whole 1000-sweep (4096^2 Ising) and 200-sweep (256^3 Potts) runs:
+    Naccept and energy after every sweep and the state checksum every
+    `every` sweeps equal the oracle's, stored in tests/golden/full_runs.json."""
+    import spparks_amd as S
+    r = FULL_RUNS[name]
+    say("full-length run", name, mode)
+    with gpu_init(S, r["app"], r["lattice"], tuple(r["shape"]), r["nspins"], r["kT"], r["seed"]) as L:
+        if mode == "bytes":
+            L.set_bitplanes(False)
+        if mode == "fused":
+            L.set_fusion(True)
+        assert str(L.checksum()) == r["checksum"][0]
+        base, nacc, eng = 0, [], []
+        for i in range(r["sweeps"] // r["every"]):
+            tot, gn, ge = L.sweeps(r["every"])
+            nacc += [int(base + x) for x in gn]
+            eng += [int(e) for e in ge]
+            base += tot
+            assert str(L.checksum()) == r["checksum"][i + 1], "state after %d sweeps" % ((i + 1) * r["every"])
+        if mode == "fused":
+            assert L.fusion
+    assert nacc == r["naccept"]
+    assert eng == r["energy"]
+
+
 BIT_CASES = [
     # two-state sq/4n lattices, which sweep on bit planes by default:
     # sub-grid rows of 1, 9, 32, 33, 63 and 65 sites, Potts Q=2, T = 0
