@@ -125,15 +125,21 @@ def checksum(spin, seed):
     """spk_hip_checksum of host arrays in reference site order: the wrapping
     sum over sites g of mix64(g*C1 + spin*C2 + seed), mix64 = splitmix64's
     finaliser."""
-    g = np.arange(len(spin), dtype=np.uint64)
+    spin, seed = np.asarray(spin), np.asarray(seed)
+    total = np.uint64(0)
+    step = 1 << 24   # bounded temporaries for lattices of up to 2^31 sites
     with np.errstate(over="ignore"):
-        z = (g * np.uint64(0xD1B54A32D192ED03)
-             + np.asarray(spin).astype(np.uint32).astype(np.uint64) * np.uint64(0xABC98388FB8FAC03)
-             + np.asarray(seed).astype(np.uint32).astype(np.uint64))
-        z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
-        z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
-        z = z ^ (z >> np.uint64(31))
-        return int(z.sum(dtype=np.uint64))
+        for lo in range(0, len(spin), step):
+            hi = min(lo + step, len(spin))
+            g = np.arange(lo, hi, dtype=np.uint64)
+            z = (g * np.uint64(0xD1B54A32D192ED03)
+                 + spin[lo:hi].astype(np.uint32).astype(np.uint64) * np.uint64(0xABC98388FB8FAC03)
+                 + seed[lo:hi].astype(np.uint32).astype(np.uint64))
+            z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+            z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+            z = z ^ (z >> np.uint64(31))
+            total += z.sum(dtype=np.uint64)
+    return int(total)
 
 
 def master_draws(seed, n):

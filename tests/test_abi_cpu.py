@@ -72,6 +72,22 @@ def test_ncolors_and_validation():
         S.ncolors("potts", "sc/26n", 8, 8, 3)
 
 
+def test_size_limits():
+    """At most 2^31-1 sites (the reference's int site IDs,
+    src/create_sites.cpp:317) and 2^31-1 colour-blocked slots; 2^30-site
+    lattices, which need 64-bit offsets, are accepted."""
+    import spparks_amd as S
+    assert S.ncolors("ising", "sq/4n", 32768, 32768) == 2
+    assert S.ncolors("potts", "sc/26n", 1024, 1024, 1024) == 8
+    assert S.ncolors("ising", "sq/4n", 46336, 46340) == 2
+    with pytest.raises(S.HipError, match="2\\^31-1 sites"):
+        S.ncolors("potts", "sc/26n", 2048, 1024, 1024)
+    with pytest.raises(S.HipError, match="2\\^31-1 slots"):
+        S.ncolors("potts", "sc/26n", 1290, 1290, 1290)
+    with pytest.raises(S.HipError, match="2\\^31-1 slots"):
+        S.ncolors("ising", "sq/4n", 46340, 46340)
+
+
 def test_table_rejects_bad_temperature():
     import spparks_amd as S
     with pytest.raises(S.HipError):

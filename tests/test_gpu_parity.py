@@ -298,19 +298,16 @@ def test_full_size_first_sweeps_bit_exact(orc, case):
 
 with open(os.path.join(GOLD, "full_runs.json")) as _f:
     FULL_RUNS = {r["name"]: r for r in json.load(_f)["runs"]}
+with open(os.path.join(GOLD, "max_size_runs.json")) as _f:
+    MAX_RUNS = {r["name"]: r for r in json.load(_f)["runs"]}
 
 
-@pytest.mark.parametrize("name,mode", [("ising-sq4n-4096", "bits"), ("ising-sq4n-4096", "bytes"),
-                                       ("ising-sq4n-4096", "fused"), ("potts-sc26n-256", "default")],
-                         ids=["ising-sq4n-4096-bits", "ising-sq4n-4096-bytes",
-                              "ising-sq4n-4096-fused", "potts-sc26n-256"])
-def test_full_length_runs_match_oracle_fixture(name, mode):
-    """The whole 1000-sweep (4096^2 Ising) and 200-sweep (256^3 Potts) runs:
-    Naccept and energy after every sweep and the state checksum every
-    `every` sweeps equal the oracle's, stored in tests/golden/full_runs.json."""
+def run_against_fixture(r, mode):
+    """Runs fixture entry r on the GPU (mode: bits, bytes, fused or default)
+    and compares Naccept and energy after every sweep and the state checksum
+    every r["every"] sweeps."""
     import spparks_amd as S
-    r = FULL_RUNS[name]
-    say("full-length run", name, mode)
+    say("fixture run", r["name"], mode)
     with gpu_init(S, r["app"], r["lattice"], tuple(r["shape"]), r["nspins"], r["kT"], r["seed"]) as L:
         if mode == "bytes":
             L.set_bitplanes(False)
@@ -328,6 +325,29 @@ def test_full_length_runs_match_oracle_fixture(name, mode):
             assert L.fusion
     assert nacc == r["naccept"]
     assert eng == r["energy"]
+
+
+@pytest.mark.parametrize("name,mode", [("ising-sq4n-4096", "bits"), ("ising-sq4n-4096", "bytes"),
+                                       ("ising-sq4n-4096", "fused"), ("potts-sc26n-256", "default")],
+                         ids=["ising-sq4n-4096-bits", "ising-sq4n-4096-bytes",
+                              "ising-sq4n-4096-fused", "potts-sc26n-256"])
+def test_full_length_runs_match_oracle_fixture(name, mode):
+    """The whole 1000-sweep (4096^2 Ising) and 200-sweep (256^3 Potts) runs:
+    Naccept and energy after every sweep and the state checksum every
+    `every` sweeps equal the oracle's, stored in tests/golden/full_runs.json."""
+    run_against_fixture(FULL_RUNS[name], mode)
+
+
+@pytest.mark.parametrize("name,mode", [("ising-sq4n-32768", "bits"), ("ising-sq4n-32768", "bytes"),
+                                       ("potts-sc26n-1024", "default")],
+                         ids=["ising-sq4n-32768-bits", "ising-sq4n-32768-bytes", "potts-sc26n-1024"])
+def test_maximum_size_lattices_match_oracle_fixture(name, mode):
+    """Lattices of 2^30 sites (tests/golden/max_size_runs.json): the seed
+    array passes 2^32 bytes, so the per-colour kernels run with 64-bit
+    offsets, and the bit planes hold 2^29-site colours."""
+    r = MAX_RUNS[name]
+    assert r["shape"][0] * r["shape"][1] * r["shape"][2] * 4 >= 1 << 32
+    run_against_fixture(r, mode)
 
 
 BIT_CASES = [
