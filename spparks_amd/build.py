@@ -98,6 +98,18 @@ def build_emulator():
     return out
 
 
+def build_abi_consumer():
+    """tests/abi/consumer: a plain C99 caller of libspk_hip.so and
+    libspparks_hip.so through their headers.  Test infrastructure."""
+    src = os.path.join(ROOT, "tests", "abi", "consumer.c")
+    if not os.path.exists(src):
+        return None
+    out = os.path.join(ROOT, "tests", "abi", "consumer")
+    run(["gcc", "-std=c99", "-pedantic", "-Wall", "-Wextra", "-Werror", "-O1", "-I" + INCLUDE, "-o", out, src,
+         "-L" + LIBDIR, "-lspparks_hip", "-lspk_hip", "-Wl,-rpath," + LIBDIR])
+    return out
+
+
 def build():
     build_hip()
     build_front()

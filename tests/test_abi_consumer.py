@@ -1,0 +1,47 @@
+"""A plain C99 program (tests/abi/consumer.c, built by build()) drives both
+libraries through include/spk_hip.h and include/spparks_hip.h alone."""
+import json
+import os
+import subprocess
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+EXE = os.path.join(ROOT, "tests", "abi", "consumer")
+
+
+def consumer(mode, timeout):
+    if not os.path.exists(EXE):
+        raise FileNotFoundError("tests/abi/consumer is not built; run __graft_entry__.build()")
+    print("consumer", mode, flush=True)
+    p = subprocess.run([EXE, mode], capture_output=True, text=True, timeout=timeout)
+    assert p.returncode == 0, p.stdout + p.stderr
+    return [line.split() for line in p.stdout.splitlines()]
+
+
+def test_host_entry_points_from_c():
+    import spparks_amd as S
+    out = consumer("host", 60)
+    assert out[0] == ["ncolors", "8"]
+    assert out[1][:2] == ["odd_edge", "-1"] and "incommensurate" in " ".join(out[1])
+    with open(os.path.join(ROOT, "tests", "golden", "metropolis_table.json")) as f:
+        want = json.load(f)["tables"]["2.269"]
+    thr = [l for l in out if l[0] == "thr"]
+    assert [int(l[2]) for l in thr] == want["thr"][:5]
+    assert [float.fromhex(l[3]) for l in thr[1:]] == [float.fromhex(h) for h in want["exp_bits"][1:5]]
+    draws = [float.fromhex(l[2]) for l in out if l[0] == "draw"]
+    assert draws == list(S.master_draws(12345, 5))
+
+
+@pytest.mark.gpu
+def test_batch_and_library_from_c():
+    """Two replicas in a batch equal the same scripts run alone (checksum,
+    Naccept, energy); the reference's library calls run the Potts Q=100
+    128x128 script to its recorded energy after 100 sweeps."""
+    out = consumer("gpu", 120)
+    reps = [l for l in out if l[0] == "replica"]
+    assert len(reps) == 2
+    for l in reps:
+        assert l[3:6] == l[7:10], l
+    lib = [l for l in out if l[0] == "library"][0]
+    assert lib[1:] == ["nglobal", "16384", "energy", "97818"]
