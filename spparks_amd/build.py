@@ -18,8 +18,8 @@ LIBDIR = os.path.join(HERE, "lib")
 INCLUDE = os.path.join(ROOT, "include")
 
 HIP_SOURCES = ["spk_common.hip", "spk_plain.hip", "spk_batch.hip", "spk_serve.hip"]
-FRONT_SOURCES = ["front/spk_front.cpp"]
-CXX_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-Wall", "-Wextra", "-ffp-contract=off"]
+FRONT_SOURCES = ["front/spk_front.cpp", "front/spk_dump_writer.cpp"]
+CXX_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-pthread", "-Wall", "-Wextra", "-ffp-contract=off"]
 
 HIPCC_FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++20", "-fPIC",

@@ -78,6 +78,7 @@ Spparks::Spparks(FILE *screen, FILE *logfile) : screen_(screen), logfile_(logfil
 }
 
 Spparks::~Spparks() {
+  (void)writer_.drain();
   for (auto &d : dumps_)
     if (d.fp) std::fclose(d.fp);
   if (h_) spk_hip_destroy(h_);
@@ -511,6 +512,7 @@ void Spparks::cmd_dump(const std::vector<std::string> &a) {
     if (d.fields.size() < 2) throw Error("Dump sites requires two or more fields");
   }
   if (!d.multifile) {
+    drain_dumps();   // the file may be one a queued snapshot still writes
     d.fp = std::fopen(d.file.c_str(), "w");
     if (!d.fp) throw Error("Cannot open dump file");
   }
@@ -519,6 +521,7 @@ void Spparks::cmd_dump(const std::vector<std::string> &a) {
 
 void Spparks::cmd_undump(const std::vector<std::string> &a) {
   if (a.size() != 1) throw Error("Illegal undump command");
+  drain_dumps();
   for (size_t i = 0; i < dumps_.size(); i++)
     if (dumps_[i].id == a[0]) {
       if (dumps_[i].fp) std::fclose(dumps_[i].fp);
@@ -632,69 +635,76 @@ void Spparks::download() {
 // src/dump_text.cpp:380-392; DumpSites::write_header,
 // src/dump_sites.cpp:52-66), then one line per site in id order, each value
 // followed by a space (DumpText::write_text, src/dump_text.cpp:405-421).
+// The state is copied from the device now; formatting and writing happen on
+// the dump writer's thread while the run goes on.
 void Spparks::write_dump(Dump &d) {
-  download();
+  DumpJob job;
+  bool seeds = false;
+  for (const auto &f : d.fields) {
+    DumpField c = DumpField::Site;
+    if (f == "id") c = DumpField::Id;
+    else if (f == "seed") c = DumpField::Seed;
+    else if (f == "x") c = DumpField::X;
+    else if (f == "y") c = DumpField::Y;
+    else if (f == "z") c = DumpField::Z;
+    else if (f == "energy") c = DumpField::Energy;
+    seeds |= c == DumpField::Seed;
+    job.fields.push_back(c);
+  }
+  job.spin.resize(nsites_);
+  if (seeds) job.seed.resize(nsites_);
+  hip(spk_hip_download(h_, job.spin.data(), seeds ? job.seed.data() : nullptr));
   if (d.multifile) {
     const size_t star = d.file.find('*');
     const std::string name = d.file.substr(0, star) + std::to_string(d.idump) + d.file.substr(star + 1);
-    d.fp = std::fopen(name.c_str(), "w");
-    if (!d.fp) throw Error("Cannot open dump file");
+    job.fp = std::fopen(name.c_str(), "w");
+    if (!job.fp) throw Error("Cannot open dump file");
+    job.close = true;
+  } else {
+    job.fp = d.fp;
   }
   std::string cols;   // "site" is written as "type" in text dumps, as is in site files
   for (size_t i = 0; i < d.fields.size(); i++)
     cols += (i ? " " : "") + ((d.fields[i] == "site" && d.style == "text") ? std::string("type") : d.fields[i]);
+  auto add = [&](const char *fmt, auto... v) {
+    char line[512];
+    std::snprintf(line, sizeof line, fmt, v...);
+    job.header += line;
+  };
   if (d.style == "text") {
-    std::fprintf(d.fp, "ITEM: TIMESTEP\n%d %10g\n", d.idump, time_);
-    std::fprintf(d.fp, "ITEM: NUMBER OF ATOMS\n%lld\n", (long long)nsites_);
-    std::fprintf(d.fp, "ITEM: BOX BOUNDS\n%g %g\n%g %g\n%g %g\n", boxlo_[0], boxhi_[0], boxlo_[1], boxhi_[1],
-                 boxlo_[2], boxhi_[2]);
-    std::fprintf(d.fp, "ITEM: ATOMS %s\n", cols.c_str());
+    add("ITEM: TIMESTEP\n%d %10g\n", d.idump, time_);
+    add("ITEM: NUMBER OF ATOMS\n%lld\n", (long long)nsites_);
+    add("ITEM: BOX BOUNDS\n%g %g\n%g %g\n%g %g\n", boxlo_[0], boxhi_[0], boxlo_[1], boxhi_[1], boxlo_[2],
+        boxhi_[2]);
+    job.header += "ITEM: ATOMS " + cols + "\n";
   } else {
-    std::fprintf(d.fp, "Site file written by dump sites %s command at time: %d %g\n\n", d.id.c_str(), d.idump,
-                 time_);
-    std::fprintf(d.fp, "%d dimension\n", dimension_);
-    std::fprintf(d.fp, "%lld sites\n", (long long)nsites_);
-    std::fprintf(d.fp, "%s values\n", cols.c_str());
-    std::fprintf(d.fp, "%g %g xlo xhi\n", boxlo_[0], boxhi_[0]);
-    std::fprintf(d.fp, "%g %g ylo yhi\n", boxlo_[1], boxhi_[1]);
-    std::fprintf(d.fp, "%g %g zlo zhi\n", boxlo_[2], boxhi_[2]);
-    std::fprintf(d.fp, "\nValues\n\n");
+    job.header += "Site file written by dump sites " + d.id + " command at time: ";
+    add("%d %g\n\n", d.idump, time_);
+    add("%d dimension\n", dimension_);
+    add("%lld sites\n", (long long)nsites_);
+    job.header += cols + " values\n";
+    add("%g %g xlo xhi\n", boxlo_[0], boxhi_[0]);
+    add("%g %g ylo yhi\n", boxlo_[1], boxhi_[1]);
+    add("%g %g zlo zhi\n", boxlo_[2], boxhi_[2]);
+    job.header += "\nValues\n\n";
   }
-  for (int64_t g = 0; g < nsites_; g++) {
-    const int64_t i = g % nx_, j = (g / nx_) % ny_, k = g / ((int64_t)nx_ * ny_);
-    for (const auto &f : d.fields) {
-      if (f == "id") std::fprintf(d.fp, "%lld ", (long long)(g + 1));
-      else if (f == "site" || f == "i1") std::fprintf(d.fp, "%d ", host_spin_[g]);
-      else if (f == "seed") std::fprintf(d.fp, "%d ", host_seed_[g]);
-      else if (f == "x") std::fprintf(d.fp, "%g ", i * latconst_ + boxlo_[0]);
-      else if (f == "y") std::fprintf(d.fp, "%g ", j * latconst_ + boxlo_[1]);
-      else if (f == "z") std::fprintf(d.fp, "%g ", dimension_ == 3 ? k * latconst_ + boxlo_[2] : 0.5 * (boxlo_[2] + boxhi_[2]));
-      else if (f == "energy") {
-        // site_energy: #unlike neighbours of the periodic shell
-        int e = 0;
-        for (int dk = -1; dk <= 1; dk++) {
-          if (dimension_ == 2 && dk) continue;
-          for (int dj = -1; dj <= 1; dj++)
-            for (int di = -1; di <= 1; di++) {
-              const int nb = std::abs(di) + std::abs(dj) + std::abs(dk);
-              if (nb == 0) continue;
-              if ((lattice_ == "sq/4n" || lattice_ == "sc/6n") && nb != 1) continue;
-              const int64_t a = (i + di + nx_) % nx_, b = (j + dj + ny_) % ny_, c = (k + dk + nz_) % nz_;
-              e += host_spin_[a + (int64_t)nx_ * (b + (int64_t)ny_ * c)] != host_spin_[g];
-            }
-        }
-        std::fprintf(d.fp, "%g ", (double)e);
-      }
-    }
-    std::fprintf(d.fp, "\n");
+  job.nx = nx_;
+  job.ny = ny_;
+  job.nz = nz_;
+  job.dimension = dimension_;
+  job.nearest_only = lattice_ == "sq/4n" || lattice_ == "sc/6n";
+  job.latconst = latconst_;
+  for (int m = 0; m < 3; m++) {
+    job.boxlo[m] = boxlo_[m];
+    job.boxhi[m] = boxhi_[m];
   }
-  if (d.multifile) {
-    std::fclose(d.fp);
-    d.fp = nullptr;
-  } else {
-    std::fflush(d.fp);
-  }
+  writer_.submit(std::move(job));
   d.idump++;
+}
+
+// Waits for every snapshot handed to the dump writer.
+void Spparks::drain_dumps() {
+  if (!writer_.drain()) throw Error("Error writing dump file");
 }
 
 // read_sites file (src/read_sites.cpp:69-186): the free-format header
@@ -709,6 +719,7 @@ void Spparks::cmd_read_sites(const std::vector<std::string> &a) {
   if (app_.empty()) throw Error("Read_sites command before app_style set");
   if (a.size() != 1) throw Error("Illegal read_sites command");
   message("Reading site file ...\n");
+  drain_dumps();   // the file may be a snapshot still being written
   FILE *fp = std::fopen(a[0].c_str(), "r");
   if (!fp) throw Error("Cannot open file " + a[0]);
   struct Closer {
@@ -926,6 +937,11 @@ void Spparks::cmd_run(const std::vector<std::string> &a) {
         t_output_ += wall() - t1;
       }
     }
+  }
+  {
+    const double t1 = wall();
+    drain_dumps();
+    t_output_ += wall() - t1;
   }
   t_loop_ = wall() - loop_t0_;
   // Finish (src/finish.cpp:30-120)

@@ -17,6 +17,7 @@
 #include <string>
 #include <vector>
 
+#include "spk_dump_writer.h"
 #include "spk_hip.h"
 #include "spk_master_rng.h"
 
@@ -100,6 +101,7 @@ class Spparks {
   void stats_line(bool timeflag);
   void stats_header();
   void write_dump(Dump &d);
+  void drain_dumps();
   double next_time(double t, int logfreq, double delta, int nrepeat, double scale, double delay) const;
 
   // helpers
@@ -153,6 +155,7 @@ class Spparks {
   double t_solve_ = 0.0, t_output_ = 0.0, t_loop_ = 0.0;
 
   std::vector<int32_t> host_spin_, host_seed_;
+  DumpWriter writer_;   // snapshots being written; drained at the end of a run
 };
 
 }  // namespace spkfront

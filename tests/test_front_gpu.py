@@ -152,6 +152,22 @@ def test_dump_text_snapshots(orc, tmp_path):
         assert np.array_equal(vals[:, 4], (spin[nb] != spin[:, None]).sum(1))
 
 
+def test_dump_write_error_stops_the_run(tmp_path):
+    """Snapshots are written on a background thread; a failed write (a
+    full device) still ends the run with an error once they are drained."""
+    if not os.path.exists("/dev/full"):
+        pytest.skip("no /dev/full")
+    p = tmp_path / "in.full"
+    p.write_text("seed 7\napp_style ising\ndimension 2\nlattice sq/4n 1.0\n"
+                 "region box block 0 64 0 64 -0.5 0.5\ncreate_box box\ncreate_sites box\n"
+                 "set site range 1 2\nsweep color/strict\ntemperature 2.0\n"
+                 "dump 1 text 1.0 /dev/full id site\nrun 3.0\nprint after\n")
+    r = subprocess.run([EXE, "-in", str(p), "-log", "none"], capture_output=True, text=True, cwd=tmp_path,
+                       timeout=60)
+    assert r.returncode != 0 and "Error writing dump file" in r.stdout + r.stderr
+    assert "after" not in r.stdout
+
+
 def test_invalid_values_error(tmp_path):
     for body, msg in (("", "invalid values"), ("set site value 11\n", "invalid values")):
         p = tmp_path / "in.bad"
