@@ -57,6 +57,12 @@ extern "C" int spk_hip_serve(const spk_hip_request *req, int n, int64_t pool_byt
         }
         break;             // wait for retirements
       }
+      if (rc == SPK_HIP_EINVAL) {   // not a valid solve (odd edge, bad Q, ...): refused alone
+        out[r] = spk_hip_result{t, t, 0, 0.0, 0, SPK_HIP_EINVAL};
+        completed++;
+        waiting.pop_front();
+        continue;
+      }
       if (rc) return rc;
       owner[id] = r;
       out[r] = spk_hip_result{now_s(t0), 0.0, 0, 0.0, 0, 0};
