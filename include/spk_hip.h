@@ -268,8 +268,8 @@ int spk_hip_batch_checksum(spk_hip_batch *b, int id, uint64_t *sum);
    monotonic clock from the start of the call; arrivals in the future are
    waited for.  A request larger than the whole pool is refused
    (SPK_HIP_ENOMEM), and so is a request that is not a valid solve, such as
-   an odd edge or 2^30 sites or more (SPK_HIP_EINVAL); the other requests
-   are served as usual.  No counterpart in the reference.
+   an odd edge or 2^30 sites or more (SPK_HIP_EINVAL) or a set range
+   outside 1..Q (SPK_HIP_EVALUE); the other requests are served as usual.  No counterpart in the reference.
    ====================================================================== */
 
 typedef struct spk_hip_request {
@@ -283,8 +283,8 @@ typedef struct spk_hip_result {
   int64_t naccept;
   double energy;
   uint64_t checksum;              /* spk_hip_checksum of the final state  */
-  int status;                     /* 0 done; SPK_HIP_ENOMEM or _EINVAL:
-                                     refused (see spk_hip_serve)          */
+  int status;                     /* 0 done; SPK_HIP_ENOMEM, _EINVAL or
+                                     _EVALUE: refused (see spk_hip_serve) */
 } spk_hip_result;
 
 typedef struct spk_hip_serve_stats {

@@ -171,14 +171,16 @@ class Batch:
 
 def serve(requests, pool_bytes, max_live, block_sweeps=10, block=0, grid=0):
     """Continuous batching of `requests` (dicts with app, lattice, shape,
-    nspins, kT, seed, sweeps, arrive; sorted by arrive) through
-    spk_hip_serve.  Returns (results as dicts, stats dict)."""
+    nspins, kT, seed, sweeps, arrive, and optionally lo, hi for `set site
+    range lo hi`, default 1 Q; sorted by arrive) through spk_hip_serve.
+    Returns (results as dicts, stats dict)."""
     L = _bind()
     n = len(requests)
     arr = (Request * max(n, 1))()
     for i, r in enumerate(requests):
         arr[i].replica = replica_desc(r["app"], r["lattice"], *r["shape"], nspins=r["nspins"],
-                                      temperature=r["kT"], seed=r["seed"], nsweeps=r["sweeps"])
+                                      temperature=r["kT"], seed=r["seed"], lo=r.get("lo", 1), hi=r.get("hi"),
+                                      nsweeps=r["sweeps"])
         arr[i].arrive = float(r["arrive"])
     out = (Result * max(n, 1))()
     st = ServeStats()

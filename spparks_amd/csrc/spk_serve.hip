@@ -57,8 +57,8 @@ extern "C" int spk_hip_serve(const spk_hip_request *req, int n, int64_t pool_byt
         }
         break;             // wait for retirements
       }
-      if (rc == SPK_HIP_EINVAL) {   // not a valid solve (odd edge, bad Q, ...): refused alone
-        out[r] = spk_hip_result{t, t, 0, 0.0, 0, SPK_HIP_EINVAL};
+      if (rc == SPK_HIP_EINVAL || rc == SPK_HIP_EVALUE) {   // not a valid solve: refused alone
+        out[r] = spk_hip_result{t, t, 0, 0.0, 0, rc};
         completed++;
         waiting.pop_front();
         continue;

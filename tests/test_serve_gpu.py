@@ -75,19 +75,22 @@ def test_stream_results_equal_solo_runs(orc):
 
 
 def test_invalid_request_is_refused_alone():
-    """A request that is not a valid solve (an odd edge) is refused with
-    SPK_HIP_EINVAL; the requests around it are served as usual."""
+    """Requests that are not valid solves (an odd edge: SPK_HIP_EINVAL; a
+    set range beyond Q: SPK_HIP_EVALUE) are refused alone; the requests
+    around them are served as usual."""
     import spparks_amd as S
     from spparks_amd import batch as bmod
-    reqs = small_stream(4, rate=1000.0, seed=8)
-    bad = dict(reqs[1], id=1, shape=[9, 8, 8])
-    stream = [reqs[0], bad] + [dict(r, id=i + 2) for i, r in enumerate(reqs[2:])]
-    say("serving with one invalid request")
+    reqs = small_stream(5, rate=1000.0, seed=8)
+    odd = dict(reqs[1], shape=[9, 8, 8])
+    beyond_q = dict(reqs[3], hi=reqs[3]["nspins"] + 1)   # set site range 1 Q+1
+    stream = [reqs[0], odd, reqs[2], beyond_q, reqs[4]]
+    bad = (odd, beyond_q)
+    say("serving with two invalid requests")
     res, st = bmod.serve(stream, pool_bytes=64 << 20, max_live=8, block_sweeps=4, block=128, grid=64)
-    assert [o["status"] for o in res] == [0, -1, 0, 0]
-    assert st["site_updates"] == sum(np.prod(r["shape"]) * r["sweeps"] for r in stream if r is not bad)
+    assert [o["status"] for o in res] == [0, -1, 0, -5, 0]
+    assert st["site_updates"] == sum(np.prod(r["shape"]) * r["sweeps"] for r in stream if r not in bad)
     for r, o in zip(stream, res):
-        if r is not bad:
+        if r not in bad:
             cs, nacc, e, _ = solo(S, r)
             assert (o["checksum"], o["naccept"], o["energy"]) == (cs, nacc, e)
 
