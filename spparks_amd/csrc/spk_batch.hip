@@ -431,6 +431,7 @@ int ensure_events(spk_hip_batch *b, size_t n) {
 extern "C" {
 
 int spk_hip_replica_bytes(const spk_hip_desc *d, int64_t *bytes) {
+  if (!bytes) return set_error(SPK_HIP_EINVAL, "null argument");
   Geom g;
   int rc = make_geom(d, &g);
   if (rc) return rc;

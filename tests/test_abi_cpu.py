@@ -96,10 +96,55 @@ def test_table_rejects_bad_temperature():
         S.metropolis_table(-1.0, 8)
 
 
-def test_launch_settings_reject_null_handles():
-    """The launch-setting calls check their handle before touching the GPU."""
-    from spparks_amd.hip import load_library
-    L = load_library()
-    for fn, arg in (("spk_hip_set_bitplanes", 1), ("spk_hip_set_resident", 1), ("spk_hip_set_launch", 256),
-                    ("spk_hip_set_graph", 32), ("spk_hip_set_fusion", 1)):
-        assert getattr(L, fn)(None, arg) == -1, fn   # SPK_HIP_EINVAL
+def test_every_entry_point_rejects_null_arguments():
+    """Every call that takes a handle returns SPK_HIP_EINVAL for a null one
+    (or a null output pointer) before any device work; the destroy calls
+    accept null.  Runs without a GPU, through a binding with no argtypes."""
+    from spparks_amd import hip
+    L = C.CDLL(hip.LIB_PATH)
+    L.spk_hip_last_error.restype = C.c_char_p
+    null = C.c_void_p(None)
+    i64, i32, dbl, u64 = C.c_int64(), C.c_int(), C.c_double(), C.c_uint64()
+    desc = hip.Desc(0, 0, 8, 8, 1, 3, 1.0)
+    calls = {
+        "spk_hip_create": (C.byref(desc), null),
+        "spk_hip_set_temperature": (null, C.c_double(1.0)),
+        "spk_hip_upload": (null, null, null),
+        "spk_hip_download": (null, null, null),
+        "spk_hip_set_range": (null, 1, 1, 2),
+        "spk_hip_set_value": (null, 1),
+        "spk_hip_init_seeds": (null, C.c_double(0.5)),
+        "spk_hip_sweeps": (null, 1, null, null, null),
+        "spk_hip_energy": (null, C.byref(dbl)),
+        "spk_hip_checksum": (null, C.byref(u64)),
+        "spk_hip_sync": (null,),
+        "spk_hip_set_launch": (null, 256),
+        "spk_hip_set_resident": (null, 1),
+        "spk_hip_set_bitplanes": (null, 1),
+        "spk_hip_set_fusion": (null, 1),
+        "spk_hip_get_fusion": (null, C.byref(i32)),
+        "spk_hip_set_graph": (null, 32),
+        "spk_hip_last_timing": (null, C.byref(dbl), C.byref(i64)),
+        "spk_hip_batch_create": (C.c_int64(1 << 20), 4, null),
+        "spk_hip_replica_bytes": (C.byref(desc), null),
+        "spk_hip_batch_add": (null, null, C.byref(i32)),
+        "spk_hip_batch_remove": (null, 0),
+        "spk_hip_batch_upload": (null, 0, null, null),
+        "spk_hip_batch_download": (null, 0, null, null),
+        "spk_hip_batch_run": (null, 1, 1),
+        "spk_hip_batch_status": (null, 0, null, null, null, null),
+        "spk_hip_batch_trace": (null, 0, 0, null, null),
+        "spk_hip_batch_set_launch": (null, 0, 0),
+        "spk_hip_batch_last_timing": (null, null, null, null, null, null),
+        "spk_hip_batch_pool": (null, null, null, null),
+        "spk_hip_batch_checksum": (null, 0, C.byref(u64)),
+        "spk_hip_serve": (null, 0, C.c_int64(1 << 20), 4, 4, 0, 0, null, null),
+    }
+    host_only = {"spk_hip_last_error", "spk_hip_ncolors", "spk_hip_metropolis_table", "spk_hip_master_draws",
+                 "spk_hip_destroy", "spk_hip_batch_destroy"}
+    assert set(calls) | host_only == set(header_symbols())
+    for name, args in calls.items():
+        assert getattr(L, name)(*args) == -1, name   # SPK_HIP_EINVAL
+        assert L.spk_hip_last_error(), name          # with a message
+    L.spk_hip_destroy(null)
+    L.spk_hip_batch_destroy(null)
