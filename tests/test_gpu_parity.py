@@ -475,6 +475,32 @@ def test_random_lattices_every_path_bit_exact(orc, case):
         assert np.array_equal(gn, nacc) and np.array_equal(ge, eng), (resident, planes)
 
 
+def test_argument_edges(orc):
+    """sweeps(0) is a no-op; a negative sweep count, workgroup sizes that are
+    not powers of two in 64..1024 and graph chunks that are not multiples of
+    8 are rejected; the lattice then runs on exactly as the oracle does."""
+    import spparks_amd as S
+    case = ("potts", "sq/8n", (16, 12, 1), 100, 0.5, 56789)
+    spin0, seeds0, spin1, seeds1, nacc, eng = oracle_run(orc, *case, 3)
+    with gpu_init(S, *case) as L:
+        assert L.sweeps(0) == (0, None, None)
+        gs, gk = L.download()
+        assert np.array_equal(gs, spin0) and np.array_equal(gk, seeds0)
+        with pytest.raises(S.HipError, match="nsweeps"):
+            L.sweeps(-1)
+        for b in (0, 32, 100, 2048):
+            with pytest.raises(S.HipError, match="power of two"):
+                L.set_launch(b)
+        for c in (-8, 7, 12):
+            with pytest.raises(S.HipError, match="chunk_sweeps"):
+                L.set_graph(c)
+        L.set_graph(16)
+        _, gn, ge = L.sweeps(3)
+        gs, gk = L.download()
+    assert np.array_equal(gs, spin1) and np.array_equal(gk, seeds1)
+    assert np.array_equal(gn, nacc) and np.array_equal(ge, eng)
+
+
 def test_bitplane_setting_rejects_bad_values():
     import spparks_amd as S
     with gpu_init(S, "ising", "sq/4n", (64, 48, 1), 2, 2.269, 1) as L:
