@@ -345,7 +345,7 @@ def run(args):
     traffic, traffic_detail = load_traffic(args.workload + (
         "-fused" if fused else "-bytes" if lattice == "sq/4n" and not planes and not resident and q <= 2 else ""))
     if resident:
-        traffic, traffic_detail = None, None   # measured for the per-colour kernels only
+        traffic, traffic_detail = load_traffic(args.workload + "-resident")
     out = {
         "metric": "G site-updates/s & % HBM peak; batched vs plain speedup; p99 solve latency",
         "value": round(value, 4),
