@@ -475,6 +475,40 @@ def test_random_lattices_every_path_bit_exact(orc, case):
         assert np.array_equal(gn, nacc) and np.array_equal(ge, eng), (resident, planes)
 
 
+def test_two_handles_from_two_threads():
+    """Distinct handles driven from distinct threads at once (the library
+    calls release the GIL) give the same states as the same runs one after
+    the other; each thread's error text is its own."""
+    import threading
+    import spparks_amd as S
+    cases = [("ising", "sq/4n", (512, 384, 1), 2, 2.269, 21, 60),
+             ("potts", "sc/26n", (32, 24, 16), 5000, 0.6, 22, 25)]
+
+    def run(case, out, k):
+        app, lat, shape, q, kt, seed, n = case
+        with gpu_init(S, app, lat, shape, q, kt, seed) as L:
+            for _ in range(3):
+                L.sweeps(n // 3, per_sweep=False)
+            try:
+                L.set_launch(100)
+            except S.HipError as e:
+                out[k] = (L.checksum(), str(e))
+
+    alone = [None, None]
+    for k, c in enumerate(cases):
+        run(c, alone, k)
+    both = [None, None]
+    say("two threads")
+    ts = [threading.Thread(target=run, args=(c, both, k)) for k, c in enumerate(cases)]
+    for t in ts:
+        t.start()
+    for t in ts:
+        t.join(timeout=100)
+    assert not any(t.is_alive() for t in ts)
+    assert both == alone
+    assert all("power of two" in e for _, e in both)
+
+
 def test_argument_edges(orc):
     """sweeps(0) is a no-op; a negative sweep count, workgroup sizes that are
     not powers of two in 64..1024 and graph chunks that are not multiples of
