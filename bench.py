@@ -33,7 +33,15 @@ WORKLOADS = {
                 "Ising sq/4n 8192x8192 periodic, kT=2.269, seed 12345 (four times the 4096x4096 lattice)"),
     "potts-sc26n-256": ("potts", "sc/26n", (256, 256, 256), 10000, 0.5, 24680, 200,
              "Potts Q=10000 sc/26n 256^3 periodic, kT=0.5, mask yes, seed 24680"),
+    # the strict-capable proposal variants on the same lattices
+    "potts-neigh-sc26n-256": ("potts/neigh", "sc/26n", (256, 256, 256), 10000, 0.5, 24680, 50,
+             "potts/neigh Q=10000 sc/26n 256^3 periodic, kT=0.5, seed 24680"),
+    "potts-neighonly-sc26n-256": ("potts/neighonly", "sc/26n", (256, 256, 256), 10000, 0.5, 24680, 50,
+             "potts/neighonly Q=10000 sc/26n 256^3 periodic, kT=0.5, seed 24680"),
+    "ising-single-sq4n-4096": ("ising/single", "sq/4n", (4096, 4096, 1), 2, 2.269, 12345, 200,
+             "ising/single sq/4n 4096x4096 periodic, kT=2.269, seed 12345"),
 }
+VARIANTS = ("potts/neigh", "potts/neighonly", "ising/single")
 
 NCOLORS = {"sq/4n": 2, "sq/8n": 4, "sc/6n": 2, "sc/26n": 8}
 
@@ -41,7 +49,7 @@ NCOLORS = {"sq/4n": 2, "sq/8n": 4, "sc/6n": 2, "sc/26n": 8}
 def spin_bytes(app, q):
     """Device storage of one spin: the narrowest of uint8/uint16/int32
     holding 1..Q."""
-    qmax = q if app == "potts" else 2
+    qmax = q if app.startswith("potts") else 2
     return 1 if qmax <= 255 else 2 if qmax <= 65535 else 4
 
 
@@ -70,7 +78,9 @@ def cpu_baseline(app, lattice, shape, q, kt, seed, budget_s, threads):
     whole-lattice sweeps of the same workload until budget_s has elapsed."""
     from oracle import oracle as O
     d = O.make_desc(app, lattice, *shape, q, kt)
+    t0 = time.perf_counter()
     spin, seeds = O.script_init(d, seed, 1, q)
+    setup_s = time.perf_counter() - t0
     n = shape[0] * shape[1] * shape[2]
     done, t0 = 0, time.perf_counter()
     while True:
@@ -83,7 +93,8 @@ def cpu_baseline(app, lattice, shape, q, kt, seed, budget_s, threads):
             "cores": threads, "kind": "port",
             "sample": "%d whole-lattice strict sweeps of the same workload from the same initial "
                       "state, oracle/spk_oracle.c (OpenMP over the sites of one colour) in %.1f s"
-                      % (done, el)}
+                      % (done, el),
+            "setup_s": round(setup_s, 4)}
 
 
 def _hash(a, b):
@@ -279,8 +290,12 @@ def run(args):
     K = NCOLORS[lattice]
     with S.Lattice(app, lattice, *shape, nspins=q, temperature=kt) as L:
         s0, rseed = S.strict_init_params(seed)
+        L.sync()
+        t_setup = time.perf_counter()
         L.set_range(s0, 1, q)
         L.init_seeds(rseed)
+        L.sync()
+        t_setup = time.perf_counter() - t_setup
         if args.block:
             L.set_launch(args.block)
         if args.no_resident:
@@ -312,9 +327,16 @@ def run(args):
     # the whole state in one workgroup's LDS: one launch per call (fused
     # launches are also fewer than sweeps, but report their sweeps per launch)
     resident = not fused and nlaunch < steps
-    planes = (lattice == "sq/4n" and (app == "ising" or q == 2) and not fused and not args.no_bitplanes
-              and not resident)
-    if resident:
+    planes = (lattice == "sq/4n" and (app in ("ising", "ising/single") or q == 2) and not fused
+              and not args.no_bitplanes and not resident)
+    variant = app in VARIANTS and not planes
+    if variant:
+        # one site per thread, neighbours gathered in the reference's order;
+        # the same bytes as the per-colour design
+        kernel = "k_sweep_variant<%s, %s, %s>" % (lattice, tname, app)
+        bytes_per_launch = B * n // K
+        B_kernel = B
+    elif resident:
         # all sweeps of the call in one single-workgroup launch: the state
         # crosses HBM once each way; the kernel is bound by the latency of
         # its colour passes in LDS, not by HBM
@@ -371,12 +393,17 @@ def run(args):
                      "bytes_per_launch": bytes_per_launch,
                      "avg_launch_us": round(per_launch_s * 1e6, 3),
                      "bytes_per_update": round(B_kernel, 3)},
-        "pct_hbm_peak": round(100 * value * 1e9 * B / (HBM_PEAK_GBS * 1e9), 2),
+        "pct_hbm_peak": round(100 * value * 1e9 * B_kernel / (HBM_PEAK_GBS * 1e9), 2),
         "spin_bytes": spin_bytes(app, q),
     }
+    # device-side set-up (`set site range` by jump-ahead, strict seeds by
+    # tagreset) against the oracle's sequential restatement of the same
+    out["setup"] = {"device_s": round(t_setup, 6),
+                    "what": "set site range 1 Q and tagreset seeds for every site, on the device"}
     if not args.no_cpu_baseline:
         threads = args.cpu_threads or min(os.cpu_count() or 1, 16)
         out["cpu_baseline"] = cpu_baseline(app, lattice, shape, q, kt, seed, args.cpu_seconds, threads)
+        out["setup"].update(cpu_port_s=out["cpu_baseline"].pop("setup_s"), cpu_threads=threads)
     else:
         out["cpu_baseline"] = None
     return out

@@ -175,7 +175,11 @@ constexpr uint32_t kOrAnd = truth3([](int x, int y, int z) { return (x & y) | z;
 // at T > 0 takes a second draw and is kept when it is below thr[dE]; a
 // change with dE <= 0 is kept without one; T = 0 rejects uphill changes
 // without one.  Returns (changes, their dE sum).
-template <int SPT>
+// SINGLE: AppIsingSingle::site_event_rejection (src/app_ising_single.cpp:
+// 39-77) instead: the proposal is the flip, without a draw; a flip with
+// dE <= 0 is kept, and an uphill one at T > 0 takes the one draw and is
+// kept when it is below thr[dE].
+template <int SPT, bool SINGLE = false>
 __host__ __device__ __forceinline__ Counts bits_thread(uint32_t *__restrict__ bits, int32_t *__restrict__ sseed,
                                                        const Geom &g, const BitGeom &bg, const Rule &r, int c,
                                                        uint32_t t) {
@@ -239,6 +243,16 @@ __host__ __device__ __forceinline__ Counts bits_thread(uint32_t *__restrict__ bi
 #pragma unroll
     for (int v = 0; v < 4; v++) {
       const int i = 4 * j + v;
+      if constexpr (SINGLE) {
+        const int32_t s = (int32_t)sq[j].w[v];
+        const uint32_t upm = bit_mask(upw_q, i);
+        const int32_t thr = (int32_t)bop3<kSel>(bit_mask(s0_q, i), (uint32_t)thr2, (uint32_t)thr4);
+        const int32_t u = pm_step(s);
+        const uint32_t passm = less_mask(u, thr);
+        nq.w[v] = bop3<kSel>(upm & nzt, (uint32_t)u, (uint32_t)s);
+        ch_q = bop3<kOrAnd>(bop3<kKeep>(0xffffffffu, upm, passm), 1u << i, ch_q);
+        continue;
+      }
       const int32_t u1 = pm_step((int32_t)sq[j].w[v]);
       const uint32_t prm = less_mask(half_m1, u1);   // proposal 2 (u1 >= half)
       const uint32_t flipm = prm ^ bit_mask(own_q, i);

@@ -24,13 +24,13 @@ using namespace spk;
 
 // One colour pass of a two-state sq/4n lattice on bit planes: thread t owns
 // SPT sites of one plane word (see spk_bits.h).
-template <int SPT>
+template <int SPT, bool SINGLE>
 __global__ __launch_bounds__(1024) void k_sweep_bits(uint32_t *__restrict__ bits, int32_t *__restrict__ sseed, Geom g,
                                                     BitGeom bg, Rule r, int color,
                                                     unsigned long long *__restrict__ acc) {
   const uint32_t t = (uint32_t)xcd_tile(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
   Counts cnt{0, 0};
-  if ((int64_t)t < bg.threads) cnt = bits_thread<SPT>(bits, sseed, g, bg, r, color, t);
+  if ((int64_t)t < bg.threads) cnt = bits_thread<SPT, SINGLE>(bits, sseed, g, bg, r, color, t);
   block_accumulate_pack(cnt.nacc, cnt.de, acc, blockIdx.x & (kAccSlots - 1));
 }
 
@@ -319,7 +319,6 @@ int launch_variant(spk_hip_lattice *h, int color, unsigned long long *acc) {
 }
 
 int launch_sweep(spk_hip_lattice *h, int color, unsigned long long *acc) {
-  if (variant_app(h->d.app)) return launch_variant(h, color, acc);
   if (h->bits_active) {
     const int bs = h->block;
     unsigned grid = grid_for(h->bg.threads, bs);
@@ -328,11 +327,13 @@ int launch_sweep(spk_hip_lattice *h, int color, unsigned long long *acc) {
       hipLaunchKernelGGL(kern, dim3(grid), dim3(bs), 0, h->stream, h->bits, h->sseed, h->g, h->bg, h->rule, color,
                          acc);
     };
-    if (h->bg.spt == 8) go(k_sweep_bits<8>);
-    else if (h->bg.spt == 16) go(k_sweep_bits<16>);
-    else go(k_sweep_bits<32>);
+    const bool single = h->d.app == SPK_HIP_ISING_SINGLE;
+    if (h->bg.spt == 8) single ? go(k_sweep_bits<8, true>) : go(k_sweep_bits<8, false>);
+    else if (h->bg.spt == 16) single ? go(k_sweep_bits<16, true>) : go(k_sweep_bits<16, false>);
+    else single ? go(k_sweep_bits<32, true>) : go(k_sweep_bits<32, false>);
     return check_hip(hipGetLastError(), "bit-plane sweep launch");
   }
+  if (variant_app(h->d.app)) return launch_variant(h, color, acc);
   const int bs = h->block;
   const TileMap m = make_tile_map(h->g, bs);
   unsigned grid = (unsigned)m.nxc * m.nyg * m.nzg;
@@ -430,7 +431,7 @@ void drop_graphs(spk_hip_lattice *h) {
 // Two-state sq/4n lattices on the per-colour path sweep on bit planes.
 bool bits_eligible(const spk_hip_lattice *h) {
   return h->bitplanes && h->g.lat == SPK_HIP_SQ_4N && h->rule.nspins == 2 && h->stype == kU8 &&
-         !variant_app(h->d.app) && !h->fuse;
+         (!variant_app(h->d.app) || h->d.app == SPK_HIP_ISING_SINGLE) && !h->fuse;
 }
 
 // Packs the state into the planes (allocated on first use) and routes the

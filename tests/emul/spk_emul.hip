@@ -165,7 +165,8 @@ static int run(int app, int nx, int ny, int nz, int q, double kt, int nsw, const
     }
   }
   const char *bz = getenv("SPK_EMUL_BITS");
-  if (bz && LAT == SPK_HIP_SQ_4N && r.nspins == 2 && sizeof(T) == 1 && app < SPK_HIP_POTTS_NEIGH) {
+  if (bz && LAT == SPK_HIP_SQ_4N && r.nspins == 2 && sizeof(T) == 1 &&
+      (app < SPK_HIP_POTTS_NEIGH || app == SPK_HIP_ISING_SINGLE)) {
     // bit-plane path (spk_bits.h): pack, every pass thread by thread, unpack
     const int spt = atoi(bz);
     if (spt != 8 && spt != 16 && spt != 32) { fprintf(stderr, "SPK_EMUL_BITS must be 8, 16 or 32\n"); return 16; }
@@ -207,9 +208,13 @@ static int run(int app, int nx, int ny, int nz, int q, double kt, int nsw, const
       long long de = 0;
       for (int c = 0; c < 2; c++)
         for (uint32_t t = 0; t < (uint32_t)bg.threads; t++) {
-          const Counts cnt = spt == 8    ? bits_thread<8>(bits.data(), sseed.data(), g, bg, r, c, t)
-                             : spt == 16 ? bits_thread<16>(bits.data(), sseed.data(), g, bg, r, c, t)
-                                         : bits_thread<32>(bits.data(), sseed.data(), g, bg, r, c, t);
+          const bool single = app == SPK_HIP_ISING_SINGLE;
+          const Counts cnt = spt == 8    ? (single ? bits_thread<8, true>(bits.data(), sseed.data(), g, bg, r, c, t)
+                                               : bits_thread<8>(bits.data(), sseed.data(), g, bg, r, c, t))
+                             : spt == 16 ? (single ? bits_thread<16, true>(bits.data(), sseed.data(), g, bg, r, c, t)
+                                               : bits_thread<16>(bits.data(), sseed.data(), g, bg, r, c, t))
+                                         : (single ? bits_thread<32, true>(bits.data(), sseed.data(), g, bg, r, c, t)
+                                               : bits_thread<32>(bits.data(), sseed.data(), g, bg, r, c, t));
           na += cnt.nacc;
           de += cnt.de;
         }

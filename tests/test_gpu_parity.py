@@ -362,6 +362,13 @@ BIT_CASES = [
     ("ising", "sq/4n", (40, 34, 1), 2, 0.0, 12, 5),
     ("potts", "sq/4n", (64, 2, 1), 2, 0.0, 13, 4),
     ("ising", "sq/4n", (256, 130, 1), 2, 2.269, 14, 6),
+    # ising/single: flip proposals, one draw only for an uphill flip; with
+    # planes off it runs the one-site-per-thread variant kernel
+    ("ising/single", "sq/4n", (18, 10, 1), 2, 2.269, 14, 7),
+    ("ising/single", "sq/4n", (66, 34, 1), 2, 1.2, 15, 6),
+    ("ising/single", "sq/4n", (130, 4, 1), 2, 3.5, 16, 5),
+    ("ising/single", "sq/4n", (40, 8, 1), 2, 0.0, 17, 5),
+    ("ising/single", "sq/4n", (256, 130, 1), 2, 2.269, 18, 6),
 ]
 
 
