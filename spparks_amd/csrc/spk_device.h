@@ -444,18 +444,43 @@ __host__ __device__ __forceinline__ void natural_of(int c, int a, int b, int e, 
   }
 }
 
-// neighbour values of natural site (x, y, z) in the reference's order
+// neighbour values of natural site (x, y, z) in the reference's order.
+// The colour-blocked slot (blocked_index) is a sum of one part per axis, so
+// the three periodic coordinates v-1, v, v+1 of each axis are turned into
+// their parts once and a neighbour's slot is two additions:
+//   sq/8n, sc/26n: the colour is a sum of per-axis parity bits, so its
+//     Mp-multiple rides in the axis parts;
+//   sq/4n, sc/6n: every nearest neighbour has the other colour (edges are
+//     even), so one colour base serves all of them.
 template <int LAT, typename T>
 __host__ __device__ __forceinline__ void gather_ordered(const T *__restrict__ spin, const Geom &g, int x, int y,
                                                         int z, int32_t (&nb)[NeighCount<LAT>::value]) {
+  constexpr bool is3 = LAT == SPK_HIP_SC_6N || LAT == SPK_HIP_SC_26N;
+  constexpr bool checker = LAT == SPK_HIP_SQ_4N || LAT == SPK_HIP_SC_6N;
+  const int xs[3] = {wrap_dec(x, g.nx), x, wrap_inc(x, g.nx)};
+  const int ys[3] = {wrap_dec(y, g.ny), y, wrap_inc(y, g.ny)};
+  const int zs[3] = {is3 ? wrap_dec(z, g.nz) : 0, z, is3 ? wrap_inc(z, g.nz) : 0};
+  const int64_t Mp = g.Mp, px = g.px, pxy = (int64_t)g.px * g.hy;
+  int64_t lx[3], ly[3], lz[3];
+#pragma unroll
+  for (int d = 0; d < 3; d++) {
+    if constexpr (checker) {
+      lx[d] = xs[d] >> 1;
+      ly[d] = px * ys[d];
+      lz[d] = is3 ? pxy * zs[d] : 0;
+    } else {
+      lx[d] = (xs[d] >> 1) + (xs[d] & 1) * Mp;
+      ly[d] = px * (ys[d] >> 1) + 2 * (ys[d] & 1) * Mp;
+      lz[d] = is3 ? pxy * (zs[d] >> 1) + 4 * (zs[d] & 1) * Mp : 0;
+    }
+  }
+  // checkerboards: colour (x + y + z) & 1 of the site, flipped
+  const T *base = checker ? spin + (int64_t)(((x + y + z) & 1) ^ 1) * Mp : spin;
 #pragma unroll
   for (int m = 0; m < NeighCount<LAT>::value; m++) {
     int dx = 0, dy = 0, dz = 0;
     Shell<LAT>::offset(m, dx, dy, dz);
-    const int xi = dx < 0 ? wrap_dec(x, g.nx) : dx > 0 ? wrap_inc(x, g.nx) : x;
-    const int yj = dy < 0 ? wrap_dec(y, g.ny) : dy > 0 ? wrap_inc(y, g.ny) : y;
-    const int zk = dz < 0 ? wrap_dec(z, g.nz) : dz > 0 ? wrap_inc(z, g.nz) : z;
-    nb[m] = (int32_t)spin[blocked_index(g, xi, yj, zk)];
+    nb[m] = (int32_t)base[lx[dx + 1] + ly[dy + 1] + lz[dz + 1]];
   }
 }
 
