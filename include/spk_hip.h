@@ -149,7 +149,7 @@ int spk_hip_sync(spk_hip_lattice *h);
    covers a tile of sub-grid rows (x, y and z), rows in y and z. */
 int spk_hip_set_launch(spk_hip_lattice *h, int block);
 
-/* Two-state sq/4n lattices (Ising, or Potts with Q = 2) sweep on bit planes
+/* Two-state sq/4n lattices (Ising, ising/single, or Potts with Q = 2) sweep on bit planes
    by default: for the duration of each spk_hip_sweeps call the spins are
    held one bit per site, 32 sites of a sub-grid row per word, and the seeds
    in a wavefront-interleaved order; the state is packed on entry and
