@@ -11,8 +11,9 @@ EXE = os.path.join(ROOT, "tests", "abi", "consumer")
 
 
 def consumer(mode, timeout):
-    if not os.path.exists(EXE):
-        raise FileNotFoundError("tests/abi/consumer is not built; run __graft_entry__.build()")
+    if not os.path.exists(EXE):   # built by build(); a plain gcc line when it is missing
+        from spparks_amd import build
+        build.build_abi_consumer()
     print("consumer", mode, flush=True)
     p = subprocess.run([EXE, mode], capture_output=True, text=True, timeout=timeout)
     assert p.returncode == 0, p.stdout + p.stderr
