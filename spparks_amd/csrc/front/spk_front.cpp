@@ -16,6 +16,7 @@ namespace spkfront {
 namespace {
 
 constexpr double kEpsilon = 1.0e-6;   // src/create_sites.cpp EPSILON
+constexpr int kCallSweeps = 4096;     // most sweeps per device call of a run
 
 double wall() {
   using namespace std::chrono;
@@ -843,19 +844,29 @@ double Spparks::output_setup() {
   return std::min(std::min(stoptime_, dump_time), stats_time_);
 }
 
-// Output::compute (src/output.cpp:145-202)
-double Spparks::output_compute(bool done) {
+// Output::compute (src/output.cpp:145-202), in two halves.
+// The stats half: a line when it is due.  energy_now is
+// the energy after the current sweep when the caller has it from the
+// per-sweep counts of a sweeps call (null: ask the device).
+void Spparks::stats_compute(bool done, const double *energy_now) {
   const bool sflag = time_ >= stats_time_ || done;
-  if (sflag && diag_energy_ && diag_stats_) hip(spk_hip_energy(h_, &energy_));
-  if (sflag) {
-    stats_line(true);
-    stats_time_ = stoptime_;
-    if (stats_delta_ != 0.0) {
-      stats_time_ =
-          next_time(time_ + stats_tol_, stats_logfreq_, stats_delta_, stats_nrepeat_, stats_scale_, stats_delay_);
-      stats_time_ -= stats_tol_;
-    }
+  if (!sflag) return;
+  if (diag_energy_ && diag_stats_) {
+    if (energy_now) energy_ = *energy_now;
+    else hip(spk_hip_energy(h_, &energy_));
   }
+  stats_line(true);
+  stats_time_ = stoptime_;
+  if (stats_delta_ != 0.0) {
+    stats_time_ =
+        next_time(time_ + stats_tol_, stats_logfreq_, stats_delta_, stats_nrepeat_, stats_scale_, stats_delay_);
+    stats_time_ -= stats_tol_;
+  }
+}
+
+// The dump half: every dump due at the current time; returns the next
+// dump time (or the stop time).
+double Spparks::dumps_compute() {
   double dump_time = stoptime_;
   for (auto &d : dumps_) {
     if (time_ >= d.next_time) {
@@ -864,7 +875,7 @@ double Spparks::output_compute(bool done) {
     }
     dump_time = std::min(dump_time, d.next_time);
   }
-  return std::min(std::min(stoptime_, dump_time), stats_time_);
+  return dump_time;
 }
 
 // App::run (src/app.cpp:86-148) -> AppLattice::setup / iterate_rejection
@@ -902,15 +913,23 @@ void Spparks::cmd_run(const std::vector<std::string> &a) {
   // setup: one colour sweep per dt_rkmc = dt_sweep, cut at the stop time
   double dt_rkmc = dt_sweep_;
   if (stoptime_ - time_ < dt_rkmc) dt_rkmc = stoptime_ - time_;
-  double nextoutput = output_setup();
+  output_setup();
   first_run_ = false;
 
   loop_t0_ = wall();
   if (stoptime_ > time_) {
     bool done = false;
+    const bool want_e = diag_energy_ && diag_stats_;
+    std::vector<int64_t> acc_sw;
+    std::vector<double> eng_sw;
     while (!done) {
-      // the sweeps up to the next output or the stop time run as one
-      // device call; time advances exactly as one sweep at a time would
+      // the sweeps up to the next dump or the stop time (at most
+      // kCallSweeps) run as one device call; time advances exactly as one
+      // sweep at a time would, and the stats lines due in between are
+      // printed from the call's per-sweep Naccept and energies, as
+      // Output::compute would have printed them after each sweep
+      double dump_time = stoptime_;
+      for (const auto &d : dumps_) dump_time = std::min(dump_time, d.next_time);
       int n = 0;
       double t = time_;
       for (;;) {
@@ -920,22 +939,35 @@ void Spparks::cmd_run(const std::vector<std::string> &a) {
           done = true;
           break;
         }
-        if (t >= nextoutput) break;
-        if (n == (1 << 30)) break;
+        if (t >= dump_time) break;
+        if (n == kCallSweeps) break;
       }
+      acc_sw.resize(n);
+      eng_sw.resize(want_e ? n : 0);
       const double t0 = wall();
       int64_t nacc = 0;
-      hip(spk_hip_sweeps(h_, n, &nacc, nullptr, nullptr));
+      hip(spk_hip_sweeps(h_, n, &nacc, acc_sw.data(), want_e ? eng_sw.data() : nullptr));
       t_solve_ += wall() - t0;
-      naccept_ += nacc;
-      nattempt_ += (int64_t)n * nsites_;
-      nsweeps_ += n;
-      time_ = t;
-      if (done || time_ >= nextoutput) {
-        const double t1 = wall();
-        nextoutput = output_compute(done);
-        t_output_ += wall() - t1;
+      const double t1 = wall();
+      const int64_t acc0 = naccept_, att0 = nattempt_;
+      const int sw0 = nsweeps_;
+      double tk = time_;
+      for (int k = 0; k < n; k++) {
+        tk += dt_rkmc;
+        const bool last = done && k == n - 1;
+        if (!(tk >= stats_time_ || last)) continue;
+        time_ = tk;
+        naccept_ = acc0 + acc_sw[k];
+        nattempt_ = att0 + (int64_t)(k + 1) * nsites_;
+        nsweeps_ = sw0 + k + 1;
+        stats_compute(last, want_e ? &eng_sw[k] : nullptr);
       }
+      time_ = t;
+      naccept_ = acc0 + nacc;
+      nattempt_ = att0 + (int64_t)n * nsites_;
+      nsweeps_ = sw0 + n;
+      (void)dumps_compute();
+      t_output_ += wall() - t1;
     }
   }
   {

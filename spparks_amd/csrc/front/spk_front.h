@@ -97,7 +97,8 @@ class Spparks {
   // run machinery
   void init_run();
   double output_setup();
-  double output_compute(bool done);
+  void stats_compute(bool done, const double *energy_now);
+  double dumps_compute();
   void stats_line(bool timeflag);
   void stats_header();
   void write_dump(Dump &d);
