@@ -127,6 +127,40 @@ def test_full_size_scripts_first_sweeps(orc, tmp_path):
         assert [r[4] for r in rows[1:]] == [float("%10g" % e) for e in eng]
 
 
+def test_zero_temperature_3d_script_with_default_dump(orc, tmp_path):
+    """A 3d grain-growth script at T = 0 with a text dump that names no
+    fields (the reference's default: id site x y z, the header calling site
+    "type"): every snapshot and every stats line against the oracle, and the
+    energy never rises at T = 0."""
+    p = tmp_path / "in.gg"
+    p.write_text("seed 56789\napp_style potts 100\ndimension 3\nlattice sc/26n 1.0\n"
+                 "region box block 0 20 0 20 0 20\ncreate_box box\ncreate_sites box\n"
+                 "set site range 1 100\nsweep color/strict\ndiag_style energy\ntemperature 0.0\n"
+                 "stats 0.2\ndump 1 text 0.5 dump.gg\nrun 1.0\n")
+    out = run_exe(tmp_path, str(p))
+    d = orc.make_desc("potts", "sc/26n", 20, 20, 20, 100, 0.0)
+    spin, seeds = orc.script_init(d, 56789, 1, 100)
+    nacc, eng = orc.sweeps(d, spin.copy(), seeds.copy(), 100)
+    rows = stats_rows(out)
+    assert [r[3] for r in rows] == [0, 20, 40, 60, 80, 100]
+    assert [r[1] for r in rows[1:]] == [int(nacc[k - 1]) for k in (20, 40, 60, 80, 100)]
+    assert [r[4] for r in rows[1:]] == [float("%10g" % eng[k - 1]) for k in (20, 40, 60, 80, 100)]
+    assert all(b <= a for a, b in zip(eng, eng[1:]))
+    snaps = open(tmp_path / "dump.gg").read().split("ITEM: TIMESTEP\n")[1:]
+    assert len(snaps) == 3   # t = 0, 0.5, 1.0
+    sp, sd = spin.copy(), seeds.copy()
+    for snap, nsw in zip(snaps, (0, 50, 50)):
+        if nsw:
+            orc.sweeps(d, sp, sd, nsw)
+        lines = snap.splitlines()
+        assert lines[7] == "ITEM: ATOMS id type x y z"
+        vals = np.array([[float(x) for x in ln.split()] for ln in lines[8:]])
+        assert np.array_equal(vals[:, 0], np.arange(1, 8001))
+        assert np.array_equal(vals[:, 1], sp)
+        g = np.arange(8000)
+        assert np.array_equal(vals[:, 2:], np.stack([g % 20, (g // 20) % 20, g // 400], 1))
+
+
 def test_dump_text_snapshots(orc, tmp_path):
     p = tmp_path / "in.dump"
     p.write_text("seed 99\napp_style ising\ndimension 2\nlattice sq/4n 1.0\n"
