@@ -110,6 +110,18 @@ def build_abi_consumer():
     return out
 
 
+def build_dump_format_check():
+    """tests/abi/dump_format: the dump writer's formatting against fprintf,
+    host only.  Test infrastructure."""
+    src = os.path.join(ROOT, "tests", "abi", "dump_format.cpp")
+    if not os.path.exists(src):
+        return None
+    out = os.path.join(ROOT, "tests", "abi", "dump_format")
+    run([cxx(), *CXX_FLAGS, "-I" + os.path.join(CSRC, "front"), "-o", out, src,
+         os.path.join(CSRC, "front", "spk_dump_writer.cpp")])
+    return out
+
+
 def build():
     build_hip()
     build_front()

@@ -46,3 +46,16 @@ def test_batch_and_library_from_c():
         assert l[3:6] == l[7:10], l
     lib = [l for l in out if l[0] == "library"][0]
     assert lib[1:] == ["nglobal", "16384", "energy", "97818"]
+
+
+def test_dump_writer_formats_like_fprintf(tmp_path):
+    """The dump writer's buffered formatting (to_chars for integers, %g for
+    coordinates and energies) writes the same bytes as one fprintf per value,
+    for every field, 2d and 3d, buffers that fill several times, and two
+    snapshots queued through the writer thread into one file."""
+    exe = os.path.join(ROOT, "tests", "abi", "dump_format")
+    if not os.path.exists(exe):
+        from spparks_amd import build
+        build.build_dump_format_check()
+    p = subprocess.run([exe, str(tmp_path)], capture_output=True, text=True, timeout=120)
+    assert p.returncode == 0 and p.stdout.strip() == "ok", p.stdout + p.stderr
