@@ -276,10 +276,15 @@ def test_temperature_change_between_runs(orc):
 FULL = [
     ("ising", "sq/4n", (4096, 4096, 1), 2, 2.269, 12345, 5),
     ("potts", "sc/26n", (256, 256, 256), 10000, 0.5, 24680, 2),
+    # the proposal variants at the sizes they are measured at
+    ("potts/neigh", "sc/26n", (256, 256, 256), 10000, 0.5, 24680, 2),
+    ("potts/neighonly", "sc/26n", (256, 256, 256), 10000, 0.5, 24680, 2),
+    ("ising/single", "sq/4n", (4096, 4096, 1), 2, 2.269, 12345, 5),
 ]
 
 
-@pytest.mark.parametrize("case", FULL, ids=["ising-sq4n-4096", "potts-sc26n-256"])
+@pytest.mark.parametrize("case", FULL, ids=["ising-sq4n-4096", "potts-sc26n-256", "potts-neigh-sc26n-256",
+                                            "potts-neighonly-sc26n-256", "ising-single-sq4n-4096"])
 def test_full_size_first_sweeps_bit_exact(orc, case):
     import spparks_amd as S
     app, lat, shape, q, kt, seed, n = case
