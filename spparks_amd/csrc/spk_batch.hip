@@ -649,7 +649,7 @@ int spk_hip_batch_run(spk_hip_batch *b, int max_sweeps, int use_graph) {
   std::vector<Shape> cand;
   if (tune) {
     for (int bs : {128, 256, 512})
-      for (int wpc : {4, 8, 16}) {
+      for (int wpc : {4, 8, 16, 32}) {
         const int grid = std::max(kNumXcd, b->ncu * wpc * 64 / bs / kNumXcd * kNumXcd);
         cand.push_back(Shape{bs, grid});
       }
