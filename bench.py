@@ -381,7 +381,8 @@ def run(args):
         "vs_baseline": None,
         "dtype": "int32",
         "data": "synthetic: set site range 1 Q from the master seed, strict seeds by tagreset",
-        "config": {"workload": "%s: %s, %d strict sweeps per step count" % (args.workload, desc, nsw),
+        "config": {"workload": "%s: %s; a step is one strict sweep of the whole lattice, %d timed"
+                               % (args.workload, desc, steps),
                    "app": app, "lattice": lattice, "shape": list(shape), "nspins": q, "kT": kt,
                    "seed": seed, "sites": n, "colors": K, "sweep": "color/strict",
                    "naccept": int(nacc), "final_energy": energy},
