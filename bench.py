@@ -393,7 +393,11 @@ def run(args):
                      "kernel": kernel,
                      "bytes_per_launch": bytes_per_launch,
                      "avg_launch_us": round(per_launch_s * 1e6, 3),
-                     "bytes_per_update": round(B_kernel, 3)},
+                     "bytes_per_update": round(B_kernel, 3),
+                     # the int32-spin byte models of the per-colour design, for
+                     # comparison only: frac above uses what this kernel moves
+                     "int32_model_bytes_per_update": {"per_colour_4K_plus_12": 4 * K + 12,
+                                                      "compulsory_spin_and_seed_16": 16}},
         "pct_hbm_peak": round(100 * value * 1e9 * B_kernel / (HBM_PEAK_GBS * 1e9), 2),
         "spin_bytes": spin_bytes(app, q),
     }
