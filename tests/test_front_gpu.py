@@ -127,6 +127,40 @@ def test_full_size_scripts_first_sweeps(orc, tmp_path):
         assert [r[4] for r in rows[1:]] == [float("%10g" % e) for e in eng]
 
 
+def test_run_options_log_undump_and_stats_delay(orc, tmp_path):
+    """sweep color/strict/hip, stats with a delay, undump, log, print,
+    run ... upto and run ... pre no post no, and a zero-length run that only
+    writes a dump: the stats schedule, the files each dump and log get, and
+    the final spins against the oracle after the sweeps the stats report."""
+    p = tmp_path / "in.opts"
+    p.write_text("seed 777\napp_style potts 50\ndimension 2\nlattice sq/8n 1.0\n"
+                 "region box block 0 32 0 24 -0.5 0.5\ncreate_box box\ncreate_sites box\n"
+                 "set site range 1 50\nsweep color/strict/hip mask no\ntemperature 0.6\n"
+                 "diag_style energy\nstats 0.02 delay 0.06\ndump 1 text 0.1 d1.txt id site\nrun 0.2\n"
+                 "undump 1\nlog log.second\nprint \"second part\"\nrun 0.4 upto\n"
+                 "run 0.1 pre no post no\ndump 3 text 1.0 end.txt id site\nrun 0.0\n")
+    out = run_exe(tmp_path, str(p), "-echo", "none")
+    rows = stats_rows(out)
+    # dt per sweep is 1/Q = 0.02: the first run prints at setup, then from
+    # t = 0.06 (the delay) after every sweep
+    first = [r[3] for r in rows if r[3] <= 10]
+    assert first[:9] == [0, 3, 4, 5, 6, 7, 8, 9, 10]
+    snaps = open(tmp_path / "d1.txt").read().split("ITEM: TIMESTEP\n")[1:]
+    assert len(snaps) == 3   # t = 0, 0.1, 0.2; undumped before the later runs
+    second = open(tmp_path / "log.second").read()
+    assert "second part" in second and "Loop time" in second
+    nsw = rows[-1][3]
+    assert 19 <= nsw <= 21 + 6   # 0.2 + up to 0.4 + 0.1 in steps of 0.02
+    d = orc.make_desc("potts", "sq/8n", 32, 24, 1, 50, 0.6)
+    spin, seeds = orc.script_init(d, 777, 1, 50)
+    nacc, eng = orc.sweeps(d, spin, seeds, nsw)
+    assert rows[-1][1] == int(nacc[-1]) and rows[-1][4] == float("%10g" % eng[-1])
+    end = open(tmp_path / "end.txt").read().split("ITEM: TIMESTEP\n")[1:]
+    assert len(end) == 1
+    vals = np.array([[int(x) for x in ln.split()] for ln in end[0].splitlines()[8:]])
+    assert np.array_equal(vals[:, 1], spin)
+
+
 def test_zero_temperature_3d_script_with_default_dump(orc, tmp_path):
     """A 3d grain-growth script at T = 0 with a text dump that names no
     fields (the reference's default: id site x y z, the header calling site
