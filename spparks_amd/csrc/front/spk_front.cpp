@@ -198,7 +198,32 @@ std::string Spparks::one(const std::string &raw) {
   else if (cmd == "variable") cmd_variable(a);
   else if (cmd == "print") cmd_print(a);
   else if (cmd == "log") cmd_log(a);
-  else if (cmd == "echo" || cmd == "processors") {
+  else if (cmd == "include") {
+    // Input::include (src/input.cpp:599-617): the file's commands, then back
+    if (a.size() != 1) throw Error("Illegal include command");
+    file(a[0].c_str());
+  } else if (cmd == "reset_time") {
+    // Input::reset_time (src/input.cpp:878-929), App::reset_time (src/app.cpp:152-155)
+    if (app_.empty()) throw Error("Reset_time command before app_style set");
+    if (a.size() < 1) throw Error("Illegal reset_time command");
+    if (a[0] == "stitch") throw Error("reset_time stitch is outside this build");
+    if (a.size() != 1) throw Error("Illegal reset_time command");
+    const double t = std::atof(a[0].c_str());
+    if (t < 0.0) throw Error("Illegal reset_time command");
+    time_ = t;
+  } else if (cmd == "dump_one") {
+    // Input::dump_one, Output::dump_one (src/output.cpp:286-299): one
+    // snapshot of a dump now, at the current time
+    if (app_.empty()) throw Error("Dump_one command before app_style set");
+    if (a.size() != 1) throw Error("Illegal dump_one command");
+    Dump *d = nullptr;
+    for (auto &x : dumps_)
+      if (x.id == a[0]) d = &x;
+    if (!d) throw Error("Could not find dump ID in dump_one command");
+    if (!h_) throw Error("Dump_one command before sites exist");
+    write_dump(*d);
+    drain_dumps();
+  } else if (cmd == "echo" || cmd == "processors") {
     // accepted: echo is cosmetic; one GPU has no processor grid
   } else if (cmd == "sweep" || cmd == "temperature") {
     // app-specific commands fall through to the app (src/input.cpp:512-514)

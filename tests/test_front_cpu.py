@@ -29,6 +29,13 @@ def run_script(tmp_path, text, *args):
 
 @pytest.mark.parametrize("text,msg", [
     ("seed 0\n", "Illegal seed command"),
+    ("include no_such_file.in\n", "Cannot open input script no_such_file.in"),
+    ("include\n", "Illegal include command"),
+    ("reset_time 1.0\n", "Reset_time command before app_style set"),
+    ("app_style potts 10\nreset_time -1.0\n", "Illegal reset_time command"),
+    ("app_style potts 10\nreset_time stitch f.db first\n", "outside this build"),
+    ("dump_one 1\n", "Dump_one command before app_style set"),
+    ("app_style potts 10\ndump_one 1\n", "Could not find dump ID in dump_one command"),
     ("bogus_command 1\n", "Unknown command"),
     ("run 1.0\n", "Run command before app_style set"),
     ("app_style potts 10\ndimension 3\nlattice sq/8n 1.0\n", "Lattice style does not match dimension"),

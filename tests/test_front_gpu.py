@@ -161,6 +161,35 @@ def test_run_options_log_undump_and_stats_delay(orc, tmp_path):
     assert np.array_equal(vals[:, 1], spin)
 
 
+def test_include_dump_one_and_reset_time(orc, tmp_path):
+    """include runs another file's commands in place; dump_one writes one
+    snapshot of a dump at the current time; reset_time moves the clock that
+    stats print without touching the state or the sweep count."""
+    (tmp_path / "in.setup").write_text("app_style ising\ndimension 2\nlattice sq/4n 1.0\n"
+                                       "region box block 0 24 0 16 -0.5 0.5\ncreate_box box\ncreate_sites box\n")
+    p = tmp_path / "in.main"
+    p.write_text("seed 31\ninclude in.setup\nset site range 1 2\nsweep color/strict\ntemperature 2.0\n"
+                 "diag_style energy\nstats 1.0\ndump 7 text 100.0 once.txt id site\nrun 3.0\n"
+                 "dump_one 7\nreset_time 10.0\nrun 2.0\n")
+    out = run_exe(tmp_path, str(p))
+    rows = stats_rows(out)
+    # ising: dt per sweep is 1/2, so run 3.0 is 6 sweeps and run 2.0 four more
+    assert [r[3] for r in rows] == [0, 2, 4, 6, 6, 8, 10]
+    assert [r[0] for r in rows] == [0.0, 1.0, 2.0, 3.0, 10.0, 11.0, 12.0]
+    d = orc.make_desc("ising", "sq/4n", 24, 16, 1, 2, 2.0)
+    spin, seeds = orc.script_init(d, 31, 1, 2)
+    spin6 = spin.copy()
+    sd6 = seeds.copy()
+    nacc, eng = orc.sweeps(d, spin6, sd6, 6)
+    snaps = open(tmp_path / "once.txt").read().split("ITEM: TIMESTEP\n")[1:]
+    assert len(snaps) == 2   # the run's first snapshot at t = 0, then dump_one at t = 3
+    assert snaps[1].splitlines()[0].split()[0] == "1"
+    vals = np.array([[int(x) for x in ln.split()] for ln in snaps[1].splitlines()[8:]])
+    assert np.array_equal(vals[:, 1], spin6)
+    nacc10, eng10 = orc.sweeps(d, spin, seeds, 10)
+    assert rows[-1][1] == int(nacc10[-1]) and rows[-1][4] == float("%10g" % eng10[-1])
+
+
 def test_zero_temperature_3d_script_with_default_dump(orc, tmp_path):
     """A 3d grain-growth script at T = 0 with a text dump that names no
     fields (the reference's default: id site x y z, the header calling site
