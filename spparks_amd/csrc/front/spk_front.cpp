@@ -561,13 +561,19 @@ void Spparks::cmd_variable(const std::vector<std::string> &a) {
   if (a.size() < 3) throw Error("Illegal variable command");
   if (a[1] != "index" && a[1] != "string" && a[1] != "equal")
     throw Error("variable style " + a[1] + " is outside this build (index, string, equal numbers)");
+  std::string value = a[2];
   if (a[1] == "equal") {
     char *end = nullptr;
-    std::strtod(a[2].c_str(), &end);
+    const double v = std::strtod(a[2].c_str(), &end);
     if (*end) throw Error("equal-style formulas are outside this build; give a number");
+    // an equal-style variable substitutes its value as "%.10g"
+    // (Variable::retrieve, src/variable.cpp:349-352)
+    char buf[64];
+    std::snprintf(buf, sizeof buf, "%.10g", v);
+    value = buf;
   }
   if (a[1] == "index" && vars_.count(a[0])) return;   // -var on the command line wins
-  vars_[a[0]] = a[2];
+  vars_[a[0]] = value;
 }
 
 void Spparks::cmd_print(const std::vector<std::string> &a) {

@@ -113,3 +113,15 @@ def test_read_sites_header_errors(tmp_path, header, values, msg):
     rc, out = run_script(tmp_path, PRE + "read_sites %s\n" % f)
     assert rc != 0
     assert msg in out, out
+
+
+def test_equal_style_variables_substitute_ten_digits(tmp_path):
+    """An equal-style variable substitutes "%.10g" of its value
+    (src/variable.cpp:349-352), so a temperature written with more digits
+    runs at the ten-digit value, as in the reference; index and string
+    variables substitute their text."""
+    rc, out = run_script(tmp_path, "variable T equal 2.269185314213\nvariable D equal 1.0e-4\n"
+                                   "variable N equal 250000\nvariable S index 0.12345678901234\n"
+                                   "print ${T} ${D} ${N} ${S}\n")
+    assert rc == 0, out
+    assert "2.269185314 0.0001 250000 0.12345678901234" in out
