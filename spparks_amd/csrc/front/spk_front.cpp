@@ -194,6 +194,7 @@ std::string Spparks::one(const std::string &raw) {
   else if (cmd == "dump") cmd_dump(a);
   else if (cmd == "read_sites") cmd_read_sites(a);
   else if (cmd == "undump") cmd_undump(a);
+  else if (cmd == "dump_modify") cmd_dump_modify(a);
   else if (cmd == "run") cmd_run(a);
   else if (cmd == "variable") cmd_variable(a);
   else if (cmd == "print") cmd_print(a);
@@ -557,6 +558,72 @@ void Spparks::cmd_undump(const std::vector<std::string> &a) {
   throw Error("Could not find undump ID");
 }
 
+// Output::dump_modify (src/output.cpp:303-316) and Dump::modify_params
+// (src/dump.cpp:644-790).  Snapshots are always in id order, so sort off and
+// sort id are accepted; per-processor files and the text style's region and
+// threshold filters are outside this build.
+void Spparks::cmd_dump_modify(const std::vector<std::string> &a) {
+  if (a.empty()) throw Error("Illegal dump_modify command");
+  Dump *d = nullptr;
+  for (auto &x : dumps_)
+    if (x.id == a[0]) d = &x;
+  if (!d) throw Error("Could not find dump ID in dump_modify command");
+  if (a.size() < 2) throw Error("Illegal dump_modify command");
+  auto need = [&](size_t i, size_t n) {
+    if (i + n > a.size()) throw Error("Illegal dump_modify command");
+  };
+  for (size_t i = 1; i < a.size();) {
+    const std::string &k = a[i];
+    if (k == "delay") {
+      need(i, 2);
+      d->delay = std::atof(a[i + 1].c_str());
+      i += 2;
+    } else if (k == "delta") {
+      need(i, 2);
+      d->delta = std::atof(a[i + 1].c_str());
+      if (d->delta <= 0.0) throw Error("Illegal dump_modify command");
+      i += 2;
+    } else if (k == "first") {
+      need(i, 2);
+      d->idump = std::atoi(a[i + 1].c_str());
+      if (d->idump < 0) throw Error("Illegal dump_modify command");
+      i += 2;
+    } else if (k == "flush") {
+      need(i, 2);
+      if (a[i + 1] != "yes" && a[i + 1] != "no") throw Error("Illegal dump_modify command");
+      i += 2;   // every snapshot is flushed when written
+    } else if (k == "logfreq" || k == "loglinfreq") {
+      need(i, 3);
+      d->nrepeat = std::atoi(a[i + 1].c_str());
+      d->scale = std::atof(a[i + 2].c_str());
+      if (d->nrepeat < 0) throw Error("Illegal dump_modify command");
+      d->logfreq = d->nrepeat == 0 ? 0 : k == "logfreq" ? 1 : 2;
+      i += 3;
+    } else if (k == "pad") {
+      need(i, 2);
+      d->pad = std::atoi(a[i + 1].c_str());
+      if (d->pad < 0) throw Error("Illegal dump_modify command");
+      i += 2;
+    } else if (k == "sort") {
+      need(i, 2);
+      if (a[i + 1] != "off" && a[i + 1] != "id")
+        throw Error("dump_modify sort by a column is outside this build (snapshots are in id order)");
+      i += 2;
+    } else if (k == "tol") {
+      need(i, 2);
+      d->tol = std::atof(a[i + 1].c_str());
+      if (d->tol < 0.0) throw Error("Illegal dump_modify command");
+      i += 2;
+    } else if ((k == "region" || k == "thresh") && i + 1 < a.size() && a[i + 1] == "none") {
+      i += 2;   // no filter: the default
+    } else if (k == "fileper" || k == "nfile" || k == "region" || k == "thresh") {
+      throw Error("dump_modify " + k + " is outside this build");
+    } else {
+      throw Error("Illegal dump_modify command");
+    }
+  }
+}
+
 void Spparks::cmd_variable(const std::vector<std::string> &a) {
   if (a.size() < 3) throw Error("Illegal variable command");
   if (a[1] != "index" && a[1] != "string" && a[1] != "equal")
@@ -688,7 +755,9 @@ void Spparks::write_dump(Dump &d) {
   hip(spk_hip_download(h_, job.spin.data(), seeds ? job.seed.data() : nullptr));
   if (d.multifile) {
     const size_t star = d.file.find('*');
-    const std::string name = d.file.substr(0, star) + std::to_string(d.idump) + d.file.substr(star + 1);
+    char idx[32];
+    std::snprintf(idx, sizeof idx, "%0*d", d.pad, d.idump);   // dump_modify pad (src/dump.cpp:377-383)
+    const std::string name = d.file.substr(0, star) + idx + d.file.substr(star + 1);
     job.fp = std::fopen(name.c_str(), "w");
     if (!job.fp) throw Error("Cannot open dump file");
     job.close = true;
@@ -732,6 +801,7 @@ void Spparks::write_dump(Dump &d) {
   }
   writer_.submit(std::move(job));
   d.idump++;
+  d.written = true;
 }
 
 // Waits for every snapshot handed to the dump writer.
@@ -855,9 +925,11 @@ void Spparks::cmd_read_sites(const std::vector<std::string> &a) {
 double Spparks::output_setup() {
   double dump_time = stoptime_;
   for (auto &d : dumps_) {
-    if (d.idump == 0) write_dump(d);
-    d.next_time = next_time(time_, 0, d.delta, 0, 0.0, 0.0);
-    dump_time = std::min(dump_time, d.next_time);
+    // Output::setup (src/output.cpp:82-100): the first snapshot once the
+    // delay has passed, then the schedule shifted by the dump's tolerance
+    if (!d.written && time_ >= d.delay) write_dump(d);
+    d.next_time = next_time(time_ + d.tol, d.logfreq, d.delta, d.nrepeat, d.scale, d.delay) - d.tol;
+    dump_time = std::min(dump_time, d.next_time - d.tol);
   }
   if (diag_energy_ && diag_stats_) hip(spk_hip_energy(h_, &energy_));
   if (first_run_) {
@@ -900,11 +972,11 @@ void Spparks::stats_compute(bool done, const double *energy_now) {
 double Spparks::dumps_compute() {
   double dump_time = stoptime_;
   for (auto &d : dumps_) {
-    if (time_ >= d.next_time) {
+    if (time_ >= d.next_time - d.tol) {   // Output::compute (src/output.cpp:180-192)
       write_dump(d);
-      d.next_time = next_time(time_, 0, d.delta, 0, 0.0, 0.0);
+      d.next_time = next_time(time_ + d.tol, d.logfreq, d.delta, d.nrepeat, d.scale, d.delay) - d.tol;
     }
-    dump_time = std::min(dump_time, d.next_time);
+    dump_time = std::min(dump_time, d.next_time - d.tol);
   }
   return dump_time;
 }
@@ -960,7 +1032,7 @@ void Spparks::cmd_run(const std::vector<std::string> &a) {
       // printed from the call's per-sweep Naccept and energies, as
       // Output::compute would have printed them after each sweep
       double dump_time = stoptime_;
-      for (const auto &d : dumps_) dump_time = std::min(dump_time, d.next_time);
+      for (const auto &d : dumps_) dump_time = std::min(dump_time, d.next_time - d.tol);
       int n = 0;
       double t = time_;
       for (;;) {

@@ -41,6 +41,10 @@ struct Dump {
   double next_time = 0.0;
   int idump = 0;
   FILE *fp = nullptr;
+  // dump_modify (src/dump.cpp:644-790)
+  double delay = 0.0, tol = 0.0, scale = 0.0;
+  int logfreq = 0, nrepeat = 0, pad = 0;
+  bool written = false;              // a snapshot has been written (the reference's !firstflag)
 };
 
 class Spparks {
@@ -89,6 +93,7 @@ class Spparks {
   void cmd_dump(const std::vector<std::string> &a);
   void cmd_read_sites(const std::vector<std::string> &a);
   void cmd_undump(const std::vector<std::string> &a);
+  void cmd_dump_modify(const std::vector<std::string> &a);
   void cmd_run(const std::vector<std::string> &a);
   void cmd_variable(const std::vector<std::string> &a);
   void cmd_print(const std::vector<std::string> &a);

@@ -29,6 +29,13 @@ def run_script(tmp_path, text, *args):
 
 @pytest.mark.parametrize("text,msg", [
     ("seed 0\n", "Illegal seed command"),
+    (PRE + "dump 1 text 1.0 f.txt\ndump_modify 2 delay 1\n", "Could not find dump ID in dump_modify command"),
+    (PRE + "dump 1 text 1.0 f.txt\ndump_modify 1\n", "Illegal dump_modify command"),
+    (PRE + "dump 1 text 1.0 f.txt\ndump_modify 1 delta 0\n", "Illegal dump_modify command"),
+    (PRE + "dump 1 text 1.0 f.txt\ndump_modify 1 logfreq 3\n", "Illegal dump_modify command"),
+    (PRE + "dump 1 text 1.0 f.txt\ndump_modify 1 bogus 1\n", "Illegal dump_modify command"),
+    (PRE + "dump 1 text 1.0 f.txt\ndump_modify 1 sort 2\n", "outside this build"),
+    (PRE + "dump 1 text 1.0 f.txt\ndump_modify 1 region box\n", "outside this build"),
     ("include no_such_file.in\n", "Cannot open input script no_such_file.in"),
     ("include\n", "Illegal include command"),
     ("reset_time 1.0\n", "Reset_time command before app_style set"),

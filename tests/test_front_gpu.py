@@ -161,6 +161,37 @@ def test_run_options_log_undump_and_stats_delay(orc, tmp_path):
     assert np.array_equal(vals[:, 1], spin)
 
 
+def test_dump_modify_schedules(orc, tmp_path):
+    """dump_modify delay / first / pad and logfreq (src/dump.cpp:644-790)
+    with the reference's schedule (Output::setup / compute, next_time,
+    src/output.cpp:82-202, :414-437) on an Ising lattice, where a sweep is
+    0.5 time units and every time is exact in binary:
+      delta 1.5, delay 4: no snapshot before t = 4, then t = 4, 4.5, 6, 7.5
+        (next_time rounds up to the delta grid); first 5, pad 3 name them
+        d.005 .. d.008;
+      delta 0.5, logfreq 2 4.0: t = 0, then 0.5, 1, 2, 4, 8 (two per
+        factor of 4)."""
+    p = tmp_path / "in.dm"
+    p.write_text("seed 41\napp_style ising\ndimension 2\nlattice sq/4n 1.0\n"
+                 "region box block 0 20 0 12 -0.5 0.5\ncreate_box box\ncreate_sites box\n"
+                 "set site range 1 2\nsweep color/strict\ntemperature 2.1\n"
+                 "dump 1 text 1.5 d.* id site\ndump_modify 1 delay 4.0 first 5 pad 3 flush yes sort id\n"
+                 "dump 2 text 0.5 log.txt id site\ndump_modify 2 logfreq 2 4.0\nrun 8.0\n")
+    run_exe(tmp_path, str(p))
+    names = sorted(f.name for f in tmp_path.iterdir() if f.name.startswith("d."))
+    assert names == ["d.005", "d.006", "d.007", "d.008"]
+    times = [float(open(tmp_path / n).read().split("\n")[1].split()[1]) for n in names]
+    assert times == [4.0, 4.5, 6.0, 7.5]
+    logsnaps = open(tmp_path / "log.txt").read().split("ITEM: TIMESTEP\n")[1:]
+    assert [float(s.split("\n")[0].split()[1]) for s in logsnaps] == [0.0, 0.5, 1.0, 2.0, 4.0, 8.0]
+    # the t = 4 snapshot is the state after 8 sweeps
+    d = orc.make_desc("ising", "sq/4n", 20, 12, 1, 2, 2.1)
+    spin, seeds = orc.script_init(d, 41, 1, 2)
+    orc.sweeps(d, spin, seeds, 8)
+    vals = np.array([[int(x) for x in ln.split()] for ln in open(tmp_path / "d.005").read().splitlines()[9:]])
+    assert np.array_equal(vals[:, 1], spin)
+
+
 def test_include_dump_one_and_reset_time(orc, tmp_path):
     """include runs another file's commands in place; dump_one writes one
     snapshot of a dump at the current time; reset_time moves the clock that
