@@ -9,6 +9,7 @@
 #include <cstdarg>
 #include <cstdlib>
 #include <cstring>
+#include <new>
 #include <sstream>
 
 namespace spkfront {
@@ -74,8 +75,23 @@ std::vector<std::string> tokenize(const std::string &line) {
 
 }  // namespace
 
-Spparks::Spparks(FILE *screen, FILE *logfile) : screen_(screen), logfile_(logfile) {
-  message("SPPARKS (spk_hip: strict colour sweep on MI355X)\n");
+Spparks::Spparks(FILE *screen, FILE *logfile, bool banner) : screen_(screen), logfile_(logfile) {
+  if (banner) message("SPPARKS (spk_hip: strict colour sweep on MI355X)\n");
+}
+
+// Input::clear (src/input.cpp:524-529): everything is destroyed and created
+// afresh (app, lattice, dumps, stats settings, master generator) except
+// what the reference's Input and universe keep: variables, screen and log.
+// The object is rebuilt in place; no caller holds a member across a command.
+void Spparks::cmd_clear(const std::vector<std::string> &a) {
+  if (!a.empty()) throw Error("Illegal clear command");
+  FILE *screen = screen_, *logfile = logfile_;
+  std::map<std::string, std::string> vars = std::move(vars_);
+  const bool quiet = quiet_;
+  this->~Spparks();
+  new (this) Spparks(screen, logfile, false);
+  vars_ = std::move(vars);
+  quiet_ = quiet;
 }
 
 Spparks::~Spparks() {
@@ -195,6 +211,7 @@ std::string Spparks::one(const std::string &raw) {
   else if (cmd == "read_sites") cmd_read_sites(a);
   else if (cmd == "undump") cmd_undump(a);
   else if (cmd == "dump_modify") cmd_dump_modify(a);
+  else if (cmd == "clear") cmd_clear(a);
   else if (cmd == "run") cmd_run(a);
   else if (cmd == "variable") cmd_variable(a);
   else if (cmd == "print") cmd_print(a);

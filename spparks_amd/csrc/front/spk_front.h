@@ -49,7 +49,7 @@ struct Dump {
 
 class Spparks {
  public:
-  Spparks(FILE *screen, FILE *logfile);
+  Spparks(FILE *screen, FILE *logfile, bool banner = true);
   ~Spparks();
   Spparks(const Spparks &) = delete;
   Spparks &operator=(const Spparks &) = delete;
@@ -94,6 +94,7 @@ class Spparks {
   void cmd_read_sites(const std::vector<std::string> &a);
   void cmd_undump(const std::vector<std::string> &a);
   void cmd_dump_modify(const std::vector<std::string> &a);
+  void cmd_clear(const std::vector<std::string> &a);
   void cmd_run(const std::vector<std::string> &a);
   void cmd_variable(const std::vector<std::string> &a);
   void cmd_print(const std::vector<std::string> &a);

@@ -29,6 +29,8 @@ def run_script(tmp_path, text, *args):
 
 @pytest.mark.parametrize("text,msg", [
     ("seed 0\n", "Illegal seed command"),
+    ("clear 1\n", "Illegal clear command"),
+    ("app_style potts 10\nclear\nrun 1.0\n", "Run command before app_style set"),
     (PRE + "dump 1 text 1.0 f.txt\ndump_modify 2 delay 1\n", "Could not find dump ID in dump_modify command"),
     (PRE + "dump 1 text 1.0 f.txt\ndump_modify 1\n", "Illegal dump_modify command"),
     (PRE + "dump 1 text 1.0 f.txt\ndump_modify 1 delta 0\n", "Illegal dump_modify command"),

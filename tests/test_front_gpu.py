@@ -161,6 +161,32 @@ def test_run_options_log_undump_and_stats_delay(orc, tmp_path):
     assert np.array_equal(vals[:, 1], spin)
 
 
+def test_clear_between_two_simulations(orc, tmp_path):
+    """clear (src/input.cpp:524-529) destroys the first simulation, lattice
+    and dumps included, and keeps the variables: the second simulation in the
+    same script runs as it would alone, master seed and all."""
+    p = tmp_path / "in.two"
+    p.write_text("variable Q index 32\n"
+                 "seed 5\napp_style ising\ndimension 2\nlattice sq/4n 1.0\n"
+                 "region box block 0 16 0 12 -0.5 0.5\ncreate_box box\ncreate_sites box\n"
+                 "set site range 1 2\nsweep color/strict\ntemperature 1.5\ndump 1 text 1.0 first.txt id site\n"
+                 "run 2.0\nclear\n"
+                 "seed 9\napp_style potts ${Q}\ndimension 3\nlattice sc/26n 1.0\n"
+                 "region box block 0 8 0 6 0 4\ncreate_box box\ncreate_sites box\n"
+                 "set site range 1 ${Q}\nsweep color/strict\ntemperature 0.7\ndiag_style energy\n"
+                 "stats 0.125\nrun 0.375\n")
+    out = run_exe(tmp_path, str(p))
+    rows = stats_rows(out)
+    # Q = 32: a sweep is 1/32 of a time unit, exact in binary
+    d = orc.make_desc("potts", "sc/26n", 8, 6, 4, 32, 0.7)
+    spin, seeds = orc.script_init(d, 9, 1, 32)
+    nacc, eng = orc.sweeps(d, spin, seeds, 12)
+    assert [r[3] for r in rows] == [0, 4, 8, 12]
+    assert [r[1] for r in rows[1:]] == [int(nacc[k - 1]) for k in (4, 8, 12)]
+    assert [r[4] for r in rows[1:]] == [float("%10g" % eng[k - 1]) for k in (4, 8, 12)]
+    assert len(open(tmp_path / "first.txt").read().split("ITEM: TIMESTEP\n")) == 4   # t = 0, 1, 2
+
+
 def test_dump_modify_schedules(orc, tmp_path):
     """dump_modify delay / first / pad and logfreq (src/dump.cpp:644-790)
     with the reference's schedule (Output::setup / compute, next_time,
