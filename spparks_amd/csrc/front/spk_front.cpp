@@ -86,12 +86,17 @@ Spparks::Spparks(FILE *screen, FILE *logfile, bool banner) : screen_(screen), lo
 void Spparks::cmd_clear(const std::vector<std::string> &a) {
   if (!a.empty()) throw Error("Illegal clear command");
   FILE *screen = screen_, *logfile = logfile_;
-  std::map<std::string, std::string> vars = std::move(vars_);
-  const bool quiet = quiet_;
+  std::map<std::string, Var> vars = std::move(vars_);
+  const bool quiet = quiet_, label_active = label_active_, jump_skip = jump_skip_;
+  const std::string label = label_, jump_to = jump_to_;
   this->~Spparks();
   new (this) Spparks(screen, logfile, false);
   vars_ = std::move(vars);
   quiet_ = quiet;
+  label_active_ = label_active;
+  jump_skip_ = jump_skip;
+  label_ = label;
+  jump_to_ = jump_to;
 }
 
 Spparks::~Spparks() {
@@ -127,32 +132,48 @@ void Spparks::hip(int rc) {
 void Spparks::file(const char *path) {
   FILE *fp = std::fopen(path, "r");
   if (!fp) throw Error(std::string("Cannot open input script ") + path);
-  try {
-    file(fp);
-  } catch (...) {
-    std::fclose(fp);
-    throw;
-  }
-  std::fclose(fp);
+  file(fp, true);
 }
 
-// Input::file (src/input.cpp:121-200): lines ending in '&' continue
-void Spparks::file(FILE *fp) {
+void Spparks::file(FILE *fp) { file(fp, false); }
+
+// Input::file (src/input.cpp:121-200): lines ending in '&' continue.  A
+// jump replaces the file being read (closing it when it was opened here).
+void Spparks::file(FILE *fp, bool owned) {
+  struct Closer {
+    FILE *&f;
+    bool &own;
+    ~Closer() {
+      if (own && f) std::fclose(f);
+    }
+  } closer{fp, owned};
   std::string line;
   char buf[4096];
-  while (std::fgets(buf, sizeof(buf), fp)) {
-    line += buf;
-    if (!line.empty() && line.back() != '\n' && !std::feof(fp)) continue;   // long line
-    size_t e = line.find_last_not_of(" \t\r\n");
-    if (e != std::string::npos && line[e] == '&') {
-      line.erase(e);
-      line += ' ';
-      continue;
+  for (;;) {
+    while (std::fgets(buf, sizeof(buf), fp)) {
+      line += buf;
+      if (!line.empty() && line.back() != '\n' && !std::feof(fp)) continue;   // long line
+      size_t e = line.find_last_not_of(" \t\r\n");
+      if (e != std::string::npos && line[e] == '&') {
+        line.erase(e);
+        line += ' ';
+        continue;
+      }
+      one(line);
+      line.clear();
+      if (!jump_to_.empty()) break;
     }
-    one(line);
+    if (jump_to_.empty() && !line.empty()) one(line);
+    if (jump_to_.empty()) return;
+    // Input::jump (src/input.cpp:621-651): read on from the named file
+    const std::string path = jump_to_;
+    jump_to_.clear();
     line.clear();
+    if (owned) std::fclose(fp);
+    fp = std::fopen(path.c_str(), "r");
+    owned = fp != nullptr;
+    if (!fp) throw Error("Cannot open input script " + path);
   }
-  if (!line.empty()) one(line);
 }
 
 std::string Spparks::substitute(const std::string &in) const {
@@ -177,7 +198,11 @@ std::string Spparks::substitute(const std::string &in) const {
     }
     auto it = vars_.find(name);
     if (it == vars_.end()) throw Error("Substitution for illegal variable " + name);
-    out += it->second;
+    // Variable::retrieve (src/variable.cpp:330-360): a loop variable is its
+    // 1-based counter, the others their current value
+    const Var &v = it->second;
+    if (v.index >= v.num) throw Error("Substitution for illegal variable " + name);
+    out += v.loop ? std::to_string(v.index + 1) : v.data[v.index];
   }
   return out;
 }
@@ -196,6 +221,8 @@ std::string Spparks::one(const std::string &raw) {
   if (w.empty()) return "";
   const std::string cmd = w[0];
   std::vector<std::string> a(w.begin() + 1, w.end());
+  // scanning for a jump's label: every other command is skipped
+  if (label_active_ && cmd != "label") return cmd;
   if (cmd == "seed") cmd_seed(a);
   else if (cmd == "app_style") cmd_app_style(a);
   else if (cmd == "dimension") cmd_dimension(a);
@@ -212,6 +239,13 @@ std::string Spparks::one(const std::string &raw) {
   else if (cmd == "undump") cmd_undump(a);
   else if (cmd == "dump_modify") cmd_dump_modify(a);
   else if (cmd == "clear") cmd_clear(a);
+  else if (cmd == "next") cmd_next(a);
+  else if (cmd == "jump") cmd_jump(a);
+  else if (cmd == "label") {
+    // Input::label (src/input.cpp:655-660)
+    if (a.size() != 1) throw Error("Illegal label command");
+    if (label_active_ && label_ == a[0]) label_active_ = false;
+  }
   else if (cmd == "run") cmd_run(a);
   else if (cmd == "variable") cmd_variable(a);
   else if (cmd == "print") cmd_print(a);
@@ -256,7 +290,12 @@ std::string Spparks::one(const std::string &raw) {
   return cmd;
 }
 
-void Spparks::set_variable(const std::string &name, const std::string &value) { vars_[name] = value; }
+void Spparks::set_variable(const std::string &name, const std::string &value) {
+  Var v;
+  v.data = {value};
+  v.num = 1;
+  vars_[name] = v;
+}
 
 void Spparks::need_app(const char *cmd) const {
   if (app_.empty()) throw Error(std::string("Command ") + cmd + " before app_style set");
@@ -641,23 +680,77 @@ void Spparks::cmd_dump_modify(const std::vector<std::string> &a) {
   }
 }
 
+// Variable::set (src/variable.cpp:80-160): an existing variable is kept
+// (so -var on the command line wins), except that equal re-defines
 void Spparks::cmd_variable(const std::vector<std::string> &a) {
   if (a.size() < 3) throw Error("Illegal variable command");
-  if (a[1] != "index" && a[1] != "string" && a[1] != "equal")
-    throw Error("variable style " + a[1] + " is outside this build (index, string, equal numbers)");
-  std::string value = a[2];
-  if (a[1] == "equal") {
+  const std::string &style = a[1];
+  if (style != "index" && style != "loop" && style != "string" && style != "equal")
+    throw Error("variable style " + style + " is outside this build (index, loop, equal numbers)");
+  auto it = vars_.find(a[0]);
+  if (it != vars_.end() && style != "equal") return;
+  Var v;
+  if (style == "index" || style == "string") {
+    if (style == "string" && a.size() != 3) throw Error("Illegal variable command");
+    v.data.assign(a.begin() + 2, a.end());
+    v.num = (int)v.data.size();
+  } else if (style == "loop") {
+    if (a.size() != 3) throw Error("Illegal variable command");
+    v.loop = true;
+    v.num = std::atoi(a[2].c_str());
+  } else {
+    if (a.size() != 3) throw Error("Illegal variable command");
+    if (it != vars_.end() && !it->second.equal) throw Error("Cannot redefine variable as a different style");
     char *end = nullptr;
-    const double v = std::strtod(a[2].c_str(), &end);
+    const double x = std::strtod(a[2].c_str(), &end);
     if (*end) throw Error("equal-style formulas are outside this build; give a number");
     // an equal-style variable substitutes its value as "%.10g"
     // (Variable::retrieve, src/variable.cpp:349-352)
     char buf[64];
-    std::snprintf(buf, sizeof buf, "%.10g", v);
-    value = buf;
+    std::snprintf(buf, sizeof buf, "%.10g", x);
+    v.data = {buf};
+    v.num = 1;
+    v.equal = true;
   }
-  if (a[1] == "index" && vars_.count(a[0])) return;   // -var on the command line wins
-  vars_[a[0]] = value;
+  vars_[a[0]] = v;
+}
+
+// Input::next_command, Variable::next (src/variable.cpp:242-280): every
+// named index or loop variable advances; one that runs out is removed and
+// the next jump is skipped, which ends a loop
+void Spparks::cmd_next(const std::vector<std::string> &a) {
+  if (a.empty()) throw Error("Illegal next command");
+  for (const auto &n : a)
+    if (!vars_.count(n)) throw Error("Invalid variable in next command");
+  const Var &first = vars_.at(a[0]);
+  for (const auto &n : a)
+    if (vars_.at(n).loop != first.loop || vars_.at(n).equal != first.equal)
+      throw Error("All variables in next command must be same style");
+  if (first.equal) throw Error("Invalid variable style with next command");
+  bool done = false;
+  for (const auto &n : a) {
+    Var &v = vars_.at(n);
+    if (++v.index >= v.num) {
+      done = true;
+      vars_.erase(n);
+    }
+  }
+  if (done) jump_skip_ = true;
+}
+
+// Input::jump (src/input.cpp:621-651): read on from the start of the named
+// file, skipping to the label when one is given
+void Spparks::cmd_jump(const std::vector<std::string> &a) {
+  if (a.empty() || a.size() > 2) throw Error("Illegal jump command");
+  if (jump_skip_) {
+    jump_skip_ = false;
+    return;
+  }
+  jump_to_ = a[0];
+  if (a.size() == 2) {
+    label_active_ = true;
+    label_ = a[1];
+  }
 }
 
 void Spparks::cmd_print(const std::vector<std::string> &a) {

@@ -57,6 +57,7 @@ class Spparks {
   // executes every command of an input script (Input::file)
   void file(const char *path);
   void file(FILE *fp);
+  void file(FILE *fp, bool owned);
   // executes one command line (Input::one); returns the command word
   std::string one(const std::string &line);
   // -var name value (Input's variable table, index style)
@@ -95,6 +96,8 @@ class Spparks {
   void cmd_undump(const std::vector<std::string> &a);
   void cmd_dump_modify(const std::vector<std::string> &a);
   void cmd_clear(const std::vector<std::string> &a);
+  void cmd_next(const std::vector<std::string> &a);
+  void cmd_jump(const std::vector<std::string> &a);
   void cmd_run(const std::vector<std::string> &a);
   void cmd_variable(const std::vector<std::string> &a);
   void cmd_print(const std::vector<std::string> &a);
@@ -120,7 +123,18 @@ class Spparks {
 
   FILE *screen_, *logfile_;
   bool quiet_ = false;
-  std::map<std::string, std::string> vars_;
+  // the variable table (Variable, src/variable.cpp): index, loop and
+  // equal (a number) styles; string is a one-value index
+  struct Var {
+    bool loop = false;
+    std::vector<std::string> data;   // index values; one formatted number for equal
+    int num = 0, index = 0;
+    bool equal = false;
+  };
+  std::map<std::string, Var> vars_;
+  // Input's jump / label / next state (src/input.cpp:190-205, :621-662)
+  bool label_active_ = false, jump_skip_ = false;
+  std::string label_, jump_to_;
 
   std::unique_ptr<spk::MasterRng> ranmaster_;
 

@@ -29,6 +29,14 @@ def run_script(tmp_path, text, *args):
 
 @pytest.mark.parametrize("text,msg", [
     ("seed 0\n", "Illegal seed command"),
+    ("next\n", "Illegal next command"),
+    ("next zz\n", "Invalid variable in next command"),
+    ("variable a index 1 2\nvariable b loop 2\nnext a b\n", "All variables in next command must be same style"),
+    ("variable e equal 1\nnext e\n", "Invalid variable style with next command"),
+    ("label\n", "Illegal label command"),
+    ("jump\n", "Illegal jump command"),
+    ("jump no_such_file.in\n", "Cannot open input script no_such_file.in"),
+    ("variable n loop 0\nprint ${n}\n", "Substitution for illegal variable n"),
     ("clear 1\n", "Illegal clear command"),
     ("app_style potts 10\nclear\nrun 1.0\n", "Run command before app_style set"),
     (PRE + "dump 1 text 1.0 f.txt\ndump_modify 2 delay 1\n", "Could not find dump ID in dump_modify command"),
@@ -134,3 +142,18 @@ def test_equal_style_variables_substitute_ten_digits(tmp_path):
                                    "print ${T} ${D} ${N} ${S}\n")
     assert rc == 0, out
     assert "2.269185314 0.0001 250000 0.12345678901234" in out
+
+
+def test_loops_with_jump_label_next(tmp_path):
+    """An index and a loop variable stepped by next, the script re-read from
+    its label by jump until the variable runs out, which skips the final
+    jump (src/input.cpp:621-662, src/variable.cpp:242-280); then a jump on
+    to another file."""
+    (tmp_path / "in.b").write_text("print \"in b\"\n")
+    rc, out = run_script(tmp_path, "variable a index 3 5 7\nvariable i loop 2\nlabel top\n"
+                                   "print \"a = ${a} i = ${i}\"\nnext a\nnext i\njump in.test top\n"
+                                   "print \"after the loop\"\njump in.b\nprint \"never\"\n")
+    assert rc == 0, out
+    lines = [l for l in out.splitlines() if l.startswith(("a =", "after", "in b", "never"))]
+    # i runs out after two passes, which skips the jump; a is still at 7
+    assert lines == ["a = 3 i = 1", "a = 5 i = 2", "after the loop", "in b"]
