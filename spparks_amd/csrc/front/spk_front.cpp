@@ -759,11 +759,13 @@ void Spparks::cmd_print(const std::vector<std::string> &a) {
   message("%s\n", s.c_str());
 }
 
+// Input::log (src/input.cpp:664-686)
 void Spparks::cmd_log(const std::vector<std::string> &a) {
   if (a.size() != 1) throw Error("Illegal log command");
   if (logfile_ && logfile_ != stdout && logfile_ != stderr) std::fclose(logfile_);
   logfile_ = a[0] == "none" ? nullptr : std::fopen(a[0].c_str(), "w");
-  if (a[0] != "none" && !logfile_) throw Error("Cannot open logfile");
+  if (a[0] != "none" && !logfile_) throw Error("Cannot open logfile " + a[0]);
+  if (logfile_) std::fprintf(logfile_, "SPPARKS (spk_hip) -- continued logfile from same input script\n");
 }
 
 // ------------------------------------------------------------ run

@@ -157,3 +157,13 @@ def test_loops_with_jump_label_next(tmp_path):
     lines = [l for l in out.splitlines() if l.startswith(("a =", "after", "in b", "never"))]
     # i runs out after two passes, which skips the jump; a is still at 7
     assert lines == ["a = 3 i = 1", "a = 5 i = 2", "after the loop", "in b"]
+
+
+def test_log_command_continues_in_a_new_file(tmp_path):
+    rc, out = run_script(tmp_path, "print one\nlog second.log\nprint two\n")
+    assert rc == 0, out
+    text = (tmp_path / "second.log").read_text()
+    assert text.startswith("SPPARKS") and "continued logfile from same input script" in text
+    assert "two" in text and "one" not in text
+    rc, out = run_script(tmp_path, "log no_such_dir/x.log\n")
+    assert rc != 0 and "Cannot open logfile no_such_dir/x.log" in out
