@@ -89,6 +89,7 @@ void Spparks::cmd_clear(const std::vector<std::string> &a) {
   std::map<std::string, Var> vars = std::move(vars_);
   const bool quiet = quiet_, label_active = label_active_, jump_skip = jump_skip_;
   const std::string label = label_, jump_to = jump_to_;
+  const int depth = file_depth_;
   this->~Spparks();
   new (this) Spparks(screen, logfile, false);
   vars_ = std::move(vars);
@@ -97,6 +98,7 @@ void Spparks::cmd_clear(const std::vector<std::string> &a) {
   jump_skip_ = jump_skip;
   label_ = label;
   jump_to_ = jump_to;
+  file_depth_ = depth;
 }
 
 Spparks::~Spparks() {
@@ -143,10 +145,13 @@ void Spparks::file(FILE *fp, bool owned) {
   struct Closer {
     FILE *&f;
     bool &own;
+    int &depth;
     ~Closer() {
       if (own && f) std::fclose(f);
+      depth--;
     }
-  } closer{fp, owned};
+  } closer{fp, owned, file_depth_};
+  file_depth_++;
   std::string line;
   char buf[4096];
   for (;;) {
@@ -742,6 +747,7 @@ void Spparks::cmd_next(const std::vector<std::string> &a) {
 // file, skipping to the label when one is given
 void Spparks::cmd_jump(const std::vector<std::string> &a) {
   if (a.empty() || a.size() > 2) throw Error("Illegal jump command");
+  if (file_depth_ == 0) throw Error("Jump command outside an input script");
   if (jump_skip_) {
     jump_skip_ = false;
     return;

@@ -134,6 +134,7 @@ class Spparks {
   std::map<std::string, Var> vars_;
   // Input's jump / label / next state (src/input.cpp:190-205, :621-662)
   bool label_active_ = false, jump_skip_ = false;
+  int file_depth_ = 0;   // input files being read (a jump needs one)
   std::string label_, jump_to_;
 
   std::unique_ptr<spk::MasterRng> ranmaster_;

@@ -167,3 +167,23 @@ def test_log_command_continues_in_a_new_file(tmp_path):
     assert "two" in text and "one" not in text
     rc, out = run_script(tmp_path, "log no_such_dir/x.log\n")
     assert rc != 0 and "Cannot open logfile no_such_dir/x.log" in out
+
+
+def test_clear_inside_a_loop_keeps_the_loop(tmp_path):
+    """clear keeps the input state (variables, the file being read, a
+    pending label), so a loop around clear runs every pass."""
+    rc, out = run_script(tmp_path, "variable i loop 3\nlabel top\nclear\nprint \"pass ${i}\"\n"
+                                   "next i\njump in.test top\nprint end\n")
+    assert rc == 0, out
+    assert [l for l in out.splitlines() if l.startswith(("pass", "end"))] == ["pass 1", "pass 2", "pass 3", "end"]
+
+
+def test_jump_from_the_library_is_refused():
+    """A jump needs an input file to read on from; through spparks_command
+    it is an error, and the instance goes on."""
+    from spparks_amd.spparks import spparks, SpparksError
+    spk = spparks(cmdargs=["-screen", "none", "-log", "none"])
+    with pytest.raises(SpparksError, match="Jump command outside an input script"):
+        spk.command("jump in.anything")
+    spk.command("variable x index 4")
+    spk.close()
