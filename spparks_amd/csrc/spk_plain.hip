@@ -1,5 +1,5 @@
 // spk_plain.hip -- libspk_hip.so, single-lattice path: the per-colour strict
-// sweep kernel, the fused energy / Naccept reductions, device-side initial
+// sweep kernels, the fused energy / Naccept reductions, device-side initial
 // state (set site range, tagreset seeds via Park-Miller jump-ahead), layout
 // conversion, and the C ABI declared in include/spk_hip.h.
 #include <hip/hip_runtime.h>
@@ -55,25 +55,17 @@ __global__ void k_bits_unpack(const uint32_t *__restrict__ bits, Geom g, BitGeom
   bits_unpack_word(spin, g, c, b, w, bits[q]);
 }
 
-// seeds <-> shadow seeds, one thread per shadow slot; padding slots hold 1
+// seeds <-> shadow seeds, one thread per 16-byte group (bits_seed_group_in
+// / _out): each wavefront reads or writes 1 KiB of consecutive canonical
+// bytes and 128-byte runs of the shadow
 __global__ void k_seeds_to_shadow(const int32_t *__restrict__ seed, Geom g, BitGeom bg, int32_t *__restrict__ sseed) {
-  const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (s >= 2 * bg.seedc) return;
-  int c, i;
-  uint32_t t;
-  bit_seed_slot(bg, s, c, t, i);
-  const int64_t home = bit_seed_home(g, bg, c, t, i);
-  sseed[s] = home < 0 ? 1 : seed[home];
+  const int64_t X = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (X < bg.seedc / 2) bits_seed_group_in(seed, g, bg, X, sseed);
 }
 
 __global__ void k_seeds_from_shadow(const int32_t *__restrict__ sseed, Geom g, BitGeom bg, int32_t *__restrict__ seed) {
-  const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (s >= 2 * bg.seedc) return;
-  int c, i;
-  uint32_t t;
-  bit_seed_slot(bg, s, c, t, i);
-  const int64_t home = bit_seed_home(g, bg, c, t, i);
-  if (home >= 0) seed[home] = sseed[s];
+  const int64_t X = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (X < bg.seedc / 2) bits_seed_group_out(sseed, g, bg, X, seed);
 }
 
 // One colour pass: every site of colour `color` performs one strict event.
@@ -161,86 +153,6 @@ __global__ __launch_bounds__(1024) void k_sweep_variant(T *__restrict__ spin, in
   block_accumulate_pack(cnt.nacc, cnt.de, acc, blockIdx.x & (kAccSlots - 1));
 }
 
-// S whole sweeps of a 2d lattice per launch, one extended tile per
-// workgroup in LDS (see FusedTile).  Reads one state buffer, writes the
-// other; per-sweep Naccept / dE go to acc rows 0..S-1.
-template <int LAT, typename T, int S, bool TWO>
-__global__ __launch_bounds__(512) void k_sweep_fused2d(const T *__restrict__ spin_in, const int32_t *__restrict__ seed_in,
-                                                       T *__restrict__ spin_out, int32_t *__restrict__ seed_out,
-                                                       Geom g, Rule r, int TX, int TY, int ntiles, int lds_sites,
-                                                       unsigned long long *__restrict__ acc) {
-  constexpr int K = LAT == SPK_HIP_SQ_4N ? 2 : 4;
-  constexpr int H = K * S;
-  extern __shared__ int32_t lds_seed[];               // lds_sites seeds, then lds_sites spins
-  __shared__ int s_thr[32];
-  const int tile = xcd_tile(blockIdx.x, gridDim.x);
-  if (tile >= ntiles) return;                          // whole workgroup, before any barrier
-  const FusedTile t = fused_tile(g, TX, TY, H, tile);
-  int32_t *sd = lds_seed;
-  T *sp = reinterpret_cast<T *>(lds_seed + lds_sites);
-  if (threadIdx.x < 28) s_thr[threadIdx.x] = r.thr[threadIdx.x];
-  const int gx0 = t.X0 - H, gy0 = t.Y0 - H;
-  const int ext = t.EW * t.EH;
-  {
-    int ly = threadIdx.x / t.EW, lx = threadIdx.x - ly * t.EW;   // then stepped without division
-    const int step_y = blockDim.x / t.EW, step_x = blockDim.x - step_y * t.EW;
-    for (int idx = threadIdx.x; idx < ext; idx += blockDim.x) {
-      const int64_t at = blocked_index2d<LAT>(g, wrap_near(gx0 + lx, g.nx), wrap_near(gy0 + ly, g.ny));
-      sp[idx] = spin_in[at];
-      sd[idx] = seed_in[at];
-      lx += step_x;
-      ly += step_y;
-      if (lx >= t.EW) {
-        lx -= t.EW;
-        ly++;
-      }
-    }
-  }
-  __syncthreads();
-  int nacc[S], de[S];
-#pragma unroll
-  for (int sw = 0; sw < S; sw++) {
-    nacc[sw] = 0;
-    de[sw] = 0;
-#pragma unroll
-    for (int c = 0; c < K; c++) {
-      const FusedPass f = fused_pass<LAT>(t, sw * K + c + 1, c);
-      for (int q = threadIdx.x; q < f.nslots; q += blockDim.x) {
-        int lx, ly;
-        if (!fused_site<LAT>(t, f, q, lx, ly)) continue;
-        const SiteOutcome o = fused_event<LAT, T, TWO>(sp, sd, t.EW, lx, ly, r, s_thr);
-        if (fused_in_core(t, lx, ly)) {
-          nacc[sw] += o.changed;
-          de[sw] += o.dE;
-        }
-      }
-      __syncthreads();
-    }
-  }
-  {
-    const int core = t.cw * t.ch;
-    int cy = threadIdx.x / t.cw, cx = threadIdx.x - cy * t.cw;
-    const int step_y = blockDim.x / t.cw, step_x = blockDim.x - step_y * t.cw;
-    for (int idx = threadIdx.x; idx < core; idx += blockDim.x) {
-      const int at = (H + cy) * t.EW + H + cx;
-      const int64_t gp = blocked_index2d<LAT>(g, t.X0 + cx, t.Y0 + cy);
-      spin_out[gp] = sp[at];
-      seed_out[gp] = sd[at];
-      cx += step_x;
-      cy += step_y;
-      if (cx >= t.cw) {
-        cx -= t.cw;
-        cy++;
-      }
-    }
-  }
-#pragma unroll
-  for (int sw = 0; sw < S; sw++) {
-    __syncthreads();   // the previous reduction's LDS slots are free again
-    block_accumulate(nacc[sw], de[sw], acc + (size_t)sw * kAccRow, blockIdx.x & (kAccSlots - 1));
-  }
-}
-
 // ===================================================================== host
 
 constexpr int kBitSites = 32;   // default sites per thread on bit planes
@@ -256,10 +168,9 @@ struct spk_hip_lattice {
   hipStream_t stream = nullptr;
   void *spin = nullptr;                // slots spins, colour-blocked, stype
   int32_t *seed = nullptr;             // slots seeds, colour-blocked
-  void *spin_alt = nullptr;            // second state buffer for sweep-fused launches
-  int32_t *seed_alt = nullptr;
-  int fuse = 0;                        // sweeps per fused launch (0: per-colour kernel only)
-  // bit planes of a two-state sq/4n lattice (spk_bits.h), live during a sweeps call
+  // bit planes of a two-state sq/4n lattice (spk_bits.h).  Once a sweeps
+  // call has packed them they hold the state (bits_active) until a call
+  // that needs the byte layout unpacks them (bits_leave).
   int bitplanes = kBitSites;           // sites per thread on the planes (0: planes off)
   bool bits_active = false;
   BitGeom bg{};
@@ -269,7 +180,6 @@ struct spk_hip_lattice {
   // lattices that fit one workgroup's LDS run every sweep in one launch
   int resident = 1;
   size_t max_lds = 0;                  // LDS bytes a workgroup may use
-  int tile_x = 0, tile_y = 0;          // fused core tile
   int32_t *stage = nullptr;            // N int32, natural order (transfers)
   unsigned long long *acc = nullptr;   // [cap][kAccRow]: per-sweep slots
   unsigned long long *accc = nullptr;  // [cap][2]: rows folded to (Naccept, dE)
@@ -281,13 +191,15 @@ struct spk_hip_lattice {
   bool energy_valid = false;
   int64_t energy = 0;                  // total energy, exact integer
   std::vector<unsigned long long> hacc;
-  // graph replay of whole chunks of sweeps
-  int chunk = 32;                      // sweeps per captured chunk (0: eager launches only)
-  struct Chunk {
-    void *spin_start;                  // state buffer the chunk starts from
+  // graph replay: a call of n sweeps replays n / chunk graphs of `chunk`
+  // sweeps, then one graph per set bit of the remainder (16, 8, ... 1)
+  int chunk = 32;                      // sweeps per full graph (0: eager launches only)
+  struct Piece {
+    int sweeps;                        // sweeps the graph runs
+    bool bits;                         // captured on the bit planes
     hipGraphExec_t exec;
   };
-  std::vector<Chunk> graphs;
+  std::vector<Piece> graphs;
   unsigned long long *pinned = nullptr;   // per-sweep rows of a call, host pinned
   int pinned_rows = 0;
 };
@@ -295,6 +207,7 @@ struct spk_hip_lattice {
 namespace {
 
 constexpr int kInitialAccRows = 1024;
+constexpr int kMaxChunk = 1024;   // sweeps per captured graph
 constexpr int kVec = 4;   // sites per thread in the sweep kernel
 
 unsigned grid_for(int64_t n, int bs) { return (unsigned)((n + bs - 1) / bs); }
@@ -370,59 +283,6 @@ int launch_sweep(spk_hip_lattice *h, int color, unsigned long long *acc) {
   return check_hip(hipGetLastError(), "sweep kernel launch");
 }
 
-constexpr int kFusedBlock = 512;
-constexpr size_t kFusedLdsMax = 64 * 1024;
-
-// picks the fused tile for a 2d lattice: S sweeps per launch, halo K*S, the
-// largest core that keeps the extended tile within 64 KB of LDS
-void choose_fusion(spk_hip_lattice *h) {
-  h->fuse = 0;
-  if (h->g.lat != SPK_HIP_SQ_4N && h->g.lat != SPK_HIP_SQ_8N) return;
-  if (variant_app(h->d.app)) return;   // fused tiles run the potts / ising rule only
-  const int S = h->g.lat == SPK_HIP_SQ_4N ? 4 : 2;
-  const int H = (h->g.lat == SPK_HIP_SQ_4N ? 2 : 4) * S;
-  const size_t per = 4 + spin_bytes(h->stype);
-  // cores sized so three workgroups' tiles fit one CU's 160 KB of LDS
-  const int tys[] = {56, 48, 40};
-  const int txs[] = {128, 96, 64};
-  const int TY = tys[h->stype], TX = txs[h->stype];
-  const size_t lds = (size_t)(std::min(TX, h->g.nx) + 2 * H) * (std::min(TY, h->g.ny) + 2 * H) * per;
-  if (lds > kFusedLdsMax) return;
-  h->fuse = S;
-  h->tile_x = TX;
-  h->tile_y = TY;
-}
-
-int launch_fused(spk_hip_lattice *h, unsigned long long *acc) {
-  const int S = h->fuse, H = (h->g.lat == SPK_HIP_SQ_4N ? 2 : 4) * S;
-  const int TX = h->tile_x, TY = h->tile_y;
-  const int ntiles = ((h->g.nx + TX - 1) / TX) * ((h->g.ny + TY - 1) / TY);
-  const unsigned grid = (unsigned)((ntiles + kNumXcd - 1) / kNumXcd * kNumXcd);
-  const int lds_sites = (std::min(TX, h->g.nx) + 2 * H) * (std::min(TY, h->g.ny) + 2 * H);
-  const size_t lds = (size_t)lds_sites * (4 + spin_bytes(h->stype));
-  const bool two = h->rule.nspins == 2;
-  with_type(h->stype, [&]<typename T>() {
-    const T *si = static_cast<const T *>(h->spin);
-    T *so = static_cast<T *>(h->spin_alt);
-    auto go = [&](auto kern) {
-      hipLaunchKernelGGL(kern, dim3(grid), dim3(kFusedBlock), lds, h->stream, si, h->seed, so, h->seed_alt, h->g,
-                         h->rule, TX, TY, ntiles, lds_sites, acc);
-    };
-    if (h->g.lat == SPK_HIP_SQ_4N) {
-      if (two) go(k_sweep_fused2d<SPK_HIP_SQ_4N, T, 4, true>);
-      else go(k_sweep_fused2d<SPK_HIP_SQ_4N, T, 4, false>);
-    } else {
-      if (two) go(k_sweep_fused2d<SPK_HIP_SQ_8N, T, 2, true>);
-      else go(k_sweep_fused2d<SPK_HIP_SQ_8N, T, 2, false>);
-    }
-  });
-  int rc = check_hip(hipGetLastError(), "fused sweep launch");
-  if (rc) return rc;
-  std::swap(h->spin, h->spin_alt);
-  std::swap(h->seed, h->seed_alt);
-  return SPK_HIP_OK;
-}
-
 void drop_graphs(spk_hip_lattice *h) {
   for (auto &c : h->graphs) (void)hipGraphExecDestroy(c.exec);
   h->graphs.clear();
@@ -431,7 +291,7 @@ void drop_graphs(spk_hip_lattice *h) {
 // Two-state sq/4n lattices on the per-colour path sweep on bit planes.
 bool bits_eligible(const spk_hip_lattice *h) {
   return h->bitplanes && h->g.lat == SPK_HIP_SQ_4N && h->rule.nspins == 2 && h->stype == kU8 &&
-         (!variant_app(h->d.app) || h->d.app == SPK_HIP_ISING_SINGLE) && !h->fuse;
+         (!variant_app(h->d.app) || h->d.app == SPK_HIP_ISING_SINGLE);
 }
 
 // Packs the state into the planes (allocated on first use) and routes the
@@ -458,8 +318,8 @@ int bits_enter(spk_hip_lattice *h) {
   const int64_t words = 2 * (int64_t)h->g.hy * h->bg.pw;
   hipLaunchKernelGGL(k_bits_pack, dim3(grid_for(words, 256)), dim3(256), 0, h->stream,
                      static_cast<const uint8_t *>(h->spin), h->g, h->bg, h->bits);
-  hipLaunchKernelGGL(k_seeds_to_shadow, dim3(grid_for(2 * h->bg.seedc, 256)), dim3(256), 0, h->stream, h->seed, h->g,
-                     h->bg, h->sseed);
+  hipLaunchKernelGGL(k_seeds_to_shadow, dim3(grid_for(h->bg.seedc / 2, 256)), dim3(256), 0, h->stream, h->seed,
+                     h->g, h->bg, h->sseed);
   int rc = check_hip(hipGetLastError(), "bit-plane pack");
   if (!rc) h->bits_active = true;
   return rc;
@@ -471,8 +331,8 @@ int bits_leave(spk_hip_lattice *h) {
   const int64_t words = 2 * (int64_t)h->g.hy * h->bg.pw;
   hipLaunchKernelGGL(k_bits_unpack, dim3(grid_for(words, 256)), dim3(256), 0, h->stream, h->bits, h->g, h->bg,
                      static_cast<uint8_t *>(h->spin));
-  hipLaunchKernelGGL(k_seeds_from_shadow, dim3(grid_for(2 * h->bg.seedc, 256)), dim3(256), 0, h->stream, h->sseed,
-                     h->g, h->bg, h->seed);
+  hipLaunchKernelGGL(k_seeds_from_shadow, dim3(grid_for(h->bg.seedc / 2, 256)), dim3(256), 0, h->stream,
+                     h->sseed, h->g, h->bg, h->seed);
   return check_hip(hipGetLastError(), "bit-plane unpack");
 }
 
@@ -500,11 +360,7 @@ int ensure_pinned(spk_hip_lattice *h, int rows) {
 // launches of sweeps whose accumulator rows start at acc; returns launches
 int launch_sweeps(spk_hip_lattice *h, int n, unsigned long long *acc, int64_t *nlaunch) {
   int rc;
-  int s = 0;
-  if (h->fuse)   // whole groups of S sweeps through the fused kernel
-    for (; s + h->fuse <= n; s += h->fuse, (*nlaunch)++)
-      if ((rc = launch_fused(h, acc + (size_t)s * kAccRow))) return rc;
-  for (; s < n; s++)
+  for (int s = 0; s < n; s++)
     for (int c = 0; c < h->g.K; c++, (*nlaunch)++)
       if ((rc = launch_sweep(h, c, acc + (size_t)s * kAccRow))) return rc;
   return SPK_HIP_OK;
@@ -520,7 +376,7 @@ int launch_fold(spk_hip_lattice *h, const unsigned long long *acc, int rows, uns
 // LDS bytes of the resident kernel's copy of the state (spins, then seeds
 // at a 16-byte boundary), or 0 when the lattice does not fit one workgroup.
 size_t resident_bytes(const spk_hip_lattice *h) {
-  if (variant_app(h->d.app) || h->fuse || !h->resident) return 0;
+  if (variant_app(h->d.app) || !h->resident) return 0;
   const size_t slots = (size_t)h->g.K * h->g.Mp;
   const size_t bytes = (slots * spin_bytes(h->stype) + 15) / 16 * 16 + slots * sizeof(int32_t);
   return bytes <= h->max_lds ? bytes : 0;
@@ -600,40 +456,49 @@ int finish_sweeps(spk_hip_lattice *h, int nsweeps, int64_t nlaunch, int64_t e0, 
 }
 
 
-// the graph of one chunk starting from the current state buffer: zero the
-// chunk's accumulator rows, then its sweeps, then fold the rows.  A chunk holds an even number
-// of fused launches, so it ends in the buffer it started from.
-int chunk_graph(spk_hip_lattice *h, hipGraphExec_t *out) {
-  for (auto &c : h->graphs)
-    if (c.spin_start == h->spin) {
-      *out = c.exec;
+// The graph of `n` sweeps on the current layout (bit planes or bytes):
+// a kernel zeroes accumulator rows [0, n) (no memset node), then the n
+// sweeps' launches, then the fold of those rows into accc[0, n).  Cached
+// per (n, layout) until a settings change drops the cache.
+int piece_graph(spk_hip_lattice *h, int n, hipGraphExec_t *out) {
+  for (auto &p : h->graphs)
+    if (p.sweeps == n && p.bits == h->bits_active) {
+      *out = p.exec;
       return SPK_HIP_OK;
     }
   int rc;
-  hipGraph_t graph;
-  void *start = h->spin;
+  hipGraph_t graph = nullptr;
   if ((rc = check_hip(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal), "begin capture")))
     return rc;
-  const int64_t n = (int64_t)h->chunk * kAccRow;
-  hipLaunchKernelGGL(k_zero_u64, dim3(grid_for(n, 256)), dim3(256), 0, h->stream, h->acc, n);
+  const int64_t words = (int64_t)n * kAccRow;
+  hipLaunchKernelGGL(k_zero_u64, dim3(grid_for(words, 256)), dim3(256), 0, h->stream, h->acc, words);
   int64_t nl = 0;
   rc = check_hip(hipGetLastError(), "zero launch");
-  if (!rc) rc = launch_sweeps(h, h->chunk, h->acc, &nl);
-  if (!rc) rc = launch_fold(h, h->acc, h->chunk, h->accc);
-  hipError_t e = hipStreamEndCapture(h->stream, &graph);
-  if (rc) return rc;
-  if ((rc = check_hip(e, "end capture"))) return rc;
-  if (h->spin != start) {   // odd number of fused launches: not replayable as is
-    (void)hipGraphDestroy(graph);
-    return set_error(SPK_HIP_ESTATE, "chunk does not return to its start buffer");
+  if (!rc) rc = launch_sweeps(h, n, h->acc, &nl);
+  if (!rc) rc = launch_fold(h, h->acc, n, h->accc);
+  const hipError_t e = hipStreamEndCapture(h->stream, &graph);
+  if (!rc) rc = check_hip(e, "end capture");
+  if (rc) {
+    if (graph) (void)hipGraphDestroy(graph);
+    return rc;
   }
   hipGraphExec_t exec;
   rc = check_hip(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0), "graph instantiate");
   (void)hipGraphDestroy(graph);
   if (rc) return rc;
-  h->graphs.push_back({start, exec});
+  h->graphs.push_back({n, h->bits_active, exec});
   *out = exec;
   return SPK_HIP_OK;
+}
+
+// Sweeps per graph replay of a call of n sweeps: n / chunk full graphs,
+// then one per set bit of the remainder, largest first.
+std::vector<int> graph_pieces(int chunk, int n) {
+  std::vector<int> p(chunk > 0 ? n / chunk : 0, chunk);
+  int rem = chunk > 0 ? n % chunk : 0;
+  for (int b = 1 << 30; b > 0; b >>= 1)
+    if (rem & b) p.push_back(b);
+  return p;
 }
 
 int launch_energy(spk_hip_lattice *h, unsigned long long *acc) {
@@ -685,8 +550,8 @@ void row_sum(const spk_hip_lattice *h, int row, int64_t *x, int64_t *y) {
 }
 
 int compute_energy(spk_hip_lattice *h, int64_t *e) {
-  int rc = ensure_acc(h, 1);
-  if (rc) return rc;
+  int rc;
+  if ((rc = bits_leave(h)) || (rc = ensure_acc(h, 1))) return rc;
   if ((rc = check_hip(hipMemsetAsync(h->acc, 0, kAccRow * sizeof(unsigned long long), h->stream), "memset")))
     return rc;
   if ((rc = launch_energy(h, h->acc)) || (rc = launch_fold(h, h->acc, 1, h->accc))) return rc;
@@ -768,7 +633,10 @@ int spk_hip_create(const spk_hip_desc *d, spk_hip_lattice **out) {
     spk_hip_destroy(h);
     return set_error(SPK_HIP_ENOMEM, "device allocation of %lld sites failed", (long long)h->N);
   }
-  if ((rc = ensure_acc(h, kInitialAccRows)) || (rc = launch_fill_spins(h, 1))) {
+  // accumulator rows and their pinned host copy for calls of up to 1024
+  // sweeps, so that such calls allocate nothing
+  if ((rc = ensure_acc(h, kInitialAccRows)) || (rc = ensure_pinned(h, kInitialAccRows)) ||
+      (rc = launch_fill_spins(h, 1))) {
     spk_hip_destroy(h);
     return rc;
   }
@@ -790,8 +658,6 @@ void spk_hip_destroy(spk_hip_lattice *h) {
   if (h->spin) (void)hipFree(h->spin);
   if (h->seed) (void)hipFree(h->seed);
   if (h->stage) (void)hipFree(h->stage);
-  if (h->spin_alt) (void)hipFree(h->spin_alt);
-  if (h->seed_alt) (void)hipFree(h->seed_alt);
   if (h->bits) (void)hipFree(h->bits);
   if (h->sseed) (void)hipFree(h->sseed);
   if (h->acc) (void)hipFree(h->acc);
@@ -817,6 +683,7 @@ int spk_hip_upload(spk_hip_lattice *h, const int32_t *spin, const int32_t *seed)
   if (!h) return set_error(SPK_HIP_EINVAL, "null handle");
   const size_t bytes = (size_t)h->N * sizeof(int32_t);
   int rc;
+  if ((rc = bits_leave(h))) return rc;   // the planes hold the state: back to bytes first
   const int qmax = app_qmax(h->d.app, h->d.nspins);
   if (spin) {
     if ((rc = check_hip(hipMemcpyAsync(h->stage, spin, bytes, hipMemcpyHostToDevice, h->stream), "upload")))
@@ -848,6 +715,7 @@ int spk_hip_download(spk_hip_lattice *h, int32_t *spin, int32_t *seed) {
   if (!h) return set_error(SPK_HIP_EINVAL, "null handle");
   const size_t bytes = (size_t)h->N * sizeof(int32_t);
   int rc;
+  if ((rc = bits_leave(h))) return rc;
   if (spin) {
     with_type(h->stype, [&]<typename T>() {
       hipLaunchKernelGGL((k_to_natural<T>), dim3(grid_for(h->N, 256)), dim3(256), 0, h->stream,
@@ -877,12 +745,14 @@ int spk_hip_set_range(spk_hip_lattice *h, int seed0, int lo, int hi) {
   if (lo < 1 || hi > qmax)
     return set_error(SPK_HIP_EVALUE, "One or more sites have invalid values (set range %d %d, Q = %d)", lo, hi,
                      qmax);
+  int rc = bits_leave(h);   // the seeds may live on the planes
+  if (rc) return rc;
   const PowTable pw = park_powers();
   with_type(h->stype, [&]<typename T>() {
     hipLaunchKernelGGL((k_set_range<T>), dim3(grid_for(h->N, 256)), dim3(256), 0, h->stream,
                        static_cast<T *>(h->spin), h->g, seed0, lo, hi - lo + 1, pw);
   });
-  int rc = check_hip(hipGetLastError(), "set_range launch");
+  rc = check_hip(hipGetLastError(), "set_range launch");
   if (rc) return rc;
   h->energy_valid = false;
   return check_hip(hipStreamSynchronize(h->stream), "set_range sync");
@@ -893,8 +763,8 @@ int spk_hip_set_value(spk_hip_lattice *h, int value) {
   const int qmax = app_qmax(h->d.app, h->d.nspins);
   if (value < 1 || value > qmax)
     return set_error(SPK_HIP_EVALUE, "One or more sites have invalid values (set value %d, Q = %d)", value, qmax);
-  int rc = launch_fill_spins(h, value);
-  if (rc) return rc;
+  int rc;
+  if ((rc = bits_leave(h)) || (rc = launch_fill_spins(h, value))) return rc;
   h->energy_valid = false;
   return check_hip(hipStreamSynchronize(h->stream), "set_value sync");
 }
@@ -902,11 +772,13 @@ int spk_hip_set_value(spk_hip_lattice *h, int value) {
 int spk_hip_init_seeds(spk_hip_lattice *h, double rseed) {
   if (!h) return set_error(SPK_HIP_EINVAL, "null handle");
   if (!(rseed >= 0.0 && rseed < 1.0)) return set_error(SPK_HIP_EINVAL, "init_seeds: rseed outside [0,1)");
+  int rc = bits_leave(h);   // the spins may live on the planes
+  if (rc) return rc;
   const double xbase = rseed * (double)kIM;
   const uint32_t a100 = park_warmup100();
   hipLaunchKernelGGL(k_init_seeds, dim3(grid_for(h->N, 256)), dim3(256), 0, h->stream, h->seed, h->g, xbase,
                      a100);
-  int rc = check_hip(hipGetLastError(), "init_seeds launch");
+  rc = check_hip(hipGetLastError(), "init_seeds launch");
   if (rc) return rc;
   return check_hip(hipStreamSynchronize(h->stream), "init_seeds sync");
 }
@@ -925,54 +797,67 @@ int spk_hip_sweeps(spk_hip_lattice *h, int nsweeps, int64_t *naccept_total, int6
   const size_t row = 2 * sizeof(unsigned long long);   // a folded row
   int64_t nlaunch = 0;
   if (const size_t lds = resident_bytes(h)) {
-    if ((rc = ensure_acc(h, std::min(nsweeps, kInitialAccRows))) || (rc = ensure_pinned(h, nsweeps)) ||
-        (rc = run_resident(h, nsweeps, lds, &nlaunch)))
+    if ((rc = bits_leave(h)) || (rc = ensure_acc(h, std::min(nsweeps, kInitialAccRows))) ||
+        (rc = ensure_pinned(h, nsweeps)) || (rc = run_resident(h, nsweeps, lds, &nlaunch)))
       return rc;
     return finish_sweeps(h, nsweeps, nlaunch, e0, naccept_total, naccept_per_sweep, energy_per_sweep);
   }
-  // whole chunks replay a captured graph; the rest launch eagerly.  Chunk
-  // rows live in device rows [0, chunk), eager rows after them; every row
-  // reaches the pinned host copy in stream order.
-  const int C = h->chunk;
-  const int nchunks = C > 0 ? nsweeps / C : 0;
-  const int rem = nsweeps - nchunks * C;
-  if ((rc = ensure_acc(h, C + rem)) || (rc = ensure_pinned(h, nsweeps))) return rc;
-  const size_t first_rem = nchunks ? C : 0;             // eager rows follow the chunk's rows
-  unsigned long long *rem_acc = h->acc + first_rem * kAccRow;
-  // two-state sq/4n: the state moves to bit planes for the call (packing
-  // and unpacking lie outside the timed events)
-  if (bits_eligible(h) && (rc = bits_enter(h))) return rc;
-  auto launch_all = [&]() -> int {
-    int rc;
+  // two-state sq/4n: the state moves to the bit planes and stays there
+  if (bits_eligible(h) && !h->bits_active && (rc = bits_enter(h))) return rc;
+  if ((rc = ensure_pinned(h, nsweeps))) return rc;
+  if (h->chunk > 0) {
+    // graph replays (graph_pieces); each piece's rows are folded into
+    // accc[0, n) and copied to the pinned rows [done, done + n) in stream order
+    const std::vector<int> pieces = graph_pieces(h->chunk, nsweeps);
+    if ((rc = ensure_acc(h, pieces[0]))) return rc;
+    std::vector<hipGraphExec_t> execs(pieces.size());
+    for (size_t k = 0; k < pieces.size(); k++)
+      if ((rc = piece_graph(h, pieces[k], &execs[k]))) return rc;
     if ((rc = check_hip(hipEventRecord(h->ev0, h->stream), "event"))) return rc;
-    for (int k = 0; k < nchunks; k++) {
-      hipGraphExec_t exec;
-      if ((rc = chunk_graph(h, &exec))) return rc;
-      if ((rc = check_hip(hipGraphLaunch(exec, h->stream), "graph launch"))) return rc;
-      if ((rc = check_hip(hipMemcpyAsync(h->pinned + (size_t)k * C * 2, h->accc, row * C, hipMemcpyDeviceToHost,
-                                         h->stream),
+    int done = 0;
+    for (size_t k = 0; k < pieces.size(); k++) {
+      if ((rc = check_hip(hipGraphLaunch(execs[k], h->stream), "graph launch")) ||
+          (rc = check_hip(hipMemcpyAsync(h->pinned + (size_t)done * 2, h->accc, row * pieces[k],
+                                         hipMemcpyDeviceToHost, h->stream),
                           "accumulator readback")))
         return rc;
+      done += pieces[k];
     }
-    if (nchunks) nlaunch += (int64_t)nchunks * (h->fuse ? C / h->fuse : (int64_t)C * h->g.K);
-    if (rem) {
-      if ((rc = check_hip(hipMemsetAsync(rem_acc, 0, (size_t)rem * kAccRow * sizeof(unsigned long long), h->stream),
-                          "memset")))
-        return rc;
-      if ((rc = launch_sweeps(h, rem, rem_acc, &nlaunch))) return rc;
-    }
-    return check_hip(hipEventRecord(h->ev1, h->stream), "event");
-  };
-  rc = launch_all();
-  const int rc_leave = bits_leave(h);   // the state returns to the u8 layout on every path
-  if (rc) return rc;
-  if ((rc = rc_leave)) return rc;
-  if (rem && ((rc = launch_fold(h, rem_acc, rem, h->accc + first_rem * 2)) ||
-              (rc = check_hip(hipMemcpyAsync(h->pinned + (size_t)nchunks * C * 2, h->accc + first_rem * 2, row * rem,
-                                             hipMemcpyDeviceToHost, h->stream),
-                              "accumulator readback"))))
-    return rc;
+    nlaunch = (int64_t)nsweeps * h->g.K;
+  } else {
+    // every launch from the host: rows [0, nsweeps)
+    if ((rc = ensure_acc(h, nsweeps))) return rc;
+    const int64_t words = (int64_t)nsweeps * kAccRow;
+    if ((rc = check_hip(hipEventRecord(h->ev0, h->stream), "event"))) return rc;
+    hipLaunchKernelGGL(k_zero_u64, dim3(grid_for(words, 256)), dim3(256), 0, h->stream, h->acc, words);
+    if ((rc = check_hip(hipGetLastError(), "zero launch")) || (rc = launch_sweeps(h, nsweeps, h->acc, &nlaunch)) ||
+        (rc = launch_fold(h, h->acc, nsweeps, h->accc)) ||
+        (rc = check_hip(hipMemcpyAsync(h->pinned, h->accc, row * nsweeps, hipMemcpyDeviceToHost, h->stream),
+                        "accumulator readback")))
+      return rc;
+  }
+  if ((rc = check_hip(hipEventRecord(h->ev1, h->stream), "event"))) return rc;
   return finish_sweeps(h, nsweeps, nlaunch, e0, naccept_total, naccept_per_sweep, energy_per_sweep);
+}
+
+int spk_hip_prepare(spk_hip_lattice *h, int nsweeps) {
+  if (!h) return set_error(SPK_HIP_EINVAL, "null handle");
+  if (nsweeps < 0) return set_error(SPK_HIP_EINVAL, "nsweeps < 0");
+  if (nsweeps == 0 || resident_bytes(h)) return SPK_HIP_OK;
+  int rc;
+  if (bits_eligible(h) && !h->bits_active && (rc = bits_enter(h))) return rc;
+  if ((rc = ensure_pinned(h, nsweeps))) return rc;
+  if (h->chunk > 0) {
+    const std::vector<int> pieces = graph_pieces(h->chunk, nsweeps);
+    if ((rc = ensure_acc(h, pieces[0]))) return rc;
+    for (int n : pieces) {
+      hipGraphExec_t exec;
+      if ((rc = piece_graph(h, n, &exec))) return rc;
+    }
+  } else if ((rc = ensure_acc(h, nsweeps))) {
+    return rc;
+  }
+  return check_hip(hipStreamSynchronize(h->stream), "prepare sync");
 }
 
 int spk_hip_energy(spk_hip_lattice *h, double *energy) {
@@ -986,8 +871,8 @@ int spk_hip_energy(spk_hip_lattice *h, double *energy) {
 
 int spk_hip_checksum(spk_hip_lattice *h, uint64_t *sum) {
   if (!h || !sum) return set_error(SPK_HIP_EINVAL, "null argument");
-  int rc = ensure_acc(h, 1);
-  if (rc) return rc;
+  int rc;
+  if ((rc = bits_leave(h)) || (rc = ensure_acc(h, 1))) return rc;
   if ((rc = check_hip(hipMemsetAsync(h->acc, 0, sizeof(unsigned long long), h->stream), "memset"))) return rc;
   with_type(h->stype, [&]<typename T>() {
     hipLaunchKernelGGL((k_checksum<T>), dim3(grid_for(h->N, 256)), dim3(256), 0, h->stream,
@@ -1026,44 +911,19 @@ int spk_hip_set_bitplanes(spk_hip_lattice *h, int enable) {
   if (!h) return set_error(SPK_HIP_EINVAL, "null handle");
   if (enable != 0 && enable != 1 && enable != 8 && enable != 16 && enable != 32)
     return set_error(SPK_HIP_EINVAL, "bit planes: 0 (off), 1 (on), or 8, 16, 32 sites per thread");
+  const int rc = bits_leave(h);   // the plane layout depends on the sites per thread
+  if (rc) return rc;
   h->bitplanes = enable == 1 ? kBitSites : enable;
   drop_graphs(h);   // captured launches name one kernel or the other
   return SPK_HIP_OK;
 }
 
-int spk_hip_set_fusion(spk_hip_lattice *h, int enable) {
-  if (!h) return set_error(SPK_HIP_EINVAL, "null handle");
-  drop_graphs(h);
-  if (!enable) {
-    h->fuse = 0;
-    return SPK_HIP_OK;
-  }
-  if (!h->spin_alt) {
-    choose_fusion(h);
-    if (!h->fuse) return SPK_HIP_OK;   // not a 2d lattice, or the tile does not fit
-    if (hipMalloc(&h->spin_alt, (size_t)h->slots * spin_bytes(h->stype)) != hipSuccess ||
-        hipMalloc(&h->seed_alt, (size_t)h->slots * sizeof(int32_t)) != hipSuccess) {
-      h->fuse = 0;
-      return set_error(SPK_HIP_ENOMEM, "second state buffer for fused sweeps not available");
-    }
-  }
-  choose_fusion(h);
-  return SPK_HIP_OK;
-}
-
 int spk_hip_set_graph(spk_hip_lattice *h, int chunk_sweeps) {
   if (!h) return set_error(SPK_HIP_EINVAL, "null handle");
-  if (chunk_sweeps < 0) return set_error(SPK_HIP_EINVAL, "chunk_sweeps < 0");
-  // an even number of fused launches per chunk (see chunk_graph)
-  if (chunk_sweeps && chunk_sweeps % 8) return set_error(SPK_HIP_EINVAL, "chunk_sweeps must be a multiple of 8");
+  if (chunk_sweeps < 0 || chunk_sweeps > kMaxChunk)
+    return set_error(SPK_HIP_EINVAL, "chunk_sweeps must be in [0, %d]", kMaxChunk);
   drop_graphs(h);
   h->chunk = chunk_sweeps;
-  return SPK_HIP_OK;
-}
-
-int spk_hip_get_fusion(spk_hip_lattice *h, int *sweeps_per_launch) {
-  if (!h || !sweeps_per_launch) return set_error(SPK_HIP_EINVAL, "null argument");
-  *sweeps_per_launch = h->fuse;
   return SPK_HIP_OK;
 }
 

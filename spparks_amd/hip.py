@@ -31,7 +31,7 @@ ABI_SYMBOLS = (
     "spk_hip_batch_download", "spk_hip_batch_run", "spk_hip_batch_status",
     "spk_hip_batch_trace", "spk_hip_batch_set_launch", "spk_hip_batch_last_timing",
     "spk_hip_batch_pool", "spk_hip_checksum", "spk_hip_batch_checksum", "spk_hip_serve",
-    "spk_hip_set_fusion", "spk_hip_get_fusion", "spk_hip_set_graph",
+    "spk_hip_set_graph", "spk_hip_prepare",
 )
 
 
@@ -81,9 +81,8 @@ def load_library(path=None):
     L.spk_hip_set_bitplanes.argtypes = [vp, C.c_int]
     L.spk_hip_set_resident.argtypes = [vp, C.c_int]
     L.spk_hip_checksum.argtypes = [vp, C.POINTER(C.c_uint64)]
-    L.spk_hip_set_fusion.argtypes = [vp, C.c_int]
     L.spk_hip_set_graph.argtypes = [vp, C.c_int]
-    L.spk_hip_get_fusion.argtypes = [vp, C.POINTER(C.c_int)]
+    L.spk_hip_prepare.argtypes = [vp, C.c_int]
     L.spk_hip_last_timing.argtypes = [vp, C.POINTER(C.c_double), C.POINTER(C.c_int64)]
     if path is None:
         _lib = L
@@ -249,17 +248,13 @@ class Lattice:
         """0/False: off; 1/True: on; 8, 16 or 32: on with that many sites per thread."""
         _check(self.L.spk_hip_set_bitplanes(self.h, int(enable)))
 
-    def set_fusion(self, enable):
-        _check(self.L.spk_hip_set_fusion(self.h, 1 if enable else 0))
-
     def set_graph(self, chunk_sweeps):
         _check(self.L.spk_hip_set_graph(self.h, int(chunk_sweeps)))
 
-    @property
-    def fusion(self):
-        v = C.c_int()
-        _check(self.L.spk_hip_get_fusion(self.h, C.byref(v)))
-        return v.value
+    def prepare(self, nsweeps):
+        """Capture the graphs (and pack the bit planes) a sweeps(nsweeps)
+        call will use; runs no sweep."""
+        _check(self.L.spk_hip_prepare(self.h, int(nsweeps)))
 
     def checksum(self):
         v = C.c_uint64()

@@ -7,9 +7,6 @@
 //   stype: 0 uint8, 1 uint16, 2 int32 spin storage; vec: sites per thread (4)
 //   SPK_EMUL_BITS=8|16|32 in the environment runs two-state sq/4n lattices
 //   on the bit-plane path (spk_bits.h), that many sites per thread.
-//   SPK_EMUL_FUSE=TXxTY in the environment (2d lattices) runs the
-//   sweep-fused tile algorithm instead, S sweeps per launch, the rest of
-//   the sweeps per colour
 //   in.bin : int32 spin[N], int32 seed[N] (reference order)
 //   out.bin: int32 spin[N], int32 seed[N], int64 naccept[nsweeps],
 //            int64 energy[nsweeps]
@@ -46,52 +43,6 @@ static long long energy(const std::vector<T> &spin, const Geom &g) {
       for (int m = 0; m < NN; m++) e += nb[m] != (int32_t)spin.at((long long)c * g.Mp + a + (long long)g.px * rest);
     }
   return e;
-}
-
-// the fused kernel's work for one launch: every tile, every pass, in order
-template <int LAT, typename T, int S>
-static void fused_launch(const std::vector<T> &sin, const std::vector<int32_t> &kin, std::vector<T> &sout,
-                         std::vector<int32_t> &kout, const Geom &g, const Rule &r, int TX, int TY,
-                         std::vector<long long> &nacc, std::vector<long long> &de) {
-  constexpr int K = LAT == SPK_HIP_SQ_4N ? 2 : 4;
-  constexpr int H = K * S;
-  const int ntiles = ((g.nx + TX - 1) / TX) * ((g.ny + TY - 1) / TY);
-  const int lds_sites = (std::min(TX, g.nx) + 2 * H) * (std::min(TY, g.ny) + 2 * H);
-  for (int tile = 0; tile < ntiles; tile++) {
-    const FusedTile t = fused_tile(g, TX, TY, H, tile);
-    std::vector<T> sp(lds_sites);          // exact-size stand-ins for LDS
-    std::vector<int32_t> sd(lds_sites);
-    const int gx0 = t.X0 - H, gy0 = t.Y0 - H;
-    for (int idx = 0; idx < t.EW * t.EH; idx++) {
-      const int ly = idx / t.EW, lx = idx - ly * t.EW;
-      const long long at = blocked_index2d<LAT>(g, wrap_near(gx0 + lx, g.nx), wrap_near(gy0 + ly, g.ny));
-      if (at != blocked_index(g, wrap_mod(gx0 + lx, g.nx), wrap_mod(gy0 + ly, g.ny), 0)) exit(9);
-      sp.at(idx) = sin.at(at);
-      sd.at(idx) = kin.at(at);
-    }
-    for (int sw = 0; sw < S; sw++)
-      for (int c = 0; c < K; c++) {
-        const FusedPass f = fused_pass<LAT>(t, sw * K + c + 1, c);
-        for (int q = 0; q < f.nslots; q++) {
-          int lx, ly;
-          if (!fused_site<LAT>(t, f, q, lx, ly)) continue;
-          if (fused_color<LAT>(t, lx, ly) != c) { fprintf(stderr, "fused_site gave a wrong colour\n"); exit(7); }
-          const SiteOutcome o = r.nspins == 2 ? fused_event<LAT, T, true>(sp.data(), sd.data(), t.EW, lx, ly, r, r.thr)
-                                              : fused_event<LAT, T, false>(sp.data(), sd.data(), t.EW, lx, ly, r, r.thr);
-          if (fused_in_core(t, lx, ly)) {
-            nacc[sw] += o.changed;
-            de[sw] += o.dE;
-          }
-        }
-      }
-    for (int idx = 0; idx < t.cw * t.ch; idx++) {
-      const int cy = idx / t.cw, cx = idx - cy * t.cw;
-      const int at = (H + cy) * t.EW + H + cx;
-      const long long gp = blocked_index(g, t.X0 + cx, t.Y0 + cy, 0);
-      sout.at(gp) = sp.at(at);
-      kout.at(gp) = sd.at(at);
-    }
-  }
 }
 
 template <int LAT, typename T, int V>
@@ -142,28 +93,6 @@ static int run(int app, int nx, int ny, int nz, int q, double kt, int nsw, const
   std::vector<long long> nacc(nsw), eng(nsw);
   long long na = 0, e = energy<LAT, T>(s, g);
   int w = 0;
-  const char *fz = getenv("SPK_EMUL_FUSE");
-  if (fz && (LAT == SPK_HIP_SQ_4N || LAT == SPK_HIP_SQ_8N)) {
-    int TX = 0, TY = 0;
-    if (sscanf(fz, "%dx%d", &TX, &TY) != 2) return 8;
-    constexpr int S = LAT == SPK_HIP_SQ_4N ? 4 : 2;
-    std::vector<T> s2(S_slots(g), (T)0);
-    std::vector<int32_t> k2(S_slots(g), 0x7ffffff0);
-    for (; w + S <= nsw; w += S) {
-      std::vector<long long> na_s(S, 0), de_s(S, 0);
-      if constexpr (LAT == SPK_HIP_SQ_4N || LAT == SPK_HIP_SQ_8N)
-        fused_launch<LAT, T, S>(s, k, s2, k2, g, r, TX, TY, na_s, de_s);
-      std::swap(s, s2);
-      std::swap(k, k2);
-      for (int q = 0; q < S; q++) {
-        na += na_s[q];
-        e += 2 * de_s[q];
-        nacc[w + q] = na;
-        eng[w + q] = e;
-      }
-      if (e != energy<LAT, T>(s, g)) { fprintf(stderr, "incremental energy drifted (fused)\n"); return 4; }
-    }
-  }
   const char *bz = getenv("SPK_EMUL_BITS");
   if (bz && LAT == SPK_HIP_SQ_4N && r.nspins == 2 && sizeof(T) == 1 &&
       (app < SPK_HIP_POTTS_NEIGH || app == SPK_HIP_ISING_SINGLE)) {
@@ -190,19 +119,38 @@ static int run(int app, int nx, int ny, int nz, int q, double kt, int nsw, const
             fprintf(stderr, "bit seed slots do not cover each site once\n");
             return 15;
           }
+    // the conversion kernels' 16-byte groups: each shadow group once, at
+    // the slot bit_seed_index gives its first site
+    {
+      std::vector<int> unit_hits(bg.seedc / 2, 0);
+      for (int64_t X = 0; X < bg.seedc / 2; X++) {
+        int c, j;
+        uint32_t t;
+        int64_t unit;
+        bit_seed_group(bg, X, c, t, j, unit);
+        if (unit < 0 || unit >= bg.seedc / 2 || bit_seed_index(bg, c, t, 4 * j) != 4 * unit) {
+          fprintf(stderr, "bit seed group does not match the slot order\n");
+          return 17;
+        }
+        unit_hits[unit]++;
+      }
+      for (int v : unit_hits)
+        if (v != 1) { fprintf(stderr, "bit seed groups do not cover each shadow group once\n"); return 18; }
+    }
     std::vector<uint32_t> bits(2 * words);
-    std::vector<int32_t> sseed(2 * bg.seedc);
+    std::vector<int32_t> sseed(2 * bg.seedc, 0x7ffffff0);
     for (int64_t q = 0; q < 2 * words; q++) {
       const int c = q >= words;
       const int64_t t = q - (c ? words : 0);
       bits[q] = bits_pack_word(reinterpret_cast<const uint8_t *>(s.data()), g, c, (int)(t / bg.pw), (int)(t % bg.pw));
     }
-    for (int64_t sl = 0; sl < 2 * bg.seedc; sl++) {
+    for (int64_t X = 0; X < bg.seedc / 2; X++) bits_seed_group_in(k.data(), g, bg, X, sseed.data());
+    for (int64_t sl = 0; sl < 2 * bg.seedc; sl++) {   // the groups agree with the slot-by-slot map
       int c, i;
       uint32_t t;
       bit_seed_slot(bg, sl, c, t, i);
       const int64_t h = bit_seed_home(g, bg, c, t, i);
-      sseed[sl] = h < 0 ? 1 : k[h];
+      if (sseed[sl] != (h < 0 ? 1 : k[h])) { fprintf(stderr, "shadow seed group mismatch\n"); return 19; }
     }
     for (; w < nsw; w++) {
       long long de = 0;
@@ -223,13 +171,7 @@ static int run(int app, int nx, int ny, int nz, int q, double kt, int nsw, const
         const int64_t t = q - (c ? words : 0);
         bits_unpack_word(reinterpret_cast<uint8_t *>(s.data()), g, c, (int)(t / bg.pw), (int)(t % bg.pw), bits[q]);
       }
-      for (int64_t sl = 0; sl < 2 * bg.seedc; sl++) {
-        int c, i;
-        uint32_t t;
-        bit_seed_slot(bg, sl, c, t, i);
-        const int64_t h = bit_seed_home(g, bg, c, t, i);
-        if (h >= 0) k[h] = sseed[sl];
-      }
+      for (int64_t X = 0; X < bg.seedc / 2; X++) bits_seed_group_out(sseed.data(), g, bg, X, k.data());
       e += 2 * de;
       if (e != energy<LAT, T>(s, g)) { fprintf(stderr, "incremental energy drifted (bit planes)\n"); return 4; }
       nacc[w] = na;

@@ -95,21 +95,6 @@ def run_emul(orc, tmp_path, case, env_extra=None, variants=None):
         assert np.array_equal(np.frombuffer(raw, np.int64, n, 8 * N + 8 * n).astype(np.float64), eng), (stype, vec)
 
 
-FUSED_CASES = [
-    # app, lattice, shape, Q, kT, seed, sweeps (fused: 4 per launch for sq/4n, 2 for sq/8n)
-    ("ising", "sq/4n", (64, 48, 1), 2, 2.269, 777, 9),
-    ("ising", "sq/4n", (2, 2, 1), 2, 2.0, 3, 8),
-    ("ising", "sq/4n", (10, 6, 1), 2, 1.5, 4, 5),
-    ("potts", "sq/4n", (18, 10, 1), 5, 1.1, 5, 8),
-    ("potts", "sq/4n", (40, 34, 1), 65535, 0.9, 47, 4),
-    ("potts", "sq/8n", (16, 12, 1), 100, 0.5, 56789, 5),
-    ("potts", "sq/8n", (2, 2, 1), 3, 0.8, 11, 4),
-    ("potts", "sq/8n", (22, 6, 1), 9, 0.9, 78, 4),
-    ("potts", "sq/8n", (34, 66, 1), 1000000, 0.5, 31, 4),
-    ("ising", "sq/8n", (12, 12, 1), 2, 0.0, 37, 4),
-]
-
-
 BIT_CASES = [
     # two-state sq/4n on bit planes: sub-grid rows of 1, 9, 32 (one full
     # word), 33 (a 1-bit last word), 63 and 65 sites; Potts Q=2; T = 0
@@ -139,17 +124,6 @@ def test_emulated_bit_planes_match_oracle(orc, tmp_path, case, spt):
     if not os.path.exists(EMUL):
         pytest.skip("tests/emul/spk_emul not built")
     run_emul(orc, tmp_path, case, {"SPK_EMUL_BITS": str(spt)}, variants=[(0, 4)])
-
-
-@pytest.mark.parametrize("tiles", ["8x4", "16x8", "128x64"])
-@pytest.mark.parametrize("case", FUSED_CASES, ids=lambda c: "%s-%s-%s-q%d" % (c[0], c[1], "x".join(map(str, c[2])), c[3]))
-def test_emulated_fused_tiles_match_oracle(orc, tmp_path, case, tiles):
-    """The sweep-fused tile algorithm (halo K*S in LDS, ping-pong buffers),
-    with tiles smaller than, equal to and larger than the lattice: partial
-    edge tiles and halos that wrap onto the same sites several times."""
-    if not os.path.exists(EMUL):
-        pytest.skip("tests/emul/spk_emul not built")
-    run_emul(orc, tmp_path, case, {"SPK_EMUL_FUSE": tiles})
 
 
 @pytest.mark.parametrize("case", CASES, ids=lambda c: "%s-%s-%s-q%d" % (c[0], c[1], "x".join(map(str, c[2])), c[3]))

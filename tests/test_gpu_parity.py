@@ -185,33 +185,38 @@ def test_launch_shape_independence(block):
     assert np.array_equal(nacc, nacc2) and np.array_equal(eng, eng2)
 
 
-@pytest.mark.parametrize("case", [("ising", "sq/4n", (256, 192, 1), 2, 2.269, 5, 13),
-                                  ("potts", "sq/8n", (130, 66, 1), 300, 0.6, 6, 7),
-                                  ("potts", "sq/4n", (18, 10, 1), 70000, 1.1, 7, 9),
-                                  ("potts", "sq/8n", (2, 2, 1), 3, 0.8, 8, 6)],
-                         ids=["ising-256x192", "potts-sq8n-130x66", "potts-sq4n-i32", "potts-sq8n-2x2"])
-def test_fused_sweeps_equal_oracle_and_unfused(orc, case):
-    """Sweep-fused launches (several sweeps per launch in LDS tiles) against
-    the oracle and against the per-colour kernels, per sweep."""
+@pytest.mark.parametrize("case", [("ising", "sq/4n", (256, 192, 1), 2, 2.269, 5),
+                                  ("potts", "sq/8n", (130, 66, 1), 300, 0.6, 6),
+                                  ("potts", "sc/26n", (12, 10, 8), 70000, 1.1, 7)],
+                         ids=["ising-256x192-planes", "potts-sq8n-130x66", "potts-sc26n-i32"])
+def test_graph_pieces_of_every_size_equal_oracle(orc, case):
+    """Calls of 1..7, 9, 16, 31 and 33 sweeps (graph replays of 32 and of
+    each power of two below it, some prepared ahead with prepare(), the bit
+    planes kept across calls) against one oracle run of the same total,
+    sweep by sweep."""
     import spparks_amd as S
-    app, lat, shape, q, kt, seed, n = case
-    _, _, spin1, seeds1, nacc, eng = oracle_run(orc, app, lat, shape, q, kt, seed, n)
-    out = []
-    for fuse in (True, False):
-        with gpu_init(S, app, lat, shape, q, kt, seed) as L:
-            L.set_fusion(fuse)
-            assert (L.fusion > 0) == fuse
-            _, gn, ge = L.sweeps(n)
-            out.append((gn, ge) + L.download())
-    for gn, ge, gs, gk in out:
-        assert np.array_equal(gn, nacc) and np.array_equal(ge, eng)
-        assert np.array_equal(gs, spin1) and np.array_equal(gk, seeds1)
+    app, lat, shape, q, kt, seed = case
+    calls = [1, 2, 3, 4, 5, 6, 7, 9, 16, 31, 33]
+    _, _, spin1, seeds1, nacc, eng = oracle_run(orc, app, lat, shape, q, kt, seed, sum(calls))
+    with gpu_init(S, app, lat, shape, q, kt, seed) as L:
+        L.set_resident(False)
+        gn, ge, base = [], [], 0
+        for k, n in enumerate(calls):
+            if k % 3 == 0:
+                L.prepare(n)
+            tot, a, e = L.sweeps(n)
+            gn += [base + int(x) for x in a]
+            ge += list(e)
+            base += tot
+        gs, gk = L.download()
+    assert np.array_equal(np.array(gn), nacc) and np.array_equal(np.array(ge), eng)
+    assert np.array_equal(gs, spin1) and np.array_equal(gk, seeds1)
 
 
 @pytest.mark.parametrize("case", [("ising", "sq/4n", (128, 96, 1), 2, 2.269, 21),
                                   ("potts", "sc/26n", (16, 12, 10), 10000, 0.5, 22),
                                   ("potts", "sq/8n", (66, 34, 1), 70000, 0.7, 23)],
-                         ids=["ising-fused", "potts-sc26n", "potts-sq8n-i32"])
+                         ids=["ising-planes", "potts-sc26n", "potts-sq8n-i32"])
 def test_graph_replays_equal_eager_launches(case):
     """Chunks of sweeps replayed from captured graphs, several replays per
     call and several calls with eager work and synchronisation in between
@@ -308,34 +313,35 @@ with open(os.path.join(GOLD, "max_size_runs.json")) as _f:
 
 
 def run_against_fixture(r, mode):
-    """Runs fixture entry r on the GPU (mode: bits, bytes, fused or default)
-    and compares Naccept and energy after every sweep and the state checksum
-    every r["every"] sweeps."""
+    """Runs fixture entry r on the GPU (mode: bits, bytes, bits-short or
+    default) and compares Naccept and energy after every sweep and the state
+    checksum every r["every"] sweeps.  bits-short splits each stretch into
+    calls of 20 and fewer sweeps (planes kept across them)."""
     import spparks_amd as S
     say("fixture run", r["name"], mode)
     with gpu_init(S, r["app"], r["lattice"], tuple(r["shape"]), r["nspins"], r["kT"], r["seed"]) as L:
         if mode == "bytes":
             L.set_bitplanes(False)
-        if mode == "fused":
-            L.set_fusion(True)
         assert str(L.checksum()) == r["checksum"][0]
         base, nacc, eng = 0, [], []
         for i in range(r["sweeps"] // r["every"]):
-            tot, gn, ge = L.sweeps(r["every"])
-            nacc += [int(base + x) for x in gn]
-            eng += [int(e) for e in ge]
-            base += tot
+            left = r["every"]
+            while left:
+                n = min(left, 20) if mode == "bits-short" else left
+                tot, gn, ge = L.sweeps(n)
+                nacc += [int(base + x) for x in gn]
+                eng += [int(e) for e in ge]
+                base += tot
+                left -= n
             assert str(L.checksum()) == r["checksum"][i + 1], "state after %d sweeps" % ((i + 1) * r["every"])
-        if mode == "fused":
-            assert L.fusion
     assert nacc == r["naccept"]
     assert eng == r["energy"]
 
 
 @pytest.mark.parametrize("name,mode", [("ising-sq4n-4096", "bits"), ("ising-sq4n-4096", "bytes"),
-                                       ("ising-sq4n-4096", "fused"), ("potts-sc26n-256", "default")],
+                                       ("ising-sq4n-4096", "bits-short"), ("potts-sc26n-256", "default")],
                          ids=["ising-sq4n-4096-bits", "ising-sq4n-4096-bytes",
-                              "ising-sq4n-4096-fused", "potts-sc26n-256"])
+                              "ising-sq4n-4096-bits-short", "potts-sc26n-256"])
 def test_full_length_runs_match_oracle_fixture(name, mode):
     """The whole 1000-sweep (4096^2 Ising) and 200-sweep (256^3 Potts) runs:
     Naccept and energy after every sweep and the state checksum every
@@ -398,10 +404,11 @@ def test_two_state_bit_planes_bit_exact(orc, case, planes):
 
 
 def test_bit_planes_equal_bytes_on_a_large_lattice():
-    """1026 x 512 Ising (sub-grid rows of 513 sites: a one-bit last word),
-    45 sweeps in calls of 13, 32 and 0: bit planes and the byte kernel give
-    the same spins, seeds, Naccept and energies (graph chunks and eager
-    remainders, packing and unpacking on every call)."""
+    """1026 x 512 Ising (sub-grid rows of 513 sites: a one-bit last word):
+    calls of 13, 32, 0, 5 and 20 sweeps with the planes kept across calls,
+    and download, energy, checksum, set_temperature, a seed upload and a
+    change of sites per thread between them: bit planes and the byte kernel
+    give the same spins, seeds, Naccept, energies and checksums."""
     import spparks_amd as S
     out = []
     for planes in (1, 8, 32, 0):
@@ -411,6 +418,20 @@ def test_bit_planes_equal_bytes_on_a_large_lattice():
             rows = []
             for n in (13, 32, 0):
                 rows.append(L.sweeps(n) if n else None)
+            rows.append(L.sweeps(5))
+            rows.append(L.download())         # the planes go back to bytes here
+            rows.append(L.sweeps(20))
+            rows.append(L.energy())
+            rows.append(L.sweeps(7))
+            rows.append(L.checksum())
+            L.set_temperature(1.5)            # graphs dropped, planes kept
+            rows.append(L.sweeps(9))
+            spin, seeds = L.download()
+            L.upload(seed=seeds[::-1].copy())  # seeds rewritten while bytes are current
+            rows.append(L.sweeps(4))
+            if planes:
+                L.set_bitplanes(16 if planes != 16 else 32)   # a new plane layout mid-run
+            rows.append(L.sweeps(6))
             spin, seeds = L.download()
             out.append((spin, seeds, rows, L.energy()))
     s2, k2, r2, e2 = out[-1]   # the byte kernel
@@ -420,7 +441,11 @@ def test_bit_planes_equal_bytes_on_a_large_lattice():
         for a, b in zip(r1, r2):
             if a is None:
                 continue
-            assert a[0] == b[0] and np.array_equal(a[1], b[1]) and np.array_equal(a[2], b[2])
+            if isinstance(a, tuple):
+                for x, y in zip(a, b):
+                    assert np.array_equal(np.asarray(x), np.asarray(y))
+            else:
+                assert a == b
 
 
 @pytest.mark.parametrize("case", [("potts", "sq/8n", (16, 12, 1), 100, 0.5, 56789),
@@ -523,8 +548,8 @@ def test_two_handles_from_two_threads():
 
 def test_argument_edges(orc):
     """sweeps(0) is a no-op; a negative sweep count, workgroup sizes that are
-    not powers of two in 64..1024 and graph chunks that are not multiples of
-    8 are rejected; the lattice then runs on exactly as the oracle does."""
+    not powers of two in 64..1024 and graph chunks outside 0..1024 are
+    rejected; the lattice then runs on exactly as the oracle does."""
     import spparks_amd as S
     case = ("potts", "sq/8n", (16, 12, 1), 100, 0.5, 56789)
     spin0, seeds0, spin1, seeds1, nacc, eng = oracle_run(orc, *case, 3)
@@ -537,10 +562,10 @@ def test_argument_edges(orc):
         for b in (0, 32, 100, 2048):
             with pytest.raises(S.HipError, match="power of two"):
                 L.set_launch(b)
-        for c in (-8, 7, 12):
+        for c in (-8, -1, 1025):
             with pytest.raises(S.HipError, match="chunk_sweeps"):
                 L.set_graph(c)
-        L.set_graph(16)
+        L.set_graph(7)
         _, gn, ge = L.sweeps(3)
         gs, gk = L.download()
     assert np.array_equal(gs, spin1) and np.array_equal(gk, seeds1)
