@@ -110,6 +110,19 @@ def build_abi_consumer():
     return out
 
 
+def build_refdecl():
+    """tests/abi/refdecl: a C caller that declares the library entry points
+    exactly as the reference's src/library.h does and links
+    libspparks_hip.so.  Test infrastructure."""
+    src = os.path.join(ROOT, "tests", "abi", "refdecl.c")
+    if not os.path.exists(src):
+        return None
+    out = os.path.join(ROOT, "tests", "abi", "refdecl")
+    run(["gcc", "-std=c99", "-pedantic", "-Wall", "-Wextra", "-Werror", "-O1", "-o", out, src,
+         "-L" + LIBDIR, "-lspparks_hip", "-Wl,-rpath," + LIBDIR])
+    return out
+
+
 def build_dump_format_check():
     """tests/abi/dump_format: the dump writer's formatting against fprintf,
     host only.  Test infrastructure."""

@@ -169,7 +169,14 @@ void Spparks::file(FILE *fp, bool owned) {
       if (!jump_to_.empty()) break;
     }
     if (jump_to_.empty() && !line.empty()) one(line);
-    if (jump_to_.empty()) return;
+    if (jump_to_.empty()) {
+      // end of file while a jump still looks for its label (src/input.cpp:155-157)
+      if (label_active_) {
+        label_active_ = false;
+        throw Error("Label wasn't found in input script");
+      }
+      return;
+    }
     // Input::jump (src/input.cpp:621-651): read on from the named file
     const std::string path = jump_to_;
     jump_to_.clear();
@@ -221,7 +228,9 @@ std::string Spparks::one(const std::string &raw) {
     if (ch == '#' && !inq) break;
     line += ch;
   }
-  line = substitute(line);
+  // no substitution while a jump looks for its label: variables used before
+  // the label may not be defined yet (src/input.cpp:314-317)
+  if (!label_active_) line = substitute(line);
   std::vector<std::string> w = tokenize(line);
   if (w.empty()) return "";
   const std::string cmd = w[0];
@@ -1047,7 +1056,7 @@ double Spparks::output_setup() {
     // delay has passed, then the schedule shifted by the dump's tolerance
     if (!d.written && time_ >= d.delay) write_dump(d);
     d.next_time = next_time(time_ + d.tol, d.logfreq, d.delta, d.nrepeat, d.scale, d.delay) - d.tol;
-    dump_time = std::min(dump_time, d.next_time - d.tol);
+    dump_time = std::min(dump_time, d.next_time);
   }
   if (diag_energy_ && diag_stats_) hip(spk_hip_energy(h_, &energy_));
   if (first_run_) {
@@ -1057,12 +1066,24 @@ double Spparks::output_setup() {
   stats_header();
   stats_line(false);
   stats_time_ = stoptime_;
-  if (stats_delta_ > 0.0) {
-    stats_time_ =
-        next_time(time_ + stats_tol_, stats_logfreq_, stats_delta_, stats_nrepeat_, stats_scale_, stats_delay_);
-    stats_time_ -= stats_tol_;
-  }
+  if (stats_delta_ > 0.0) stats_time_ = stats_next(time_);
   return std::min(std::min(stoptime_, dump_time), stats_time_);
+}
+
+// The stats time after a stats line at time t (src/output.cpp:172-177).
+double Spparks::stats_next(double t) const {
+  if (stats_delta_ == 0.0) return stoptime_;
+  return next_time(t + stats_tol_, stats_logfreq_, stats_delta_, stats_nrepeat_, stats_scale_, stats_delay_) -
+         stats_tol_;
+}
+
+// Output::compute's return value: the next time anything is due.  A dump's
+// next_time already has its tolerance taken off once (src/output.cpp:189,
+// :194-198).
+double Spparks::output_next(double stats_time) const {
+  double t = std::min(stoptime_, stats_time);
+  for (const auto &d : dumps_) t = std::min(t, d.next_time);
+  return t;
 }
 
 // Output::compute (src/output.cpp:145-202), in two halves.
@@ -1077,26 +1098,17 @@ void Spparks::stats_compute(bool done, const double *energy_now) {
     else hip(spk_hip_energy(h_, &energy_));
   }
   stats_line(true);
-  stats_time_ = stoptime_;
-  if (stats_delta_ != 0.0) {
-    stats_time_ =
-        next_time(time_ + stats_tol_, stats_logfreq_, stats_delta_, stats_nrepeat_, stats_scale_, stats_delay_);
-    stats_time_ -= stats_tol_;
-  }
+  stats_time_ = stats_next(time_);
 }
 
-// The dump half: every dump due at the current time; returns the next
-// dump time (or the stop time).
-double Spparks::dumps_compute() {
-  double dump_time = stoptime_;
+// The dump half: every dump due at the current time.
+void Spparks::dumps_compute() {
   for (auto &d : dumps_) {
     if (time_ >= d.next_time - d.tol) {   // Output::compute (src/output.cpp:180-192)
       write_dump(d);
       d.next_time = next_time(time_ + d.tol, d.logfreq, d.delta, d.nrepeat, d.scale, d.delay) - d.tol;
     }
-    dump_time = std::min(dump_time, d.next_time - d.tol);
   }
-  return dump_time;
 }
 
 // App::run (src/app.cpp:86-148) -> AppLattice::setup / iterate_rejection
@@ -1124,8 +1136,9 @@ void Spparks::cmd_run(const std::vector<std::string> &a) {
       throw Error("Illegal run command");
     }
   }
+  // a stop time before the current time runs no sweep (src/app.cpp:126-129)
   if (upto) stoptime_ -= time_;
-  if (upto && stoptime_ < time_) throw Error("Run upto value is before current time");
+  if (upto && stoptime_ < 0.0) throw Error("Run upto value is before current time");
 
   if (pre || first_run_) init_run();
   message("Setting up run ...\n");
@@ -1134,7 +1147,7 @@ void Spparks::cmd_run(const std::vector<std::string> &a) {
   // setup: one colour sweep per dt_rkmc = dt_sweep, cut at the stop time
   double dt_rkmc = dt_sweep_;
   if (stoptime_ - time_ < dt_rkmc) dt_rkmc = stoptime_ - time_;
-  output_setup();
+  double nextoutput = output_setup();
   first_run_ = false;
 
   loop_t0_ = wall();
@@ -1144,24 +1157,30 @@ void Spparks::cmd_run(const std::vector<std::string> &a) {
     std::vector<int64_t> acc_sw;
     std::vector<double> eng_sw;
     while (!done) {
-      // the sweeps up to the next dump or the stop time (at most
-      // kCallSweeps) run as one device call; time advances exactly as one
-      // sweep at a time would, and the stats lines due in between are
-      // printed from the call's per-sweep Naccept and energies, as
-      // Output::compute would have printed them after each sweep
-      double dump_time = stoptime_;
-      for (const auto &d : dumps_) dump_time = std::min(dump_time, d.next_time - d.tol);
+      // The sweeps up to the next dump snapshot or the stop time (at most
+      // kCallSweeps) run as one device call.  After each sweep
+      // iterate_rejection calls Output::compute when the time has reached
+      // nextoutput (src/app_lattice.cpp:705-708); a dump is written there
+      // when time >= next_time - tol (src/output.cpp:185).  This first pass
+      // replays that schedule to find the first sweep that writes a dump;
+      // time advances by the same additions as one sweep at a time.
       int n = 0;
-      double t = time_;
+      double t = time_, st = stats_time_, no = nextoutput;
       for (;;) {
         n++;
         t += dt_rkmc;
-        if (t >= stoptime_) {
+        const bool last = t >= stoptime_;
+        bool dump = false;
+        if (last || t >= no) {
+          if (t >= st || last) st = stats_next(t);
+          for (const auto &d : dumps_) dump |= t >= d.next_time - d.tol;
+          no = output_next(st);
+        }
+        if (last) {
           done = true;
           break;
         }
-        if (t >= dump_time) break;
-        if (n == kCallSweeps) break;
+        if (dump || n == kCallSweeps) break;
       }
       acc_sw.resize(n);
       eng_sw.resize(want_e ? n : 0);
@@ -1172,22 +1191,25 @@ void Spparks::cmd_run(const std::vector<std::string> &a) {
       const double t1 = wall();
       const int64_t acc0 = naccept_, att0 = nattempt_;
       const int sw0 = nsweeps_;
+      // the same schedule with output: stats lines from the call's per-sweep
+      // Naccept and energies; dumps can only be due after its last sweep
       double tk = time_;
       for (int k = 0; k < n; k++) {
         tk += dt_rkmc;
         const bool last = done && k == n - 1;
-        if (!(tk >= stats_time_ || last)) continue;
+        if (!(last || tk >= nextoutput)) continue;
         time_ = tk;
         naccept_ = acc0 + acc_sw[k];
         nattempt_ = att0 + (int64_t)(k + 1) * nsites_;
         nsweeps_ = sw0 + k + 1;
         stats_compute(last, want_e ? &eng_sw[k] : nullptr);
+        if (k == n - 1) dumps_compute();
+        nextoutput = output_next(stats_time_);
       }
       time_ = t;
       naccept_ = acc0 + nacc;
       nattempt_ = att0 + (int64_t)n * nsites_;
       nsweeps_ = sw0 + n;
-      (void)dumps_compute();
       t_output_ += wall() - t1;
     }
   }
@@ -1220,16 +1242,74 @@ double Spparks::energy() {
   return e;
 }
 
-const int32_t *Spparks::site_values() {
+int32_t *Spparks::site_values() {
   if (!h_) return nullptr;
   download();
+  spin_out_ = true;
+  spin_ref_ = host_spin_;
+  if (seed_out_) seed_ref_ = host_seed_;
   return host_spin_.data();
 }
 
-const int32_t *Spparks::site_seeds() {
+int32_t *Spparks::site_seeds() {
   if (!h_) return nullptr;
   download();
+  seed_out_ = true;
+  seed_ref_ = host_seed_;
+  if (spin_out_) spin_ref_ = host_spin_;
   return host_seed_.data();
+}
+
+// Site coordinates as CreateSites::structured_lattice gives them
+// (src/create_sites.cpp:344-351): x = i*latconst + boxxlo in a periodic
+// dimension; z of a 2d lattice sits at the middle of the box (:213).
+double **Spparks::site_xyz() {
+  if (nsites_ == 0) return nullptr;
+  xyz_data_.resize(3 * (size_t)nsites_);
+  xyz_rows_.resize(nsites_);
+  const double zmid = 0.5 * (boxlo_[2] + boxhi_[2]);
+  for (int64_t g = 0; g < nsites_; g++) {
+    const int64_t i = g % nx_, j = (g / nx_) % ny_, k = g / ((int64_t)nx_ * ny_);
+    double *r = &xyz_data_[3 * (size_t)g];
+    r[0] = (double)i * latconst_ + boxlo_[0];
+    r[1] = (double)j * latconst_ + boxlo_[1];
+    r[2] = dimension_ == 3 ? (double)k * latconst_ + boxlo_[2] : zmid;
+    xyz_rows_[g] = r;
+  }
+  return xyz_rows_.data();
+}
+
+void Spparks::library_enter() {
+  if (!spin_out_ && !seed_out_) return;
+  if (!h_ || (int64_t)host_spin_.size() != nsites_) {   // the lattice was recreated
+    spin_out_ = seed_out_ = false;
+    return;
+  }
+  const bool sp = spin_out_ && host_spin_ != spin_ref_;
+  const bool sd = seed_out_ && host_seed_ != seed_ref_;
+  if (!sp && !sd) return;
+  const int rc = spk_hip_upload(h_, sp ? host_spin_.data() : nullptr, sd ? host_seed_.data() : nullptr);
+  if (rc == SPK_HIP_EVALUE && sp) throw Error("One or more sites have invalid values");
+  hip(rc);
+  if (sp) {
+    spin_ref_ = host_spin_;
+    values_set_ = true;
+  }
+  if (sd) {
+    seed_ref_ = host_seed_;
+    seeds_loaded_ = true;
+  }
+}
+
+void Spparks::library_leave() {
+  if (!spin_out_ && !seed_out_) return;
+  if (!h_ || (int64_t)host_spin_.size() != nsites_) {
+    spin_out_ = seed_out_ = false;
+    return;
+  }
+  download();
+  if (spin_out_) spin_ref_ = host_spin_;
+  if (seed_out_) seed_ref_ = host_seed_;
 }
 
 }  // namespace spkfront

@@ -65,8 +65,16 @@ class Spparks {
 
   // library access (src/library.cpp:95-147)
   double energy();                       // total energy of current spins
-  const int32_t *site_values();          // host copy of the spins, id order
-  const int32_t *site_seeds();           // host copy of the strict seeds
+  // host arrays of the spins / strict seeds in id order, kept in step with
+  // the device by library_enter / library_leave once handed out
+  int32_t *site_values();
+  int32_t *site_seeds();
+  double **site_xyz();                   // xyz[i][0..2] of site i (App::xyz)
+  // around every library call: values written through a handed-out spin
+  // or seed array go to the device first (enter); after the call the
+  // arrays are refreshed from the device (leave)
+  void library_enter();
+  void library_leave();
   int64_t nglobal() const { return nsites_; }
   double time() const { return time_; }
   int64_t naccept() const { return naccept_; }
@@ -107,12 +115,14 @@ class Spparks {
   void init_run();
   double output_setup();
   void stats_compute(bool done, const double *energy_now);
-  double dumps_compute();
+  void dumps_compute();
   void stats_line(bool timeflag);
   void stats_header();
   void write_dump(Dump &d);
   void drain_dumps();
   double next_time(double t, int logfreq, double delta, int nrepeat, double scale, double delay) const;
+  double stats_next(double t) const;
+  double output_next(double stats_time) const;
 
   // helpers
   void message(const char *fmt, ...) __attribute__((format(printf, 2, 3)));
@@ -177,6 +187,11 @@ class Spparks {
   double t_solve_ = 0.0, t_output_ = 0.0, t_loop_ = 0.0;
 
   std::vector<int32_t> host_spin_, host_seed_;
+  // arrays handed out through the library, and their contents as handed out
+  bool spin_out_ = false, seed_out_ = false;
+  std::vector<int32_t> spin_ref_, seed_ref_;
+  std::vector<double> xyz_data_;
+  std::vector<double *> xyz_rows_;
   DumpWriter writer_;   // snapshots being written; drained at the end of a run
 };
 

@@ -36,11 +36,25 @@ void open_instance(int argc, char **argv, void **ptr) {
   *ptr = new (std::nothrow) Instance(screen, log);
 }
 
+// After a command or file, also one that failed part way: arrays handed
+// out through spparks_extract take the device's state.
+void leave(Instance *in) {
+  try {
+    in->spk.library_leave();
+  } catch (const std::exception &e) {
+    if (in->error.empty()) in->error = e.what();
+  }
+}
+
 }  // namespace
 
 extern "C" {
 
-void spparks_open(int argc, char **argv, void **ptr) { open_instance(argc, argv, ptr); }
+// the communicator slot is never read: one GPU per instance, no MPI
+void spparks_open(int argc, char **argv, void *comm, void **ptr) {
+  (void)comm;
+  open_instance(argc, argv, ptr);
+}
 void spparks_open_no_mpi(int argc, char **argv, void **ptr) { open_instance(argc, argv, ptr); }
 
 void spparks_close(void *ptr) { delete static_cast<Instance *>(ptr); }
@@ -49,32 +63,41 @@ void spparks_file(void *ptr, char *file) {
   auto *in = static_cast<Instance *>(ptr);
   try {
     in->error.clear();
+    in->spk.library_enter();
     in->spk.file(file);
   } catch (const std::exception &e) {
     in->error = e.what();
   }
+  leave(in);
 }
 
 char *spparks_command(void *ptr, char *str) {
   auto *in = static_cast<Instance *>(ptr);
+  char *out = nullptr;
   try {
     in->error.clear();
+    in->spk.library_enter();
     in->last_cmd = in->spk.one(str);
-    return const_cast<char *>(in->last_cmd.c_str());
+    out = const_cast<char *>(in->last_cmd.c_str());
   } catch (const std::exception &e) {
     in->error = e.what();
-    return nullptr;
   }
+  leave(in);
+  return out;
 }
 
 void *spparks_extract(void *ptr, char *name) {
   auto *in = static_cast<Instance *>(ptr);
   try {
     in->error.clear();
+    in->spk.library_enter();
     // App::extract names (src/app.cpp:163-196) plus "seeds" and "time"
-    if (!std::strcmp(name, "site") || !std::strcmp(name, "iarray1"))
-      return const_cast<int32_t *>(in->spk.site_values());
-    if (!std::strcmp(name, "seeds")) return const_cast<int32_t *>(in->spk.site_seeds());
+    if (!std::strcmp(name, "site") || !std::strcmp(name, "iarray1")) return in->spk.site_values();
+    if (!std::strcmp(name, "seeds")) return in->spk.site_seeds();
+    if (!std::strcmp(name, "xyz")) return in->spk.site_xyz();
+    // one integer and no double per site (ninteger = 1, ndouble = 0 in
+    // src/app_potts.cpp:33-34, src/app_ising.cpp:33-34)
+    if (!std::strncmp(name, "iarray", 6) || !std::strncmp(name, "darray", 6)) return nullptr;
     if (!std::strcmp(name, "nglobal") || !std::strcmp(name, "nlocal")) {
       in->ivalue = (int32_t)in->spk.nglobal();   // tagint / int in the reference
       return &in->ivalue;
@@ -108,6 +131,7 @@ double spparks_energy(void *ptr) {
   auto *in = static_cast<Instance *>(ptr);
   try {
     in->error.clear();
+    in->spk.library_enter();
     return in->spk.energy();
   } catch (const std::exception &e) {
     in->error = e.what();

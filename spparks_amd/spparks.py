@@ -62,14 +62,15 @@ class spparks:  # noqa: N801  (name kept from the reference wrapper)
         return r.decode() if r else ""
 
     def extract(self, name, type):  # noqa: A002 (argument name from the reference)
-        """type as in the reference wrapper: 0 int, 1 int*, 3 double, 4 double*."""
+        """type as in the reference wrapper: 0 int, 1 int*, 2 int**, 3 double,
+        4 double*, 5 double** (e.g. extract("xyz", 5)[i][1])."""
         name = name.encode("utf-8")
-        if type in (0, 1):
-            self.lib.spparks_extract.restype = C.POINTER(C.c_int)
-        elif type in (3, 4):
-            self.lib.spparks_extract.restype = C.POINTER(C.c_double)
-        else:
+        restype = {0: C.POINTER(C.c_int), 1: C.POINTER(C.c_int), 2: C.POINTER(C.POINTER(C.c_int)),
+                   3: C.POINTER(C.c_double), 4: C.POINTER(C.c_double),
+                   5: C.POINTER(C.POINTER(C.c_double))}.get(type)
+        if restype is None:
             return None
+        self.lib.spparks_extract.restype = restype
         ptr = self.lib.spparks_extract(self.spk, name)
         if not ptr:
             self._raise_on_error()

@@ -48,6 +48,31 @@ def test_batch_and_library_from_c():
     assert lib[1:] == ["nglobal", "16384", "energy", "97818"]
 
 
+@pytest.mark.gpu
+def test_reference_prototypes_and_live_site_array(tmp_path):
+    """A C caller that declares the entry points exactly as the reference's
+    src/library.h (spparks_open with an MPI_Comm) runs the Potts Q=100
+    128x128 script to its recorded energy; "xyz" gives create_sites'
+    coordinates; values written through "site" reach the device, and the
+    array shows the device's spins after a further run."""
+    exe = os.path.join(ROOT, "tests", "abi", "refdecl")
+    if not os.path.exists(exe):
+        from spparks_amd import build
+        build.build_refdecl()
+    script = os.path.join(ROOT, "tests", "scripts", "in.potts_sq8n_128")
+    print("refdecl", script, flush=True)
+    p = subprocess.run([exe, script], capture_output=True, text=True, timeout=120, cwd=str(tmp_path))
+    assert p.returncode == 0, p.stdout + p.stderr
+    out = {l.split()[0]: l.split()[1:] for l in p.stdout.splitlines()}
+    assert out["nglobal"] == ["16384", "energy", "97818"]
+    assert out["xyz"] == ["16383", "127", "127", "0"]
+    assert p.stdout.splitlines()[1].split() == ["xyz", "129", "1", "1", "0"]
+    assert out["iarray1_is_site"] == ["1", "iarray2_null", "1", "darray1_null", "1"]
+    assert out["striped"] == ["energy", "98304"]
+    e, host = out["after_run"][1], out["after_run"][3]
+    assert e == host and e != "98304"
+
+
 def test_dump_writer_formats_like_fprintf(tmp_path):
     """The dump writer's buffered formatting (to_chars for integers, %g for
     coordinates and energies) writes the same bytes as one fprintf per value,

@@ -187,3 +187,26 @@ def test_jump_from_the_library_is_refused():
         spk.command("jump in.anything")
     spk.command("variable x index 4")
     spk.close()
+
+
+def test_jump_to_a_missing_label_is_an_error(tmp_path):
+    """A jump whose label never comes ends with the reference's error at the
+    end of the file (src/input.cpp:155-157), and the instance is not left
+    skipping commands; lines passed over while looking for a label are not
+    substituted (src/input.cpp:314-317), so an undefined variable there is
+    no error."""
+    rc, out = run_script(tmp_path, "print start\njump in.test nolabel\nprint never\n")
+    assert rc != 0 and "Label wasn't found in input script" in out, out
+    assert "never" not in out and out.count("start") >= 1
+    rc, out = run_script(tmp_path, "jump in.test skip\nprint ${undefined}\nlabel skip\nprint done\n")
+    assert rc == 0, out
+    assert "done" in out and "Substitution" not in out
+    from spparks_amd.spparks import spparks, SpparksError
+    p = tmp_path / "in.nolabel"
+    p.write_text("jump %s nolabel\n" % p)
+    spk = spparks(cmdargs=["-screen", "none", "-log", "none"])
+    with pytest.raises(SpparksError, match="Label wasn't found in input script"):
+        spk.file(str(p))
+    with pytest.raises(SpparksError, match="Substitution for illegal variable"):
+        spk.command("print ${nothing}")   # commands run again: not skipped
+    spk.close()

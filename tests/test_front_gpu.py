@@ -463,3 +463,49 @@ def test_read_sites_value_errors(tmp_path, values, msg):
                        timeout=110)
     assert r.returncode != 0
     assert msg in r.stdout + r.stderr
+
+
+@pytest.mark.parametrize("stats,want", [
+    ("3.0", [0.0, 1.0, 2.5, 4.0, 5.5]),
+    ("0.5", [0.0, 0.5, 1.0, 2.0, 2.5, 3.5, 4.0, 5.0, 5.5])])
+def test_dump_tolerance_follows_the_output_schedule(orc, tmp_path, stats, want):
+    """dump_modify tol 0.6 on a dump every 1.5 time units (a sweep is 0.5):
+    Output::compute runs when the time reaches nextoutput, the earliest of
+    the stats time and each dump's next_time, which already has the
+    tolerance taken off once; a dump then writes when time >= next_time -
+    tol (src/output.cpp:145-202, src/app_lattice.cpp:705-708).  With stats
+    every 3.0 the snapshots fall at 0, 1, 2.5, 4 and 5.5; with stats every
+    sweep compute runs after each sweep and they fall at 0, 0.5, 1, 2, 2.5,
+    3.5, 4, 5 and 5.5.  Each snapshot holds the oracle's state at its time."""
+    p = tmp_path / "in.tol"
+    p.write_text("seed 43\napp_style ising\ndimension 2\nlattice sq/4n 1.0\n"
+                 "region box block 0 20 0 12 -0.5 0.5\ncreate_box box\ncreate_sites box\n"
+                 "set site range 1 2\nsweep color/strict\ntemperature 2.1\nstats %s\n"
+                 "dump 1 text 1.5 snaps.txt id site\ndump_modify 1 tol 0.6\nrun 6.0\n" % stats)
+    run_exe(tmp_path, str(p))
+    snaps = open(tmp_path / "snaps.txt").read().split("ITEM: TIMESTEP\n")[1:]
+    times = [float(s.split("\n")[0].split()[1]) for s in snaps]
+    assert times == want
+    d = orc.make_desc("ising", "sq/4n", 20, 12, 1, 2, 2.1)
+    spin, seeds = orc.script_init(d, 43, 1, 2)
+    done = 0
+    for t, s in zip(times, snaps):
+        n = int(round(t / 0.5))
+        if n > done:
+            orc.sweeps(d, spin, seeds, n - done)
+            done = n
+        vals = np.array([[int(x) for x in ln.split()] for ln in s.splitlines()[8:]])
+        assert np.array_equal(vals[:, 1], spin), t
+
+
+def test_run_upto_before_the_current_time_runs_no_sweep(tmp_path):
+    """run X upto with X below the current time is legal (only X < 0 is an
+    error, src/app.cpp:126-129): it sets up, prints the stats line and runs
+    no sweep."""
+    p = tmp_path / "in.upto"
+    p.write_text("seed 44\napp_style ising\ndimension 2\nlattice sq/4n 1.0\n"
+                 "region box block 0 8 0 8 -0.5 0.5\ncreate_box box\ncreate_sites box\n"
+                 "set site range 1 2\nsweep color/strict\ntemperature 2.1\ndiag_style energy\nstats 1.0\n"
+                 "run 2.0\nrun 1.0 upto\n")
+    rows = stats_rows(run_exe(tmp_path, str(p)))
+    assert [r[3] for r in rows] == [0, 2, 4, 4]
