@@ -44,58 +44,71 @@ constexpr int kVec = 4;
 constexpr int kBatchSlots = 8;                         // accumulator slots per replica sweep row
 constexpr int kBatchRow = kBatchSlots * kSlotStride;   // uint64 words per row
 
+constexpr int kTile = 64;   // threads per tile: one wavefront, 4 sites each
+
 // One colour step of a block for the replicas of one (lattice, storage)
-// variant.  Workgroups (XCD-remapped) take contiguous chunks of the step's
-// tile list; a tile is `blockDim.x` threads of one replica, each thread 4
-// consecutive sites of one sub-grid row.
+// variant.  A tile is one wavefront's work: 64 packs (4 consecutive sites of
+// one sub-grid row each) of one replica's colour pass.  Every wavefront of
+// the persistent grid takes a contiguous chunk of the step's tile list and
+// walks it alone: the replica's thresholds go to the wavefront's own LDS
+// table when the replica changes, and each tile's counts leave by one
+// wavefront reduction and one atomic per counter, so no wavefront ever
+// waits for another.  Consecutive wavefronts (XCD-remapped) take
+// consecutive chunks, so neighbouring tiles share an XCD's L2.
 template <int LAT, typename T>
 __global__ __launch_bounds__(1024) void k_batch_step(char *__restrict__ pool, const BatchRep *__restrict__ reps,
                                                      const StepDesc *__restrict__ steps, const int *__restrict__ act,
                                                      const int *__restrict__ pre, int j, int L,
                                                      unsigned long long *__restrict__ acc) {
-  __shared__ int s_thr[32];
+  __shared__ int s_thr[16][32];   // one threshold table per wavefront
   const StepDesc sd = steps[j];
-  const int G = gridDim.x;
-  const int lw = xcd_tile(blockIdx.x, G);
-  const int chunk = (sd.total + G - 1) / G;
-  const int t0 = lw * chunk;
-  const int t1 = min(sd.total, t0 + chunk);
-  // every thread walks the same tiles, so i, r and cur are uniform across
-  // the workgroup and all threads take the same branches to each barrier;
-  // the replica's table entry is read with uniform addresses (scalar loads)
-  int i = 0;     // position in the step's replica list (monotone over the chunk)
-  int cur = -1;  // replica whose thresholds are in LDS
-  for (int t = t0; t < t1; t++) {
-    while (pre[sd.pre_first + i + 1] <= t) i++;
+  const int wpb = blockDim.x >> 6, wid = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int W = gridDim.x * wpb;
+  const int gw = xcd_tile(blockIdx.x, gridDim.x) * wpb + wid;
+  const int chunk = (sd.total + W - 1) / W;
+  int t = gw * chunk;
+  const int t1 = min(sd.total, t + chunk);
+  if (t >= t1) return;   // nothing below waits for another wavefront
+  // i, r and the tile are uniform across the wavefront: the replica table
+  // and step lists are read with scalar loads
+  const int *pp = pre + sd.pre_first;
+  int lo = 0, hi = sd.count - 1;   // the replica holding tile t: last i with pp[i] <= t
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (pp[mid] <= t) lo = mid;
+    else hi = mid - 1;
+  }
+  int i = lo, cur = -1;
+  int *thr = s_thr[wid];
+  for (; t < t1; t++) {
+    while (pp[i + 1] <= t) i++;
     const int r = act[sd.first + i];
     const BatchRep &rep = reps[r];
-    __syncthreads();   // previous tile's reduction and LDS reads are done
     if (r != cur) {
-      if (threadIdx.x < 28) s_thr[threadIdx.x] = rep.rule.thr[threadIdx.x];
-      __syncthreads();
+      if (lane < 28) thr[lane] = rep.rule.thr[lane];
+      wave_lds_sync();
       cur = r;
     }
-    const int local = t - pre[sd.pre_first + i];
     const int p = (int)(((int64_t)j * rep.P) / L);
     const int K = rep.g.K;
-    const int64_t ti = (int64_t)local * blockDim.x + threadIdx.x;
+    const uint32_t ti = (uint32_t)(t - pp[i]) * kTile + (uint32_t)lane;
     Counts cnt{0, 0};
-    if (ti < (int64_t)(rep.g.Mp / kVec)) {
+    if (ti < (uint32_t)(rep.g.Mp / kVec)) {
       const Geom g = rep.g;
       int a0, b, e;
-      pack_site(g, (uint32_t)ti, a0, b, e);
+      pack_site(g, ti, a0, b, e);
       T *sp = reinterpret_cast<T *>(pool + rep.spin_off);
       int32_t *sdp = reinterpret_cast<int32_t *>(pool + rep.seed_off);
       const Rule &rule = rep.rule;
       bool two = false;
       if constexpr (sizeof(T) == 1) two = rule.nspins == 2;   // two states are stored as u8
       if (two)
-        cnt = sweep_sites<LAT, T, kVec, true, uint32_t>(sp, sdp, g, rule, s_thr, p % K, a0, b, e);
+        cnt = sweep_sites<LAT, T, kVec, true, uint32_t>(sp, sdp, g, rule, thr, p % K, a0, b, e);
       else
-        cnt = sweep_sites<LAT, T, kVec, false, uint32_t>(sp, sdp, g, rule, s_thr, p % K, a0, b, e);
+        cnt = sweep_sites<LAT, T, kVec, false, uint32_t>(sp, sdp, g, rule, thr, p % K, a0, b, e);
     }
     // tiles of one replica pass spread over the row's slots (one line each)
-    block_accumulate_pack(cnt.nacc, cnt.de, acc + (int64_t)(rep.acc_row + p / K) * kBatchRow, t & (kBatchSlots - 1));
+    wave_accumulate_pack(cnt.nacc, cnt.de, acc + (int64_t)(rep.acc_row + p / K) * kBatchRow, t & (kBatchSlots - 1));
   }
 }
 
@@ -284,8 +297,7 @@ int replica_energy(spk_hip_batch *b, const Replica &R, int64_t *e) {
 // floor((j+1)P_r/L) > floor(j P_r/L); its pass index is then floor(j P_r/L),
 // so its passes run once each, in order.  Steps of variant k live at
 // h_steps[k*L + j].
-int build_schedule(spk_hip_batch *b, int max_sweeps, int block, int *L_out, int *rows_out,
-                   int64_t *updates_out) {
+int build_schedule(spk_hip_batch *b, int max_sweeps, int *L_out, int *rows_out, int64_t *updates_out) {
   b->h_reps.clear();
   std::vector<int> vrep;   // variant of each table entry
   int L = 0, rows = 0;
@@ -306,7 +318,7 @@ int build_schedule(spk_hip_batch *b, int max_sweeps, int block, int *L_out, int 
     br.stype = R.stype;
     br.P = R.g.K * R.block_sweeps;
     br.acc_row = R.acc_row;
-    br.tiles = (int)((R.g.Mp / kVec + block - 1) / block);
+    br.tiles = (int)((R.g.Mp / kVec + kTile - 1) / kTile);
     br.spin_off = R.spin_off;
     br.seed_off = R.seed_off;
     b->h_reps.push_back(br);
@@ -359,9 +371,11 @@ int upload_schedule(spk_hip_batch *b, int rows, bool clear_acc) {
       (rc = grow(&b->d_act, &b->cap_act, std::max<int>(1, b->h_act.size()), "step lists", &moved)) ||
       (rc = grow(&b->d_pre, &b->cap_pre, std::max<int>(1, b->h_pre.size()), "step lists", &moved)) ||
       (rc = grow(&b->d_acc, &b->cap_acc, std::max(1, rows) * kBatchRow, "accumulators", &moved)) ||
-      (rc = grow(&b->d_accc, &b->cap_accc, std::max(2, 2 * rows), "accumulators", &moved)))
+      (rc = grow(&b->d_accc, &b->cap_accc, std::max(2, 2 * rows), "accumulators", &moved))) {
+    if (moved) drop_graphs(b);   // an earlier table moved before a later one failed
     return rc;
-  if (moved) drop_graphs(b);
+  }
+  if (moved) drop_graphs(b);   // captured launches hold the old table addresses
   auto put = [&](void *dst, const void *src, size_t n) {
     return n ? check_hip(hipMemcpyAsync(dst, src, n, hipMemcpyHostToDevice, b->stream), "table upload") : 0;
   };
@@ -644,130 +658,105 @@ int spk_hip_batch_run(spk_hip_batch *b, int max_sweeps, int use_graph) {
   if (!b) return set_error(SPK_HIP_EINVAL, "null batch");
   if (max_sweeps <= 0) return set_error(SPK_HIP_EINVAL, "max_sweeps must be positive");
   int rc;
-  const bool tune = b->block == 0;
-  // candidate shapes for the autotuner: threads per tile x waves per CU
-  std::vector<Shape> cand;
-  if (tune) {
-    for (int bs : {128, 256, 512})
-      for (int wpc : {4, 8, 16, 32}) {
-        const int grid = std::max(kNumXcd, b->ncu * wpc * 64 / bs / kNumXcd * kNumXcd);
-        cand.push_back(Shape{bs, grid});
-      }
-  } else {
-    cand.push_back(Shape{b->block, b->grid});
-  }
-  // tiles depend on the block size, so candidates with different block
-  // sizes need their own schedules; the step order per replica is the same
   int L = 0, rows = 0;
   int64_t updates = 0;
-  if ((rc = build_schedule(b, max_sweeps, cand[0].block, &L, &rows, &updates))) return rc;
+  if ((rc = build_schedule(b, max_sweeps, &L, &rows, &updates))) return rc;
   b->last_updates = updates;
   b->last_nlaunch = (int64_t)L * (int64_t)std::max<size_t>(1, b->variants.size());
   if (L == 0) {
     b->last_ms = 0.0;
     return SPK_HIP_OK;
   }
-  if ((rc = ensure_events(b, 2 * (size_t)L + 2)) || (rc = ensure_streams(b))) return rc;
+  if ((rc = ensure_events(b, 2 * (size_t)L + 2)) || (rc = ensure_streams(b)) ||
+      (rc = upload_schedule(b, rows, true)))
+    return rc;
 
-  // sites updated in step j, for a candidate's rate
-  auto step_work = [&](int j) {
-    double w = 0.0;
-    for (size_t k = 0; k < b->variants.size(); k++) {
-      const StepDesc &sd = b->h_steps[k * L + j];
-      for (int a = 0; a < sd.count; a++) w += (double)b->h_reps[b->h_act[sd.first + a]].g.M;
-    }
-    return w;
-  };
-  const int C = (int)cand.size();
-  const int per = tune && L / C >= 8 ? L / C / 8 * 8 : 0;   // whole sweeps of an 8-colour replica
-  if (tune && per) {
-    // each candidate runs `per` consecutive steps as a replay does (no wait
-    // between steps) under one event pair; the schedule repeats every sweep,
-    // so every chunk holds the same mix of replicas.  The steps after the
-    // last chunk run with the winner.
-    std::vector<double> rate(C, 0.0);
+  if (b->block == 0) {
+    // autotune: candidate shapes (workgroup size x wavefronts per CU) run
+    // the block's own steps, as a replay runs them (no wait between steps);
+    // results do not depend on the shape.  The tile list is the same for
+    // every shape (a tile is one wavefront's work).
+    std::vector<Shape> cand;
+    for (int bs : {128, 256, 512})
+      for (int wpc : {8, 16, 32}) {
+        const int grid = std::max(kNumXcd, b->ncu * wpc * 64 / bs / kNumXcd * kNumXcd);
+        cand.push_back(Shape{bs, grid});
+      }
+    auto step_work = [&](int j) {   // sites updated in step j
+      double w = 0.0;
+      for (size_t k = 0; k < b->variants.size(); k++) {
+        const StepDesc &sd = b->h_steps[k * L + j];
+        for (int a = 0; a < sd.count; a++) w += (double)b->h_reps[b->h_act[sd.first + a]].g.M;
+      }
+      return w;
+    };
+    const int C = (int)cand.size();
+    // each candidate gets whole sweeps of an 8-colour replica when the
+    // block has room, else single steps in turn
+    const int per = L / C >= 8 ? L / C / 8 * 8 : 1;
+    std::vector<double> t_ms(C, 0.0), work(C, 0.0);
     double tot = 0.0;
-    for (int c = 0; c < C; c++) {
-      const int j0 = c * per;
-      if ((rc = build_schedule(b, max_sweeps, cand[c].block, &L, &rows, &updates))) return rc;
-      if ((rc = upload_schedule(b, rows, c == 0))) return rc;   // one clear per block
+    int j = 0;
+    auto timed = [&](int j0, int j1, const Shape &sh, double *ms_out) -> int {
+      int rc;
       if ((rc = check_hip(hipEventRecord(b->events[0], b->stream), "event")) ||
-          (rc = launch_steps(b, j0, j0 + per, L, cand[c].block, cand[c].grid)) ||
+          (rc = launch_steps(b, j0, j1, L, sh.block, sh.grid)) ||
           (rc = check_hip(hipEventRecord(b->events[1], b->stream), "event")) ||
           (rc = check_hip(hipStreamSynchronize(b->stream), "tune sync")))
         return rc;
       float ms = 0.f;
       if ((rc = check_hip(hipEventElapsedTime(&ms, b->events[0], b->events[1]), "event time"))) return rc;
-      double w = 0.0;
-      for (int j = j0; j < j0 + per; j++) w += step_work(j);
-      rate[c] = w / std::max((double)ms, 1e-9);
+      *ms_out = ms;
+      return SPK_HIP_OK;
+    };
+    auto trial = [&](int c) -> int {   // candidate c takes the next `per` steps
+      const int j1 = std::min(L, j + per);
+      double ms = 0.0;
+      const int rc = timed(j, j1, cand[c], &ms);
+      if (rc) return rc;
+      for (int q = j; q < j1; q++) work[c] += step_work(q);
+      t_ms[c] += ms;
       tot += ms;
+      j = j1;
+      return SPK_HIP_OK;
+    };
+    if (per > 1) {
+      for (int c = 0; c < C; c++)
+        if ((rc = trial(c))) return rc;
+    } else {
+      while (j < L)
+        for (int c = 0; c < C && j < L; c++)
+          if ((rc = trial(c))) return rc;
     }
-    size_t best = 0;
+    int best = 0;
     for (int c = 1; c < C; c++)
-      if (rate[c] > rate[best]) best = c;
-    b->block = cand[best].block;
-    b->grid = cand[best].grid;
-    if (C * per < L) {
-      if ((rc = build_schedule(b, max_sweeps, b->block, &L, &rows, &updates)) ||
-          (rc = upload_schedule(b, rows, false)) ||
-          (rc = check_hip(hipEventRecord(b->events[0], b->stream), "event")) ||
-          (rc = launch_steps(b, C * per, L, L, b->block, b->grid)) ||
-          (rc = check_hip(hipEventRecord(b->events[1], b->stream), "event")) ||
-          (rc = check_hip(hipStreamSynchronize(b->stream), "run sync")))
-        return rc;
-      float ms = 0.f;
-      if ((rc = check_hip(hipEventElapsedTime(&ms, b->events[0], b->events[1]), "event time"))) return rc;
-      tot += ms;
-    }
-    b->last_ms = tot;
-  } else if (tune) {
-    // too few steps for whole-sweep chunks: step j runs with candidate j % C;
-    // every candidate's block size has its own tile lists, uploaded before
-    // the step that uses it
-    std::vector<double> t_ms(cand.size(), 0.0), work(cand.size(), 0.0);
-    int j = 0;
-    double tot = 0.0;
-    while (j < L) {
-      for (size_t c = 0; c < cand.size() && j < L; c++, j++) {
-        if ((rc = build_schedule(b, max_sweeps, cand[c].block, &L, &rows, &updates))) return rc;
-        if ((rc = upload_schedule(b, rows, j == 0))) return rc;   // one clear per block
-        if ((rc = check_hip(hipEventRecord(b->events[2 * j], b->stream), "event")) ||
-            (rc = launch_steps(b, j, j + 1, L, cand[c].block, cand[c].grid)) ||
-            (rc = check_hip(hipEventRecord(b->events[2 * j + 1], b->stream), "event")) ||
-            (rc = check_hip(hipStreamSynchronize(b->stream), "tune sync")))
-          return rc;
-        float ms = 0.f;
-        if ((rc = check_hip(hipEventElapsedTime(&ms, b->events[2 * j], b->events[2 * j + 1]), "event time")))
-          return rc;
-        work[c] += step_work(j);
-        t_ms[c] += ms;
-        tot += ms;
-      }
-    }
-    size_t best = 0;
-    for (size_t c = 1; c < cand.size(); c++)
       if (work[c] / std::max(t_ms[c], 1e-9) > work[best] / std::max(t_ms[best], 1e-9)) best = c;
     b->block = cand[best].block;
     b->grid = cand[best].grid;
+    if (j < L) {   // the steps after the trial chunks run with the winner
+      double ms = 0.0;
+      if ((rc = timed(j, L, cand[best], &ms))) return rc;
+      tot += ms;
+    }
     b->last_ms = tot;
   } else {
-    if ((rc = upload_schedule(b, rows, true))) return rc;
-    const Shape sh = cand[0];
+    const Shape sh{b->block, b->grid};
     int mask = 0;
     for (int v : b->variants) mask |= 1 << v;
     if (use_graph) {
       auto key = std::make_tuple(L, sh.block, sh.grid, mask);
       auto it = b->graphs.find(key);
       if (it == b->graphs.end()) {
-        hipGraph_t graph;
+        hipGraph_t graph = nullptr;
         if ((rc = check_hip(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal), "begin capture")))
           return rc;
-        if ((rc = launch_steps(b, 0, L, L, sh.block, sh.grid))) {
-          (void)hipStreamEndCapture(b->stream, &graph);
+        rc = launch_steps(b, 0, L, L, sh.block, sh.grid);
+        const hipError_t e = hipStreamEndCapture(b->stream, &graph);
+        if (!rc) rc = check_hip(e, "end capture");
+        if (rc) {
+          if (graph) (void)hipGraphDestroy(graph);
           return rc;
         }
-        if ((rc = check_hip(hipStreamEndCapture(b->stream, &graph), "end capture"))) return rc;
         hipGraphExec_t exec;
         rc = check_hip(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0), "graph instantiate");
         (void)hipGraphDestroy(graph);
