@@ -4,16 +4,26 @@
     python bench.py [--steps K] [--warmup W] [--workload ising-sq4n-4096]
 
 A step is one strict sweep (every colour once) over the whole lattice, i.e.
-nx*ny*nz site updates.  The default workload is the 4096x4096 sq/4n Ising
-lattice at kT=2.269 (1000 sweeps); `--workload` selects another single-GPU
-configuration.  Inputs are generated on the device by the `set site range`
-and strict-seed initialisation kernels before the timed region, so the timed
-region contains only the sweeps.  Prints one JSON line.
+nx*ny*nz site updates.  The headline workload is the 4096x4096 sq/4n Ising
+lattice at kT=2.269 (K timed sweeps; its own run is 1000 sweeps).  Inputs are
+generated on the device by the `set site range` and strict-seed
+initialisation kernels before the timed region, so the timed region contains
+only the sweeps.  Prints one JSON line.
+
+The default run also measures every other configuration and nests the
+results under "configs": the 128x128 Potts script (one-workgroup kernel,
+with a latency figure instead of an HBM fraction), the whole 1000-sweep
+4096^2 Ising run, the 256^3 sc/26n Potts run, the 512-replica ensemble
+(batched against one at a time, every replica compared), serving streams
+at 10, 50 and 200 requests/s, and the three proposal variants.
+`--workload` runs one workload alone; `--no-configs` keeps the headline only.
 """
 from __future__ import annotations
 
 import argparse
+import copy
 import json
+import math
 import os
 import sys
 import time
@@ -73,6 +83,20 @@ def load_traffic(workload):
     return (t["bytes_per_launch"] if t else None), t
 
 
+def host_cores():
+    """CPUs this process may run on: its affinity set, capped by a cgroup v2
+    CPU quota when one is set."""
+    n = len(os.sched_getaffinity(0))
+    try:
+        with open("/sys/fs/cgroup/cpu.max") as f:
+            quota, period = f.read().split()[:2]
+        if quota != "max":
+            n = min(n, max(1, math.ceil(int(quota) / int(period))))
+    except (OSError, ValueError):
+        pass
+    return n
+
+
 def cpu_baseline(app, lattice, shape, q, kt, seed, budget_s, threads):
     """The oracle (C restatement, OpenMP within a colour) on this host's cores:
     whole-lattice sweeps of the same workload until budget_s has elapsed."""
@@ -90,7 +114,7 @@ def cpu_baseline(app, lattice, shape, q, kt, seed, budget_s, threads):
         if el >= budget_s or done >= 1000:
             break
     return {"value": round(n * done / el / 1e9, 6), "unit": "G site-updates/s",
-            "cores": threads, "kind": "port",
+            "cores": threads, "host_cpus": os.cpu_count(), "kind": "port",
             "sample": "%d whole-lattice strict sweeps of the same workload from the same initial "
                       "state, oracle/spk_oracle.c (OpenMP over the sites of one colour) in %.1f s"
                       % (done, el),
@@ -128,17 +152,16 @@ def run_ensemble(args):
     def fill(B):
         return [B.add(d) for d in descs]
 
-    # autotune on a throwaway copy of the ensemble (results do not depend on
-    # the shape, but its sweeps would count against the timed trajectory)
-    if args.block:
-        shape = (args.block, args.grid or 2048)
-    else:
-        with bmod.Batch(pool, len(reps)) as B:
-            fill(B)
-            B.set_launch(0, 0)
-            B.run(24, graph=False)   # about 20 colour steps per candidate shape
-            t = B.last_timing()
-            shape = (t["block"], t["grid"])
+    # autotune (or, with --block, warm up) on a throwaway copy of the
+    # ensemble: results do not depend on the shape, but its sweeps would
+    # count against the timed trajectory, and the first launches of each
+    # kernel load its code
+    with bmod.Batch(pool, len(reps)) as B:
+        fill(B)
+        B.set_launch(args.block, args.grid or 2048) if args.block else B.set_launch(0, 0)
+        B.run(24, graph=False)   # about 20 colour steps per candidate shape
+        t = B.last_timing()
+        shape = (t["block"], t["grid"])
 
     with bmod.Batch(pool, len(reps)) as B:
         ids = fill(B)
@@ -394,10 +417,28 @@ def run(args):
     }
     # device-side set-up (`set site range` by jump-ahead, strict seeds by
     # tagreset) against the oracle's sequential restatement of the same
+    if resident:
+        # one workgroup on one CU: the colour passes of a launch follow each
+        # other behind a barrier and the state crosses HBM once per launch,
+        # so the time per pass, not an HBM fraction, says how close it is
+        # to what bounds it (the issue time of a pass's instructions, from
+        # the committed SQ counters, when present)
+        passes = steps * K
+        per_pass_us = dev_ms * 1e3 / passes
+        roof = {"bound": "latency", "kernel": kernel, "per_pass_us": round(per_pass_us, 3),
+                "passes_per_launch": passes // nlaunch, "avg_launch_us": round(per_launch_s * 1e6, 3),
+                "bytes_per_launch": bytes_per_launch, "traffic": traffic, "traffic_measurement": traffic_detail}
+        lat = load_traffic(args.workload + "-latency")[1]
+        if lat:
+            roof["issue_floor_us_per_pass"] = lat["issue_floor_us_per_pass"]
+            roof["frac"] = round(lat["issue_floor_us_per_pass"] / per_pass_us, 4)
+            roof["issue_floor_measurement"] = lat
+        out["roofline"] = roof
+        out["pct_hbm_peak"] = None
     out["setup"] = {"device_s": round(t_setup, 6),
                     "what": "set site range 1 Q and tagreset seeds for every site, on the device"}
     if not args.no_cpu_baseline:
-        threads = args.cpu_threads or min(os.cpu_count() or 1, 16)
+        threads = args.cpu_threads or host_cores()
         out["cpu_baseline"] = cpu_baseline(app, lattice, shape, q, kt, seed, args.cpu_seconds, threads)
         out["setup"].update(cpu_port_s=out["cpu_baseline"].pop("setup_s"), cpu_threads=threads)
     else:
@@ -430,6 +471,10 @@ def main():
                     help="bit planes: sites per thread (0 = default)")
     ap.add_argument("--chunk", type=int, default=32, help="sweeps per replayed graph (0 = eager launches)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-configs", action="store_true",
+                    help="headline workload only (no nested measurement of the other configurations)")
+    ap.add_argument("--serve-n-nested", type=int, nargs=3, default=[40, 120, 300], metavar=("N10", "N50", "N200"),
+                    help="serving requests per offered load in the nested configs")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -448,7 +493,52 @@ def main():
     elif args.workload == "serving-sc26n":
         print(json.dumps(run_serving(args)), flush=True)
     else:
-        print(json.dumps(run(args)), flush=True)
+        out = run(args)
+        if args.workload == "ising-sq4n-4096" and not args.no_configs:
+            out["configs"] = run_configs(args)
+        print(json.dumps(out), flush=True)
+
+
+# keys every single-workload line repeats; a nested config keeps the rest
+_SHARED = ("metric", "n_gpus", "higher_is_better", "scaling", "vs_baseline", "data")
+
+
+def _nested(r, seconds):
+    out = {k: v for k, v in r.items() if k not in _SHARED}
+    out["bench_seconds"] = round(seconds, 2)
+    return out
+
+
+def run_configs(args):
+    """Every configuration besides the headline, each at its own workload:
+    cfg1 potts-sq8n-128 (100 sweeps), cfg2 ising-sq4n-4096 (1000 sweeps),
+    cfg3 potts-sc26n-256 (200 sweeps), cfg4 ensemble-512 (100 sweeps, every
+    replica against its one-at-a-time run), cfg5 serving-sc26n (shorter
+    streams than --workload serving-sc26n), and the proposal variants.  The
+    single lattices get a short CPU sample of the oracle each."""
+    cfgs = {}
+    label = {"potts-sq8n-128": "cfg1", "ising-sq4n-4096": "cfg2", "potts-sc26n-256": "cfg3"}
+    for wl in ("potts-sq8n-128", "ising-sq4n-4096", "potts-sc26n-256", "potts-neigh-sc26n-256",
+               "potts-neighonly-sc26n-256", "ising-single-sq4n-4096"):
+        a = copy.copy(args)
+        a.workload, a.steps, a.warmup = wl, None, 5
+        a.cpu_seconds = 3.0
+        a.no_cpu_baseline = args.no_cpu_baseline or wl not in label
+        t0 = time.perf_counter()
+        r = run(a)
+        cfgs[label.get(wl, wl) + ":" + wl] = _nested(r, time.perf_counter() - t0)
+        print(json.dumps({"config_done": wl, "value": r["value"]}), file=sys.stderr, flush=True)
+    a = copy.copy(args)
+    a.steps, a.block, a.grid = None, 0, 0
+    t0 = time.perf_counter()
+    cfgs["cfg4:ensemble-512"] = _nested(run_ensemble(a), time.perf_counter() - t0)
+    print(json.dumps({"config_done": "ensemble-512"}), file=sys.stderr, flush=True)
+    a = copy.copy(args)
+    a.serve_n = args.serve_n_nested
+    a.serve_check = 4
+    t0 = time.perf_counter()
+    cfgs["cfg5:serving-sc26n"] = _nested(run_serving(a), time.perf_counter() - t0)
+    return cfgs
 
 
 if __name__ == "__main__":

@@ -11,6 +11,7 @@
 #include <tuple>
 #include <vector>
 
+#include "spk_batch_async.h"
 #include "spk_device.h"
 #include "spk_host.h"
 #include "spk_kernels.h"
@@ -50,9 +51,10 @@ constexpr int kTile = 64;   // threads per tile: one wavefront, 4 sites each
 // variant.  A tile is one wavefront's work: 64 packs (4 consecutive sites of
 // one sub-grid row each) of one replica's colour pass.  Every wavefront of
 // the persistent grid takes a contiguous chunk of the step's tile list and
-// walks it alone: the replica's thresholds go to the wavefront's own LDS
-// table when the replica changes, and each tile's counts leave by one
-// wavefront reduction and one atomic per counter, so no wavefront ever
+// walks it alone, replica by replica: a replica's table entry is read once
+// (scalar loads) and its thresholds go to the wavefront's own LDS table;
+// the counts of all its tiles in the chunk stay in registers and leave by
+// one wavefront reduction and one atomic per counter.  No wavefront ever
 // waits for another.  Consecutive wavefronts (XCD-remapped) take
 // consecutive chunks, so neighbouring tiles share an XCD's L2.
 template <int LAT, typename T>
@@ -69,8 +71,8 @@ __global__ __launch_bounds__(1024) void k_batch_step(char *__restrict__ pool, co
   int t = gw * chunk;
   const int t1 = min(sd.total, t + chunk);
   if (t >= t1) return;   // nothing below waits for another wavefront
-  // i, r and the tile are uniform across the wavefront: the replica table
-  // and step lists are read with scalar loads
+  // i, the replica and the tile are uniform across the wavefront: the
+  // replica table and step lists are read with scalar loads
   const int *pp = pre + sd.pre_first;
   int lo = 0, hi = sd.count - 1;   // the replica holding tile t: last i with pp[i] <= t
   while (lo < hi) {
@@ -78,37 +80,36 @@ __global__ __launch_bounds__(1024) void k_batch_step(char *__restrict__ pool, co
     if (pp[mid] <= t) lo = mid;
     else hi = mid - 1;
   }
-  int i = lo, cur = -1;
   int *thr = s_thr[wid];
-  for (; t < t1; t++) {
-    while (pp[i + 1] <= t) i++;
-    const int r = act[sd.first + i];
-    const BatchRep &rep = reps[r];
-    if (r != cur) {
-      if (lane < 28) thr[lane] = rep.rule.thr[lane];
-      wave_lds_sync();
-      cur = r;
-    }
+  for (int i = lo; t < t1; i++) {
+    const int first = pp[i], tend = min(t1, pp[i + 1]);
+    const BatchRep &rep = reps[act[sd.first + i]];
+    const Geom g = rep.g;
+    const Rule &rule = rep.rule;
+    if (lane < 28) thr[lane] = rule.thr[lane];
+    wave_lds_sync();
     const int p = (int)(((int64_t)j * rep.P) / L);
-    const int K = rep.g.K;
-    const uint32_t ti = (uint32_t)(t - pp[i]) * kTile + (uint32_t)lane;
-    Counts cnt{0, 0};
-    if (ti < (uint32_t)(rep.g.Mp / kVec)) {
-      const Geom g = rep.g;
-      int a0, b, e;
-      pack_site(g, ti, a0, b, e);
-      T *sp = reinterpret_cast<T *>(pool + rep.spin_off);
-      int32_t *sdp = reinterpret_cast<int32_t *>(pool + rep.seed_off);
-      const Rule &rule = rep.rule;
-      bool two = false;
-      if constexpr (sizeof(T) == 1) two = rule.nspins == 2;   // two states are stored as u8
-      if (two)
-        cnt = sweep_sites<LAT, T, kVec, true, uint32_t>(sp, sdp, g, rule, thr, p % K, a0, b, e);
-      else
-        cnt = sweep_sites<LAT, T, kVec, false, uint32_t>(sp, sdp, g, rule, thr, p % K, a0, b, e);
+    const int c = p % g.K;
+    T *sp = reinterpret_cast<T *>(pool + rep.spin_off);
+    int32_t *sdp = reinterpret_cast<int32_t *>(pool + rep.seed_off);
+    const uint32_t npack = (uint32_t)(g.Mp / kVec);
+    bool two = false;
+    if constexpr (sizeof(T) == 1) two = rule.nspins == 2;   // two states are stored as u8
+    int nacc = 0, de = 0;
+    for (; t < tend; t++) {
+      const uint32_t ti = (uint32_t)(t - first) * kTile + (uint32_t)lane;
+      if (ti < npack) {
+        int a0, b, e;
+        pack_site(g, ti, a0, b, e);
+        const Counts cnt = two ? sweep_sites<LAT, T, kVec, true, uint32_t>(sp, sdp, g, rule, thr, c, a0, b, e)
+                               : sweep_sites<LAT, T, kVec, false, uint32_t>(sp, sdp, g, rule, thr, c, a0, b, e);
+        nacc += cnt.nacc;
+        de += cnt.de;
+      }
     }
-    // tiles of one replica pass spread over the row's slots (one line each)
-    wave_accumulate_pack(cnt.nacc, cnt.de, acc + (int64_t)(rep.acc_row + p / K) * kBatchRow, t & (kBatchSlots - 1));
+    // the chunk's tiles of this replica pass: one sum per counter, into one
+    // of the row's slots (one cache line each)
+    wave_accumulate(nacc, de, acc + (int64_t)(rep.acc_row + p / g.K) * kBatchRow, gw & (kBatchSlots - 1));
   }
 }
 
@@ -138,6 +139,7 @@ struct Replica {
   std::vector<int64_t> trace_eng;
   int block_sweeps = 0;             // sweeps in the current block
   int acc_row = 0;
+  bool energy_pending = false;      // initial energy on its way to pinned memory
   int variant = 0;                  // lat*3 + stype: which step kernel runs it
 };
 
@@ -202,7 +204,6 @@ struct spk_hip_batch {
   // staging for transfers and per-replica set-up
   int32_t *stage = nullptr;
   int64_t stage_n = 0;
-  unsigned long long *scratch = nullptr;   // kAccRow + 2 u64 for set-up reductions
   // block tables (device) and their host images
   BatchRep *d_reps = nullptr;
   StepDesc *d_steps = nullptr;
@@ -224,6 +225,16 @@ struct spk_hip_batch {
   std::vector<hipEvent_t> events;
   double last_ms = 0.0;
   int64_t last_updates = 0, last_nlaunch = 0;
+  // a block launched and not yet completed (batch_launch / batch_complete)
+  bool inflight = false;
+  int inflight_rows = 0;
+  unsigned long long *h_rows = nullptr;   // pinned: folded rows of the block in flight
+  int cap_h_rows = 0;
+  // per replica slot: an energy accumulator row, the folded initial energy
+  // and the checksum, copied to pinned memory in stream order
+  unsigned long long *d_erow = nullptr;   // [slots][kAccRow]
+  unsigned long long *d_slot = nullptr;   // [slots][4]: energy, (unused), checksum, (unused)
+  unsigned long long *h_slot = nullptr;   // pinned copy of d_slot
 };
 
 namespace {
@@ -268,28 +279,58 @@ int ensure_stage(spk_hip_batch *b, int64_t n) {
   return SPK_HIP_OK;
 }
 
-// total energy of one replica (exact integer)
-int replica_energy(spk_hip_batch *b, const Replica &R, int64_t *e) {
-  int rc = check_hip(hipMemsetAsync(b->scratch, 0, kAccRow * sizeof(unsigned long long), b->stream), "memset");
-  if (rc) return rc;
+// Total energy of replica slot `id` (exact integer) on the batch stream,
+// into the slot's device row and then its pinned copy; nothing waits.  The
+// value is read by resolve_energies once the stream has passed it.
+int replica_energy_async(spk_hip_batch *b, int id, Replica &R) {
+  unsigned long long *row = b->d_erow + (size_t)id * kAccRow;
+  hipLaunchKernelGGL(k_fill_u64, dim3(grid_for(kAccRow, 256)), dim3(256), 0, b->stream, row, (int64_t)kAccRow);
   const int64_t n = (int64_t)R.g.M * R.g.K;
   with_lattice(R.g.lat, [&]<int LAT>() {
     with_type(R.stype, [&]<typename T>() {
       hipLaunchKernelGGL((k_energy<LAT, T>), dim3(grid_for(n, 256)), dim3(256), 0, b->stream,
-                         reinterpret_cast<const T *>(b->pool + R.spin_off), R.g, b->scratch);
+                         reinterpret_cast<const T *>(b->pool + R.spin_off), R.g, row);
     });
   });
-  if ((rc = check_hip(hipGetLastError(), "energy launch"))) return rc;
-  hipLaunchKernelGGL(k_fold_rows, dim3(1), dim3(64), 0, b->stream, b->scratch, kAccSlots, kAccRow,
-                     b->scratch + kAccRow);
-  if ((rc = check_hip(hipGetLastError(), "fold launch"))) return rc;
-  unsigned long long h[2];
-  if ((rc = check_hip(hipMemcpyAsync(h, b->scratch + kAccRow, sizeof h, hipMemcpyDeviceToHost, b->stream),
-                      "readback")))
-    return rc;
-  if ((rc = check_hip(hipStreamSynchronize(b->stream), "sync"))) return rc;
-  *e = (int64_t)h[0];
-  return SPK_HIP_OK;
+  hipLaunchKernelGGL(k_fold_rows, dim3(1), dim3(64), 0, b->stream, row, kAccSlots, kAccRow, b->d_slot + 4 * (size_t)id);
+  int rc = check_hip(hipGetLastError(), "energy launch");
+  if (!rc)
+    rc = check_hip(hipMemcpyAsync(b->h_slot + 4 * (size_t)id, b->d_slot + 4 * (size_t)id, sizeof(unsigned long long),
+                                  hipMemcpyDeviceToHost, b->stream),
+                   "energy readback");
+  if (!rc) R.energy_pending = true;
+  return rc;
+}
+
+// Takes the initial energies that have arrived in pinned memory; the
+// caller has synchronised the batch stream.
+void resolve_energies(spk_hip_batch *b) {
+  for (int id = 0; id < (int)b->reps.size(); id++) {
+    Replica &R = b->reps[id];
+    if (!R.energy_pending) continue;
+    R.energy = (int64_t)b->h_slot[4 * (size_t)id];
+    R.energy_pending = false;
+  }
+}
+
+// Queues the checksum of replica slot `id` (after everything already on the
+// batch stream) into its device slot and pinned copy.
+int replica_checksum_async(spk_hip_batch *b, int id) {
+  const Replica &R = b->reps[id];
+  unsigned long long *out = b->d_slot + 4 * (size_t)id + 2;
+  hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, b->stream, out, (int64_t)1);
+  const int64_t n = (int64_t)R.g.M * R.g.K;
+  with_type(R.stype, [&]<typename T>() {
+    hipLaunchKernelGGL((k_checksum<T>), dim3(grid_for(n, 256)), dim3(256), 0, b->stream,
+                       reinterpret_cast<const T *>(b->pool + R.spin_off),
+                       reinterpret_cast<const int32_t *>(b->pool + R.seed_off), R.g, out);
+  });
+  int rc = check_hip(hipGetLastError(), "checksum launch");
+  if (!rc)
+    rc = check_hip(hipMemcpyAsync(b->h_slot + 4 * (size_t)id + 2, out, sizeof(unsigned long long),
+                                  hipMemcpyDeviceToHost, b->stream),
+                   "checksum readback");
+  return rc;
 }
 
 // Builds the block's replica table and, per variant present, the step
@@ -470,10 +511,16 @@ int spk_hip_batch_create(int64_t pool_bytes, int max_replicas, spk_hip_batch **o
   }
   b->ncu = prop.multiProcessorCount;
   b->pool_bytes = round_up(pool_bytes, 256);
-  if (hipMalloc(&b->pool, (size_t)b->pool_bytes) != hipSuccess ||
-      hipMalloc(&b->scratch, (kAccRow + 2) * sizeof(unsigned long long)) != hipSuccess) {
+  if (hipMalloc(&b->pool, (size_t)b->pool_bytes) != hipSuccess) {
     spk_hip_batch_destroy(b);
     return set_error(SPK_HIP_ENOMEM, "device pool of %lld bytes not available", (long long)pool_bytes);
+  }
+  const size_t slots = (size_t)max_replicas;
+  if (hipMalloc(&b->d_erow, slots * kAccRow * sizeof(unsigned long long)) != hipSuccess ||
+      hipMalloc(&b->d_slot, slots * 4 * sizeof(unsigned long long)) != hipSuccess ||
+      hipHostMalloc(&b->h_slot, slots * 4 * sizeof(unsigned long long), hipHostMallocDefault) != hipSuccess) {
+    spk_hip_batch_destroy(b);
+    return set_error(SPK_HIP_ENOMEM, "per-replica rows for %d replicas not available", max_replicas);
   }
   b->alloc = new PoolAlloc(b->pool_bytes);
   b->reps.resize(max_replicas);
@@ -495,7 +542,10 @@ void spk_hip_batch_destroy(spk_hip_batch *b) {
   }
   if (b->fork_ev) (void)hipEventDestroy(b->fork_ev);
   if (b->pool) (void)hipFree(b->pool);
-  if (b->scratch) (void)hipFree(b->scratch);
+  if (b->d_erow) (void)hipFree(b->d_erow);
+  if (b->d_slot) (void)hipFree(b->d_slot);
+  if (b->h_slot) (void)hipHostFree(b->h_slot);
+  if (b->h_rows) (void)hipHostFree(b->h_rows);
   if (b->stage) (void)hipFree(b->stage);
   if (b->d_reps) (void)hipFree(b->d_reps);
   if (b->d_steps) (void)hipFree(b->d_steps);
@@ -508,7 +558,12 @@ void spk_hip_batch_destroy(spk_hip_batch *b) {
   delete b;
 }
 
-int spk_hip_batch_add(spk_hip_batch *b, const spk_hip_replica_desc *r, int *id) {
+}  // extern "C"
+
+// Admits a replica and queues its set-up and initial energy on the batch
+// stream without waiting (after a block in flight, if any); the energy
+// arrives in pinned memory and is taken when the stream is next waited for.
+int spk::batch_add_async(spk_hip_batch *b, const spk_hip_replica_desc *r, int *id) {
   if (!b || !r || !id) return set_error(SPK_HIP_EINVAL, "null argument");
   Replica R;
   int rc = make_geom(&r->lattice, &R.g);
@@ -564,7 +619,8 @@ int spk_hip_batch_add(spk_hip_batch *b, const spk_hip_replica_desc *r, int *id) 
   });
   hipLaunchKernelGGL(k_init_seeds, dim3(gs), dim3(256), 0, b->stream, seed, R.g, draws[4] * (double)kIM,
                      park_warmup100());
-  if ((rc = check_hip(hipGetLastError(), "replica set-up launch")) || (rc = replica_energy(b, R, &R.energy))) {
+  if ((rc = check_hip(hipGetLastError(), "replica set-up launch")) ||
+      (rc = replica_energy_async(b, slot, R))) {
     b->alloc->release(R.off, R.bytes);
     return rc;
   }
@@ -574,9 +630,20 @@ int spk_hip_batch_add(spk_hip_batch *b, const spk_hip_replica_desc *r, int *id) 
   return SPK_HIP_OK;
 }
 
+extern "C" {
+
+int spk_hip_batch_add(spk_hip_batch *b, const spk_hip_replica_desc *r, int *id) {
+  if (!b || !r || !id) return set_error(SPK_HIP_EINVAL, "null argument");
+  int rc;
+  if ((rc = batch_complete(b)) || (rc = batch_add_async(b, r, id))) return rc;
+  if ((rc = check_hip(hipStreamSynchronize(b->stream), "replica set-up sync"))) return rc;
+  resolve_energies(b);
+  return SPK_HIP_OK;
+}
+
 int spk_hip_batch_remove(spk_hip_batch *b, int id) {
   int rc = check_id(b, id);
-  if (rc) return rc;
+  if (rc || (rc = batch_complete(b))) return rc;
   Replica &R = b->reps[id];
   b->alloc->release(R.off, R.bytes);
   R = Replica();
@@ -585,7 +652,7 @@ int spk_hip_batch_remove(spk_hip_batch *b, int id) {
 
 int spk_hip_batch_upload(spk_hip_batch *b, int id, const int32_t *spin, const int32_t *seed) {
   int rc = check_id(b, id);
-  if (rc) return rc;
+  if (rc || (rc = batch_complete(b))) return rc;
   Replica &R = b->reps[id];
   const int64_t n = (int64_t)R.g.M * R.g.K;
   if ((rc = ensure_stage(b, n))) return rc;
@@ -610,14 +677,15 @@ int spk_hip_batch_upload(spk_hip_batch *b, int id, const int32_t *spin, const in
     }
     if ((rc = check_hip(hipGetLastError(), "layout kernel"))) return rc;
   }
+  if (spin && (rc = replica_energy_async(b, id, R))) return rc;
   if ((rc = check_hip(hipStreamSynchronize(b->stream), "upload sync"))) return rc;
-  if (spin) return replica_energy(b, R, &R.energy);
+  resolve_energies(b);
   return SPK_HIP_OK;
 }
 
 int spk_hip_batch_download(spk_hip_batch *b, int id, int32_t *spin, int32_t *seed) {
   int rc = check_id(b, id);
-  if (rc) return rc;
+  if (rc || (rc = batch_complete(b))) return rc;
   Replica &R = b->reps[id];
   const int64_t n = (int64_t)R.g.M * R.g.K;
   if ((rc = ensure_stage(b, n))) return rc;
@@ -654,10 +722,147 @@ int spk_hip_batch_set_launch(spk_hip_batch *b, int block, int grid) {
   return SPK_HIP_OK;
 }
 
+}  // extern "C"
+
+namespace spk {
+
+bool batch_inflight(const spk_hip_batch *b) { return b && b->inflight; }
+
+// Folds the block's accumulator rows and queues their copy to pinned memory.
+static int queue_rows(spk_hip_batch *b, int rows) {
+  if (rows > b->cap_h_rows) {   // only between blocks: nothing reads h_rows now
+    unsigned long long *p = nullptr;
+    const int cap = std::max(rows, 2 * b->cap_h_rows);
+    if (hipHostMalloc(&p, (size_t)cap * 2 * sizeof(unsigned long long), hipHostMallocDefault) != hipSuccess)
+      return set_error(SPK_HIP_ENOMEM, "pinned rows for %d sweeps not available", cap);
+    if (b->h_rows) (void)hipHostFree(b->h_rows);
+    b->h_rows = p;
+    b->cap_h_rows = cap;
+  }
+  hipLaunchKernelGGL(k_fold_rows, dim3(rows), dim3(64), 0, b->stream, b->d_acc, kBatchSlots, kBatchRow, b->d_accc);
+  int rc = check_hip(hipGetLastError(), "fold launch");
+  if (!rc)
+    rc = check_hip(hipMemcpyAsync(b->h_rows, b->d_accc, 2 * (size_t)rows * sizeof(unsigned long long),
+                                  hipMemcpyDeviceToHost, b->stream),
+                   "accumulator readback");
+  return rc;
+}
+
+// After the stream has passed the block: each replica's per-sweep Naccept
+// and energy from the folded rows.
+static void apply_rows(spk_hip_batch *b) {
+  resolve_energies(b);
+  for (auto &R : b->reps) {
+    if (!R.live || R.block_sweeps == 0) continue;
+    for (int s = 0; s < R.block_sweeps; s++) {
+      R.naccept += (int64_t)b->h_rows[2 * (size_t)(R.acc_row + s)];
+      R.energy += 2 * (int64_t)b->h_rows[2 * (size_t)(R.acc_row + s) + 1];
+      R.trace_acc.push_back(R.naccept);
+      R.trace_eng.push_back(R.energy);
+    }
+    R.done += R.block_sweeps;
+    R.block_sweeps = 0;
+  }
+}
+
+// Queues one block (every replica with sweeps left advances by up to
+// max_sweeps) with the launch shape already set, and returns without
+// waiting.  Work queued on the batch stream afterwards (admissions,
+// checksums) runs after the block.
+int batch_launch(spk_hip_batch *b, int max_sweeps, bool use_graph) {
+  if (b->inflight) return set_error(SPK_HIP_ESTATE, "a block is already in flight");
+  if (b->block == 0) return set_error(SPK_HIP_ESTATE, "no launch shape set");
+  int rc;
+  int L = 0, rows = 0;
+  int64_t updates = 0;
+  if ((rc = build_schedule(b, max_sweeps, &L, &rows, &updates))) return rc;
+  b->last_updates = updates;
+  b->last_nlaunch = (int64_t)L * (int64_t)std::max<size_t>(1, b->variants.size());
+  b->last_ms = 0.0;
+  if (L == 0) return SPK_HIP_OK;
+  if ((rc = ensure_events(b, 2 * (size_t)L + 2)) || (rc = ensure_streams(b)) ||
+      (rc = upload_schedule(b, rows, true)))
+    return rc;
+  const Shape sh{b->block, b->grid};
+  int mask = 0;
+  for (int v : b->variants) mask |= 1 << v;
+  if (use_graph) {
+    auto key = std::make_tuple(L, sh.block, sh.grid, mask);
+    auto it = b->graphs.find(key);
+    if (it == b->graphs.end()) {
+      hipGraph_t graph = nullptr;
+      if ((rc = check_hip(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal), "begin capture")))
+        return rc;
+      rc = launch_steps(b, 0, L, L, sh.block, sh.grid);
+      const hipError_t e = hipStreamEndCapture(b->stream, &graph);
+      if (!rc) rc = check_hip(e, "end capture");
+      if (rc) {
+        if (graph) (void)hipGraphDestroy(graph);
+        return rc;
+      }
+      hipGraphExec_t exec;
+      rc = check_hip(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0), "graph instantiate");
+      (void)hipGraphDestroy(graph);
+      if (rc) return rc;
+      it = b->graphs.emplace(key, exec).first;
+    }
+    if ((rc = check_hip(hipEventRecord(b->events[0], b->stream), "event")) ||
+        (rc = check_hip(hipGraphLaunch(it->second, b->stream), "graph launch")) ||
+        (rc = check_hip(hipEventRecord(b->events[1], b->stream), "event")))
+      return rc;
+  } else {
+    if ((rc = check_hip(hipEventRecord(b->events[0], b->stream), "event")) ||
+        (rc = launch_steps(b, 0, L, L, sh.block, sh.grid)) ||
+        (rc = check_hip(hipEventRecord(b->events[1], b->stream), "event")))
+      return rc;
+  }
+  if ((rc = queue_rows(b, rows))) return rc;
+  b->inflight = true;
+  b->inflight_rows = rows;
+  return SPK_HIP_OK;
+}
+
+// Waits for the batch stream; applies the rows of the block in flight (if
+// any) and the initial energies that have arrived.
+int batch_complete(spk_hip_batch *b) {
+  int rc;
+  if ((rc = check_hip(hipStreamSynchronize(b->stream), "run sync"))) return rc;
+  if (!b->inflight) {
+    resolve_energies(b);
+    return SPK_HIP_OK;
+  }
+  b->inflight = false;
+  float ms = 0.f;
+  if ((rc = check_hip(hipEventElapsedTime(&ms, b->events[0], b->events[1]), "event time"))) return rc;
+  b->last_ms = ms;
+  apply_rows(b);
+  return SPK_HIP_OK;
+}
+
+// Queues the checksum of a replica after everything on the batch stream;
+// batch_checksum_value reads it once the stream has passed it.
+int batch_queue_checksum(spk_hip_batch *b, int id) { return replica_checksum_async(b, id); }
+uint64_t batch_checksum_value(const spk_hip_batch *b, int id) { return b->h_slot[4 * (size_t)id + 2]; }
+
+int batch_left(const spk_hip_batch *b, int id) {
+  const Replica &R = b->reps[id];
+  return R.live ? R.desc.nsweeps - R.done : 0;
+}
+
+}  // namespace spk
+
+extern "C" {
+
 int spk_hip_batch_run(spk_hip_batch *b, int max_sweeps, int use_graph) {
   if (!b) return set_error(SPK_HIP_EINVAL, "null batch");
   if (max_sweeps <= 0) return set_error(SPK_HIP_EINVAL, "max_sweeps must be positive");
   int rc;
+  if ((rc = batch_complete(b))) return rc;
+  if (b->block != 0) {
+    if ((rc = batch_launch(b, max_sweeps, use_graph != 0))) return rc;
+    return batch_complete(b);
+  }
+  // autotune on this block's own steps, then its rows as a launched block's
   int L = 0, rows = 0;
   int64_t updates = 0;
   if ((rc = build_schedule(b, max_sweeps, &L, &rows, &updates))) return rc;
@@ -670,7 +875,6 @@ int spk_hip_batch_run(spk_hip_batch *b, int max_sweeps, int use_graph) {
   if ((rc = ensure_events(b, 2 * (size_t)L + 2)) || (rc = ensure_streams(b)) ||
       (rc = upload_schedule(b, rows, true)))
     return rc;
-
   if (b->block == 0) {
     // autotune: candidate shapes (workgroup size x wavefronts per CU) run
     // the block's own steps, as a replay runs them (no wait between steps);
@@ -739,72 +943,16 @@ int spk_hip_batch_run(spk_hip_batch *b, int max_sweeps, int use_graph) {
       tot += ms;
     }
     b->last_ms = tot;
-  } else {
-    const Shape sh{b->block, b->grid};
-    int mask = 0;
-    for (int v : b->variants) mask |= 1 << v;
-    if (use_graph) {
-      auto key = std::make_tuple(L, sh.block, sh.grid, mask);
-      auto it = b->graphs.find(key);
-      if (it == b->graphs.end()) {
-        hipGraph_t graph = nullptr;
-        if ((rc = check_hip(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal), "begin capture")))
-          return rc;
-        rc = launch_steps(b, 0, L, L, sh.block, sh.grid);
-        const hipError_t e = hipStreamEndCapture(b->stream, &graph);
-        if (!rc) rc = check_hip(e, "end capture");
-        if (rc) {
-          if (graph) (void)hipGraphDestroy(graph);
-          return rc;
-        }
-        hipGraphExec_t exec;
-        rc = check_hip(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0), "graph instantiate");
-        (void)hipGraphDestroy(graph);
-        if (rc) return rc;
-        it = b->graphs.emplace(key, exec).first;
-      }
-      if ((rc = check_hip(hipEventRecord(b->events[0], b->stream), "event")) ||
-          (rc = check_hip(hipGraphLaunch(it->second, b->stream), "graph launch")) ||
-          (rc = check_hip(hipEventRecord(b->events[1], b->stream), "event")))
-        return rc;
-    } else {
-      if ((rc = check_hip(hipEventRecord(b->events[0], b->stream), "event")) ||
-          (rc = launch_steps(b, 0, L, L, sh.block, sh.grid)) ||
-          (rc = check_hip(hipEventRecord(b->events[1], b->stream), "event")))
-        return rc;
-    }
-    if ((rc = check_hip(hipStreamSynchronize(b->stream), "run sync"))) return rc;
-    float ms = 0.f;
-    if ((rc = check_hip(hipEventElapsedTime(&ms, b->events[0], b->events[1]), "event time"))) return rc;
-    b->last_ms = ms;
   }
-
-  // per-replica Naccept / energy per sweep of the block
-  b->h_acc.resize(2 * (size_t)rows);
-  hipLaunchKernelGGL(k_fold_rows, dim3(rows), dim3(64), 0, b->stream, b->d_acc, kBatchSlots,
-                     kBatchRow, b->d_accc);
-  if ((rc = check_hip(hipGetLastError(), "fold launch"))) return rc;
-  if ((rc = check_hip(hipMemcpyAsync(b->h_acc.data(), b->d_accc, 2 * (size_t)rows * sizeof(unsigned long long),
-                                     hipMemcpyDeviceToHost, b->stream),
-                      "accumulator readback")) ||
-      (rc = check_hip(hipStreamSynchronize(b->stream), "readback sync")))
+  if ((rc = queue_rows(b, rows)) || (rc = check_hip(hipStreamSynchronize(b->stream), "readback sync")))
     return rc;
-  for (auto &R : b->reps) {
-    if (!R.live || R.block_sweeps == 0) continue;
-    for (int s = 0; s < R.block_sweeps; s++) {
-      R.naccept += (int64_t)b->h_acc[2 * (size_t)(R.acc_row + s)];
-      R.energy += 2 * (int64_t)b->h_acc[2 * (size_t)(R.acc_row + s) + 1];
-      R.trace_acc.push_back(R.naccept);
-      R.trace_eng.push_back(R.energy);
-    }
-    R.done += R.block_sweeps;
-  }
+  apply_rows(b);
   return SPK_HIP_OK;
 }
 
 int spk_hip_batch_status(spk_hip_batch *b, int id, int *done, int *left, int64_t *naccept, double *energy) {
   int rc = check_id(b, id);
-  if (rc) return rc;
+  if (rc || (rc = batch_complete(b))) return rc;
   const Replica &R = b->reps[id];
   if (done) *done = R.done;
   if (left) *left = R.desc.nsweeps - R.done;
@@ -815,7 +963,7 @@ int spk_hip_batch_status(spk_hip_batch *b, int id, int *done, int *left, int64_t
 
 int spk_hip_batch_trace(spk_hip_batch *b, int id, int n, int64_t *naccept, double *energy) {
   int rc = check_id(b, id);
-  if (rc) return rc;
+  if (rc || (rc = batch_complete(b))) return rc;
   const Replica &R = b->reps[id];
   if (n < 0 || n > (int)R.trace_acc.size()) return set_error(SPK_HIP_EINVAL, "trace has %d sweeps", (int)R.trace_acc.size());
   for (int s = 0; s < n; s++) {
@@ -852,20 +1000,10 @@ int spk_hip_batch_checksum(spk_hip_batch *b, int id, uint64_t *sum) {
   int rc = check_id(b, id);
   if (rc) return rc;
   if (!sum) return set_error(SPK_HIP_EINVAL, "null argument");
-  const Replica &R = b->reps[id];
-  const int64_t n = (int64_t)R.g.M * R.g.K;
-  if ((rc = check_hip(hipMemsetAsync(b->scratch, 0, sizeof(unsigned long long), b->stream), "memset"))) return rc;
-  with_type(R.stype, [&]<typename T>() {
-    hipLaunchKernelGGL((k_checksum<T>), dim3(grid_for(n, 256)), dim3(256), 0, b->stream,
-                       reinterpret_cast<const T *>(b->pool + R.spin_off),
-                       reinterpret_cast<const int32_t *>(b->pool + R.seed_off), R.g, b->scratch);
-  });
-  if ((rc = check_hip(hipGetLastError(), "checksum launch"))) return rc;
-  unsigned long long v = 0;
-  if ((rc = check_hip(hipMemcpyAsync(&v, b->scratch, sizeof v, hipMemcpyDeviceToHost, b->stream), "readback")) ||
+  if ((rc = batch_complete(b)) || (rc = replica_checksum_async(b, id)) ||
       (rc = check_hip(hipStreamSynchronize(b->stream), "sync")))
     return rc;
-  *sum = v;
+  *sum = b->h_slot[4 * (size_t)id + 2];
   return SPK_HIP_OK;
 }
 

@@ -1159,6 +1159,19 @@ __device__ __forceinline__ void wave_lds_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+// block_accumulate for one wavefront alone: one wavefront reduction per
+// value (32-bit: a wavefront's sums stay far below 2^31), then one atomic per
+// value from lane 0.  Uses no LDS and waits for no other wavefront.
+__device__ __forceinline__ void wave_accumulate(int x, int y, unsigned long long *acc, int slot) {
+  x = wave_sum(x);
+  y = wave_sum(y);
+  if ((threadIdx.x & 63) == 0) {
+    unsigned long long *a = acc + (size_t)slot * kSlotStride;
+    if (x) atomicAdd(&a[0], (unsigned long long)(long long)x);
+    if (y) atomicAdd(&a[1], (unsigned long long)(long long)y);
+  }
+}
+
 // block_accumulate_pack for one wavefront alone (same bounds on x and y):
 // one wavefront reduction, then one atomic per value from lane 0.  Uses no
 // LDS and waits for no other wavefront.

@@ -1,12 +1,19 @@
 // spk_serve.hip -- libspk_hip.so, serving: continuous batching of a stream of
-// solves over the batched-replica API (see spk_hip_serve in spk_hip.h).
+// solves over the batched-replica path (see spk_hip_serve in spk_hip.h).
 // Host code only: the device work is the batch's step kernels.
+//
+// One block is in flight at a time.  While it runs, the host admits the
+// requests that have arrived (their set-up kernels queue behind the block)
+// and queues the checksums of the requests the block finishes; it then waits
+// once per block, reports and retires those requests, and launches the next
+// block.  The host waits for nothing else.
 #include <algorithm>
 #include <chrono>
 #include <deque>
 #include <thread>
 #include <vector>
 
+#include "spk_batch_async.h"
 #include "spk_host.h"
 
 using namespace spk;
@@ -35,19 +42,22 @@ extern "C" int spk_hip_serve(const spk_hip_request *req, int n, int64_t pool_byt
   int rc = spk_hip_batch_create(pool_bytes, max_live, &g.b);
   if (rc) return rc;
   if ((rc = spk_hip_batch_set_launch(g.b, block, grid))) return rc;
+  bool tuned = block != 0;   // a zero shape is tuned on the first block, which waits
 
   std::vector<int> owner(max_live, -1);   // replica id -> request index
   std::deque<int> waiting;                // arrived, not yet admitted (FIFO)
   int next = 0, completed = 0, live = 0;
   const auto t0 = std::chrono::steady_clock::now();
-  while (completed < n) {
-    double t = now_s(t0);
+
+  // admits, in arrival order, every arrived request while the pool and the
+  // slot table have room; requests that can never run are refused alone
+  auto admit = [&]() -> int {
+    const double t = now_s(t0);
     while (next < n && req[next].arrive <= t) waiting.push_back(next++);
-    // admit in arrival order while the pool and the slot table have room
     while (!waiting.empty()) {
       const int r = waiting.front();
       int id = -1;
-      rc = spk_hip_batch_add(g.b, &req[r].replica, &id);
+      const int rc = batch_add_async(g.b, &req[r].replica, &id);
       if (rc == SPK_HIP_ENOMEM) {
         if (live == 0) {   // larger than the empty pool: can never run
           out[r] = spk_hip_result{t, t, 0, 0.0, 0, SPK_HIP_ENOMEM};
@@ -73,31 +83,49 @@ extern "C" int spk_hip_serve(const spk_hip_request *req, int n, int64_t pool_byt
     int64_t used = 0;
     spk_hip_batch_pool(g.b, &used, nullptr, nullptr);
     st.max_pool_used = std::max(st.max_pool_used, used);
+    return SPK_HIP_OK;
+  };
+
+  std::vector<int> finishing;
+  while (completed < n) {
+    if ((rc = admit())) return rc;
     if (live == 0) {
       if (next >= n) break;
       const double wait = req[next].arrive - now_s(t0);
       if (wait > 0) std::this_thread::sleep_for(std::chrono::duration<double>(wait));
       continue;
     }
-    const double tb = now_s(t0);
-    if ((rc = spk_hip_batch_run(g.b, block_sweeps, 1))) return rc;
-    st.busy += now_s(t0) - tb;
+    // the requests this block finishes, known before it runs
+    finishing.clear();
+    for (int id = 0; id < max_live; id++)
+      if (owner[id] >= 0 && batch_left(g.b, id) <= block_sweeps) finishing.push_back(id);
+    if (!tuned) {
+      // the first block tunes the launch shape on its own steps and waits
+      if ((rc = spk_hip_batch_run(g.b, block_sweeps, 1))) return rc;
+      tuned = true;
+    } else if ((rc = batch_launch(g.b, block_sweeps, true))) {
+      return rc;
+    }
+    for (int id : finishing)
+      if ((rc = batch_queue_checksum(g.b, id))) return rc;
+    // while the block runs: admit what has arrived (queued behind it)
+    if ((rc = admit())) return rc;
+    if ((rc = batch_complete(g.b))) return rc;
+    double ms = 0.0;
+    spk_hip_batch_last_timing(g.b, &ms, nullptr, nullptr, nullptr, nullptr);
+    st.busy += ms / 1e3;
     st.blocks++;
-    // retire finished requests at once
-    for (int id = 0; id < max_live; id++) {
+    // report and retire the finished requests
+    const double tf = now_s(t0);
+    for (int id : finishing) {
       const int r = owner[id];
-      if (r < 0) continue;
-      int done = 0, left = 0;
       int64_t nacc = 0;
       double e = 0.0;
-      if ((rc = spk_hip_batch_status(g.b, id, &done, &left, &nacc, &e))) return rc;
-      if (left > 0) continue;
-      uint64_t sum = 0;
-      if ((rc = spk_hip_batch_checksum(g.b, id, &sum))) return rc;
-      out[r].finish = now_s(t0);
+      if ((rc = spk_hip_batch_status(g.b, id, nullptr, nullptr, &nacc, &e))) return rc;
+      out[r].finish = tf;
       out[r].naccept = nacc;
       out[r].energy = e;
-      out[r].checksum = sum;
+      out[r].checksum = batch_checksum_value(g.b, id);
       const spk_hip_desc &d = req[r].replica.lattice;
       st.site_updates += (int64_t)d.nx * d.ny * d.nz * req[r].replica.nsweeps;
       if ((rc = spk_hip_batch_remove(g.b, id))) return rc;
