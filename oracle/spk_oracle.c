@@ -14,6 +14,12 @@
  *
  * Build: gcc -O2 -ffp-contract=off -fopenmp (no -ffast-math, no -march: the
  * reference is built with plain -O on x86-64, which has no FMA contraction).
+ *
+ * Parity unpinned: orc_color's numbering of the sq/4n, sc/6n and sc/26n
+ * colours.  It restates Lattice::id2color (src/lattice.cpp:234-283) as
+ * read; the partition invariants the tests check hold equally for a
+ * relabelled numbering, and no reference-held vector covers these lattices
+ * (only the sq/8n numbering is fixed, by the recorded 128x128 Potts run).
  */
 #include "spk_oracle.h"
 
