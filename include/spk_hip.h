@@ -129,6 +129,22 @@ int spk_hip_init_seeds(spk_hip_lattice *h, double rseed);
 int spk_hip_sweeps(spk_hip_lattice *h, int nsweeps, int64_t *naccept_total,
                    int64_t *naccept_per_sweep, double *energy_per_sweep);
 
+/* spk_hip_sweeps for callers that cannot allocate memory when they run
+   sweeps: code under HIP graph capture, serving loops, harnesses that
+   forbid hipMalloc.  No counterpart in the reference.  It allocates
+   nothing and does not wait: it queues nsweeps sweeps (0..256) on the
+   handle's stream, through the per-colour kernels on the byte layout
+   (bit planes are unpacked first), with the accumulator rows in a
+   library-owned device array of 2 MiB per GPU, and writes 2*nsweeps
+   uint64 to `rows`, device memory the caller owns: for sweep s,
+   rows[2s] = accepted flips and rows[2s+1] = the sum of ef-ei over them as
+   a two's-complement int64, so the energy after sweep s is the energy
+   before the call plus twice the running sum.  Results are those of
+   spk_hip_sweeps once the stream reaches them (spk_hip_sync).  All calls on
+   one GPU share the array: calls that overlap on one GPU, from different
+   handles or threads, race. */
+int spk_hip_sweeps_noalloc(spk_hip_lattice *h, int nsweeps, unsigned long long *rows);
+
 /* Total energy: sum over sites of #unlike neighbours
    (src/diag_energy.cpp:53-66, exact in double). */
 int spk_hip_energy(spk_hip_lattice *h, double *energy);

@@ -7,6 +7,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <new>
 #include <string>
 #include <algorithm>
@@ -567,6 +568,31 @@ int compute_energy(spk_hip_lattice *h, int64_t *e) {
   return SPK_HIP_OK;
 }
 
+// ---------------------------------------------- sweeps without allocation
+// Accumulator rows of spk_hip_sweeps_noalloc live in a file-scope device
+// array, one copy per GPU, held until the process ends.  Host code uses the
+// address hipGetSymbolAddress gives on each device (looked up once per
+// device, when the first handle is created there), never the array's name,
+// which in host code is only a stand-in.
+constexpr int kNoAllocSweeps = 256;   // sweeps per call (8 KiB of rows each: 2 MiB)
+constexpr int kMaxDevices = 64;
+static __device__ __align__(256) unsigned long long scratch[(size_t)kNoAllocSweeps * kAccRow];
+std::mutex g_scratch_mu;
+unsigned long long *g_scratch[kMaxDevices] = {};
+
+// Looks up the scratch array's address on the current device `device`.
+int scratch_lookup(int device) {
+  if (device < 0 || device >= kMaxDevices)
+    return set_error(SPK_HIP_EDEVICE, "device %d beyond the %d this library serves", device, kMaxDevices);
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (g_scratch[device]) return SPK_HIP_OK;
+  void *p = nullptr;
+  const int rc = check_hip(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)), "scratch symbol address");
+  if (rc) return rc;
+  g_scratch[device] = static_cast<unsigned long long *>(p);
+  return SPK_HIP_OK;
+}
+
 int validate_on_device(spk_hip_lattice *h, const int32_t *buf, int lo, int hi, int64_t *nbad) {
   int rc = ensure_acc(h, 1);
   if (rc) return rc;
@@ -612,7 +638,7 @@ int spk_hip_create(const spk_hip_desc *d, spk_hip_lattice **out) {
   h->stype = spin_type_for(d->app, d->nspins);
   // measured best on MI355X: 512 for 4096^2 sq/4n, 256 for 256^3 sc/26n
   h->block = (g.lat == SPK_HIP_SQ_4N || g.lat == SPK_HIP_SQ_8N) ? 512 : 256;
-  if ((rc = check_hip(hipGetDevice(&h->device), "hipGetDevice")) ||
+  if ((rc = check_hip(hipGetDevice(&h->device), "hipGetDevice")) || (rc = scratch_lookup(h->device)) ||
       (rc = check_hip(hipStreamCreate(&h->stream), "hipStreamCreate")) ||
       (rc = check_hip(hipEventCreate(&h->ev0), "hipEventCreate")) ||
       (rc = check_hip(hipEventCreate(&h->ev1), "hipEventCreate"))) {
@@ -838,6 +864,28 @@ int spk_hip_sweeps(spk_hip_lattice *h, int nsweeps, int64_t *naccept_total, int6
   }
   if ((rc = check_hip(hipEventRecord(h->ev1, h->stream), "event"))) return rc;
   return finish_sweeps(h, nsweeps, nlaunch, e0, naccept_total, naccept_per_sweep, energy_per_sweep);
+}
+
+int spk_hip_sweeps_noalloc(spk_hip_lattice *h, int nsweeps, unsigned long long *rows) {
+  if (!h || !rows) return set_error(SPK_HIP_EINVAL, "null argument");
+  if (nsweeps < 0 || nsweeps > kNoAllocSweeps)
+    return set_error(SPK_HIP_EINVAL, "nsweeps %d outside [0, %d] for sweeps without allocation", nsweeps,
+                     kNoAllocSweeps);
+  if (nsweeps == 0) return SPK_HIP_OK;
+  unsigned long long *acc = g_scratch[h->device];   // looked up when the handle was created
+  if (!acc) return set_error(SPK_HIP_ESTATE, "no scratch rows on device %d", h->device);
+  // the state in the byte layout (an unpack when the planes hold it: kernels
+  // only, into buffers that exist), then per-colour launches on the stream
+  int rc = bits_leave(h);
+  if (rc) return rc;
+  const int64_t words = (int64_t)nsweeps * kAccRow;
+  hipLaunchKernelGGL(k_zero_u64, dim3(grid_for(words, 256)), dim3(256), 0, h->stream, acc, words);
+  int64_t nlaunch = 0;
+  if ((rc = check_hip(hipGetLastError(), "zero launch")) || (rc = launch_sweeps(h, nsweeps, acc, &nlaunch)) ||
+      (rc = launch_fold(h, acc, nsweeps, rows)))
+    return rc;
+  h->energy_valid = false;   // the energy changed on the device only
+  return SPK_HIP_OK;
 }
 
 int spk_hip_prepare(spk_hip_lattice *h, int nsweeps) {

@@ -31,7 +31,7 @@ ABI_SYMBOLS = (
     "spk_hip_batch_download", "spk_hip_batch_run", "spk_hip_batch_status",
     "spk_hip_batch_trace", "spk_hip_batch_set_launch", "spk_hip_batch_last_timing",
     "spk_hip_batch_pool", "spk_hip_checksum", "spk_hip_batch_checksum", "spk_hip_serve",
-    "spk_hip_set_graph", "spk_hip_prepare",
+    "spk_hip_set_graph", "spk_hip_prepare", "spk_hip_sweeps_noalloc",
 )
 
 
@@ -83,6 +83,7 @@ def load_library(path=None):
     L.spk_hip_checksum.argtypes = [vp, C.POINTER(C.c_uint64)]
     L.spk_hip_set_graph.argtypes = [vp, C.c_int]
     L.spk_hip_prepare.argtypes = [vp, C.c_int]
+    L.spk_hip_sweeps_noalloc.argtypes = [vp, C.c_int, vp]
     L.spk_hip_last_timing.argtypes = [vp, C.POINTER(C.c_double), C.POINTER(C.c_int64)]
     if path is None:
         _lib = L
@@ -250,6 +251,12 @@ class Lattice:
 
     def set_graph(self, chunk_sweeps):
         _check(self.L.spk_hip_set_graph(self.h, int(chunk_sweeps)))
+
+    def sweeps_noalloc(self, nsweeps, rows_ptr):
+        """Queue nsweeps sweeps that allocate nothing; per-sweep (Naccept,
+        dE) pairs go to the caller's device buffer at rows_ptr (2*nsweeps
+        uint64).  Asynchronous: sync() before reading rows."""
+        _check(self.L.spk_hip_sweeps_noalloc(self.h, int(nsweeps), C.c_void_p(int(rows_ptr))))
 
     def prepare(self, nsweeps):
         """Capture the graphs (and pack the bit planes) a sweeps(nsweeps)

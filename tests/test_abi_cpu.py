@@ -122,6 +122,7 @@ def test_every_entry_point_rejects_null_arguments():
         "spk_hip_set_resident": (null, 1),
         "spk_hip_set_bitplanes": (null, 1),
         "spk_hip_prepare": (null, 1),
+        "spk_hip_sweeps_noalloc": (null, 1, null),
         "spk_hip_set_graph": (null, 32),
         "spk_hip_last_timing": (null, C.byref(dbl), C.byref(i64)),
         "spk_hip_batch_create": (C.c_int64(1 << 20), 4, null),
