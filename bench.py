@@ -121,6 +121,14 @@ def cpu_baseline(app, lattice, shape, q, kt, seed, budget_s, threads):
             "setup_s": round(setup_s, 4)}
 
 
+def ensemble_variant(r):
+    """The step-kernel variant that runs a replica: ising, or potts-u8 /
+    potts-u16 / potts-i32 by spin storage."""
+    if r["app"] == "ising":
+        return "ising"
+    return "potts-" + {1: "u8", 2: "u16", 4: "i32"}[spin_bytes(r["app"], r["nspins"])]
+
+
 def _hash(a, b):
     import hashlib
     h = hashlib.blake2b(digest_size=16)
@@ -141,6 +149,8 @@ def run_ensemble(args):
     from spparks_amd.workloads import ensemble_replicas
 
     reps = ensemble_replicas(sweeps=args.steps or 100)
+    if args.ensemble_variant:   # the replicas of one step-kernel variant only
+        reps = [r for r in reps if ensemble_variant(r) == args.ensemble_variant]
     nsw = reps[0]["sweeps"]
     block = args.block_sweeps
     descs = [bmod.replica_desc(r["app"], r["lattice"], *r["shape"], nspins=r["nspins"], temperature=r["kT"],
@@ -214,7 +224,9 @@ def run_ensemble(args):
         "vs_baseline": None, "dtype": "int32", "data": "synthetic: 512 replicas from "
         "spparks_amd.workloads.ensemble_replicas(), each initialised from its own seed",
         "config": {"workload": "ensemble-512: batched ensemble of 512 replicas (256 Ising sq/4n L 64-1024, 256 Potts "
-                               "sc/26n L 32-128, Q 100-1e5), %d sweeps each in %d-sweep graph blocks" % (nsw, block),
+                               "sc/26n L 32-128, Q 100-1e5), %d sweeps each in %d-sweep graph blocks%s"
+                               % (nsw, block, "; only the %d %s replicas" % (len(reps), args.ensemble_variant)
+                                  if args.ensemble_variant else ""),
                    "replicas": len(reps), "sites": sum(sites), "site_updates": updates,
                    "launch_shape": {"block": shape[0], "grid": shape[1]}},
         "batched_vs_plain_speedup": round(value / plain, 3) if plain else None,
@@ -461,6 +473,8 @@ def main():
     ap.add_argument("--skip-plain", action="store_true",
                     help="ensemble: time the batched run only (no one-at-a-time comparison)")
     ap.add_argument("--block-sweeps", type=int, default=25, help="ensemble: sweeps per graph block")
+    ap.add_argument("--ensemble-variant", default="", choices=["", "ising", "potts-u8", "potts-u16", "potts-i32"],
+                    help="ensemble: only the replicas one step-kernel variant runs")
     ap.add_argument("--grid", type=int, default=0, help="ensemble: persistent workgroups (with --block)")
     ap.add_argument("--block", type=int, default=0, help="sweep-kernel workgroup size (0 = default)")
     ap.add_argument("--no-bitplanes", action="store_true",
@@ -521,7 +535,9 @@ def run_configs(args):
     for wl in ("potts-sq8n-128", "ising-sq4n-4096", "potts-sc26n-256", "potts-neigh-sc26n-256",
                "potts-neighonly-sc26n-256", "ising-single-sq4n-4096"):
         a = copy.copy(args)
-        a.workload, a.steps, a.warmup = wl, None, 5
+        # warm-up sweeps also bring the clocks up after the CPU baseline (the
+        # one-workgroup kernel of cfg1 runs on one CU and feels them most)
+        a.workload, a.steps, a.warmup = wl, None, 64
         a.cpu_seconds = 3.0
         a.no_cpu_baseline = args.no_cpu_baseline or wl not in label
         t0 = time.perf_counter()

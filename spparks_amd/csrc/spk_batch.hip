@@ -881,7 +881,7 @@ int spk_hip_batch_run(spk_hip_batch *b, int max_sweeps, int use_graph) {
     // results do not depend on the shape.  The tile list is the same for
     // every shape (a tile is one wavefront's work).
     std::vector<Shape> cand;
-    for (int bs : {128, 256, 512})
+    for (int bs : {64, 128, 256})
       for (int wpc : {8, 16, 32}) {
         const int grid = std::max(kNumXcd, b->ncu * wpc * 64 / bs / kNumXcd * kNumXcd);
         cand.push_back(Shape{bs, grid});
