@@ -80,7 +80,7 @@ def load_traffic(workload):
             t = json.load(f).get(workload)
     except (OSError, ValueError):
         t = None
-    return (t["bytes_per_launch"] if t else None), t
+    return (t.get("bytes_per_launch") if t else None), t
 
 
 def host_cores():
