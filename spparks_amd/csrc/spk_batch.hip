@@ -64,7 +64,12 @@ __global__ __launch_bounds__(1024) void k_batch_step(char *__restrict__ pool, co
                                                      unsigned long long *__restrict__ acc) {
   __shared__ int s_thr[16][32];   // one threshold table per wavefront
   const StepDesc sd = steps[j];
-  const int wpb = blockDim.x >> 6, wid = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  // the wavefront index is the same on all 64 lanes; saying so puts it, the
+  // chunk, the replica and its geometry in scalar registers and their loads
+  // on the scalar unit (the compiler treats threadIdx-derived values as
+  // per-lane otherwise)
+  const int wpb = blockDim.x >> 6, lane = threadIdx.x & 63;
+  const int wid = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
   const int W = gridDim.x * wpb;
   const int gw = xcd_tile(blockIdx.x, gridDim.x) * wpb + wid;
   const int chunk = (sd.total + W - 1) / W;
