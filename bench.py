@@ -359,11 +359,14 @@ def run(args):
     sb = spin_bytes(app, q)
     B = bytes_per_update(lattice, sb)
     tname = {1: "u8", 2: "u16", 4: "i32"}[sb]
-    # the whole state in one workgroup's LDS: one launch per call
-    resident = nlaunch < steps
+    # the whole state (spins and int32 seeds) in one workgroup's 160 KiB of
+    # LDS: one launch per call (the library's rule; proposal variants never)
+    resident = (not args.no_resident and app not in VARIANTS and n * (sb + 4) <= 160 * 1024
+                and nlaunch < steps)
     planes = (lattice == "sq/4n" and (app in ("ising", "ising/single") or q == 2)
               and not args.no_bitplanes and not resident)
     variant = app in VARIANTS and not planes
+    launches = None   # (two-sweep launches, per-colour launches) of a call on paired planes
     if variant:
         # one site per thread, neighbours gathered in the reference's order;
         # the same bytes as the per-colour design
@@ -377,6 +380,21 @@ def run(args):
         kernel = "k_sweep_resident<%s, %s> (one workgroup, state in LDS)" % (lattice, tname)
         bytes_per_launch = 2 * n * (sb + 4)
         B_kernel = bytes_per_launch / (n * steps / nlaunch)
+    elif planes and nlaunch < K * steps:
+        # two sweeps per launch (k_sweep_bits_pair): each site's int32 seed
+        # is read and written once per launch and its plane bit read and
+        # written once; the halo rows a band reads again are not counted.
+        # A call of n sweeps makes 2*(n//4) such launches and 2*(n%4)
+        # per-colour launches (8 + 3/8 bytes per update each)
+        pairs = (K * steps - nlaunch) // 3
+        colour = nlaunch - pairs
+        kernel = "k_sweep_bits_pair (two sweeps per launch, bands of 16 rows)"
+        bytes_per_launch = int((8 + 2 / 8) * n)
+        B_kernel = bytes_per_launch / (2 * n)
+        # a per-colour launch counted as its share of a pair launch's bytes
+        colour_share = (8 + 3 / 8) * n / K / bytes_per_launch
+        per_launch_s = (dev_ms / 1e3) / (pairs + colour * colour_share)
+        launches = {"two_sweep": pairs, "per_colour": colour}
     elif planes:
         # two-state sq/4n on bit planes: a colour pass reads and writes the
         # int32 seed of each of its sites and moves one bit per site of its
@@ -420,6 +438,7 @@ def run(args):
                      "bytes_per_launch": bytes_per_launch,
                      "avg_launch_us": round(per_launch_s * 1e6, 3),
                      "bytes_per_update": round(B_kernel, 3),
+                     "launches": launches,
                      # the int32-spin byte models of the per-colour design, for
                      # comparison only: frac above uses what this kernel moves
                      "int32_model_bytes_per_update": {"per_colour_4K_plus_12": 4 * K + 12,

@@ -171,10 +171,17 @@ int spk_hip_set_launch(spk_hip_lattice *h, int block);
    spk_hip_sweeps call packs the planes; they then hold the state across
    sweeps calls, and a call that needs the byte layout (upload, download,
    energy, checksum, set_range, set_value, init_seeds, set_bitplanes, the
-   one-workgroup kernel) unpacks them first.  enable = 0 keeps such lattices
-   on the byte-per-site kernel; 1 turns the planes back on; 8, 16 or 32 also
-   sets the sites each thread takes.  Results are identical either way.  No
-   counterpart in the reference (a storage choice of this implementation). */
+   one-workgroup kernel) unpacks them first.  By default (enable = 1) a
+   lattice whose sub-grid rows hold 33..64 plane words (nx 2050..4096) and
+   at least 64 rows runs two whole sweeps per launch: a workgroup owns a
+   band of 16 rows, keeps it with 4 halo rows either side in LDS, and runs
+   the four colour passes as a pipeline over its rows, so each seed is read
+   and written once per two sweeps; the last n % 4 sweeps of a call (and
+   every sweep of other lattices) take the per-colour kernel with 32 sites
+   per thread.  enable = 0 keeps such lattices on the byte-per-site kernel;
+   8, 16 or 32 selects the per-colour kernel with that many sites per thread
+   for every sweep.  Results are identical either way.  No counterpart in
+   the reference (a storage choice of this implementation). */
 int spk_hip_set_bitplanes(spk_hip_lattice *h, int enable);
 
 /* Lattices whose spins and seeds fit one workgroup's LDS (up to about 30 k

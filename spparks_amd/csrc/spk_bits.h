@@ -229,9 +229,14 @@ __host__ __device__ __forceinline__ uint32_t bit_mask(uint32_t x, int i) {
 #endif
 }
 
-// ~0 when a < b (both below 2^31, so a - b cannot overflow), else 0
-__host__ __device__ __forceinline__ uint32_t less_mask(int32_t a, int32_t b) {
-  return (uint32_t)((int32_t)((uint32_t)a - (uint32_t)b) >> 31);
+
+// (acc << 1) | (x >> 31): the sign bit of x shifted in at bit 0
+__host__ __device__ __forceinline__ uint32_t shift_in_sign(uint32_t acc, uint32_t x) {
+#ifdef __HIP_DEVICE_COMPILE__
+  return __builtin_amdgcn_alignbit(acc, x, 31u);
+#else
+  return (acc << 1) | (x >> 31);
+#endif
 }
 
 constexpr uint32_t kSel = truth3([](int x, int y, int z) { return x ? y : z; });            // x ? y : z
@@ -240,23 +245,108 @@ constexpr uint32_t kKeep = truth3([](int x, int y, int z) { return x & (!y | z);
 constexpr uint32_t kOrAnd = truth3([](int x, int y, int z) { return (x & y) | z; });        // (x & y) | z
 
 // -------------------------------------------------------------- the events
-// The SPT site events of thread t of colour c: the events of
-// AppIsing::site_event_rejection (src/app_ising.cpp:104-136) and, for Q=2,
+// The SPT site events of one thread: part q (sites 32q/TPW .. +SPT-1) of
+// word w of a colour-c row whose x-neighbours sit at a and a+1 (p = 1) or
+// a-1 and a (p = 0).  own is the row's word; up and down the other colour's
+// words of the rows above and below; x the other colour's word of the same
+// row and w; xe its neighbour word (w+1 for p = 1, w-1 for p = 0, wrapping
+// at the row's ends).  The events are those of AppIsing::
+// site_event_rejection (src/app_ising.cpp:104-136) and, for Q=2,
 // AppPotts::site_event_rejection (src/app_potts.cpp:112-146), which draw
-// the same way: the first draw proposes 1 (below half) or 2; a change
-// (the proposal differs from the spin) with dE = ef - ei = 4 - 2*ei > 0
-// at T > 0 takes a second draw and is kept when it is below thr[dE]; a
-// change with dE <= 0 is kept without one; T = 0 rejects uphill changes
-// without one.  Returns (changes, their dE sum).
+// the same way: the first draw proposes 1 (below half) or 2; a change (the
+// proposal differs from the spin) with dE = ef - ei = 4 - 2*ei > 0 at T > 0
+// takes a second draw and is kept when it is below thr[dE]; a change with
+// dE <= 0 is kept without one; T = 0 rejects uphill changes without one.
 // SINGLE: AppIsingSingle::site_event_rejection (src/app_ising_single.cpp:
 // 39-77) instead: the proposal is the flip, without a draw; a flip with
 // dE <= 0 is kept, and an uphill one at T > 0 takes the one draw and is
 // kept when it is below thr[dE].
+// sq holds the sites' seeds; sink(j, words) receives the new seeds of local
+// sites 4j..4j+3.  Returns the changed sites as a word mask (bit i = site
+// 32w+i) in ch, and (changes, their dE sum).
+template <int SPT, bool SINGLE, class Sink>
+__host__ __device__ __forceinline__ Counts bits_events(uint32_t own, uint32_t up, uint32_t down, uint32_t x,
+                                                       uint32_t xe, int w, int p, int q, const BitGeom &bg,
+                                                       const Rule &r, const Words<4> (&sq)[SPT / 4], uint32_t &ch,
+                                                       Sink &&sink) {
+  static_assert(SPT == 8 || SPT == 16 || SPT == 32, "8, 16 or 32 sites per thread");
+  const int last = bg.pw - 1;
+  // ---- the shifted x-neighbour word: site i's neighbour at a+1 (p = 1) or
+  // a-1 (p = 0); across a row's ends the periodic image (a = 0 or hx-1)
+  uint32_t xs;
+  if (p) {
+    const uint32_t e = xe & 1u;   // site 0 of word w+1, or of word 0 after the last word
+    const uint32_t at = w == last ? 1u << bg.r : 0x80000000u;
+    xs = ((x >> 1) & ~at) | (e ? at : 0u);
+  } else {
+    const uint32_t e = (xe >> (w == 0 ? bg.r : 31)) & 1u;   // site 31 of word w-1, or site hx-1
+    xs = (x << 1) | e;
+  }
+  // ---- unlike-neighbour counts ei = s0 + 2*s1 + 4*s2 of all 32 sites
+  const uint32_t d1 = own ^ up, d2 = own ^ down, d3 = own ^ x, d4 = own ^ xs;
+  const uint32_t t1 = d1 ^ d2, c1 = d1 & d2, t2 = d3 ^ d4, c2 = d3 & d4;
+  const uint32_t s0 = t1 ^ t2, c3 = t1 & t2;   // c3 excludes c1 and c2
+  const uint32_t s1 = c1 ^ c2 ^ c3, s2 = c1 & c2;
+  const uint32_t upw = ~(s1 | s2);   // ei <= 1: a change is uphill (dE = 4 or 2)
+
+  // ---- per site: proposal draw, second draw, accept; masks are 0 or ~0.
+  // At T = 0 both thresholds are 0, so no second draw passes; the draw
+  // itself is not taken (nzt clears it).  The proposal is 2 when the first
+  // draw is at least half = 2^30 (bit 30 of a value below 2^31; the caller
+  // checks half).  A site's "kept" flag is the sign bit of keep, and enters
+  // ch_q at bit 0 as the earlier sites move up (sites run from the last down,
+  // so site i ends at bit i).
+  const uint32_t nzt = r.zero_t ? 0u : 0xffffffffu;
+  const int32_t thr2 = r.zero_t ? 0 : r.thr[2], thr4 = r.zero_t ? 0 : r.thr[4];
+  // this thread's part of the word, shifted down so site i is bit i
+  const int i0 = q * SPT;
+  const uint32_t own_q = own >> i0, upw_q = upw >> i0, s0_q = s0 >> i0;
+  uint32_t ch_q = 0;
+#pragma unroll
+  for (int j = SPT / 4 - 1; j >= 0; j--) {
+    Words<4> nq;
+#pragma unroll
+    for (int v = 3; v >= 0; v--) {
+      const int i = 4 * j + v;
+      const uint32_t upm = bit_mask(upw_q, i);
+      const int32_t thr = (int32_t)bop3<kSel>(bit_mask(s0_q, i), (uint32_t)thr2, (uint32_t)thr4);   // ei 1: dE 2
+      uint32_t keep;
+      if constexpr (SINGLE) {
+        const int32_t s = (int32_t)sq[j].w[v];
+        const int32_t u = pm_step(s);
+        const uint32_t pass = (uint32_t)u - (uint32_t)thr;   // sign bit: u < thr
+        nq.w[v] = bop3<kSel>(upm & nzt, (uint32_t)u, (uint32_t)s);
+        keep = bop3<kKeep>(0xffffffffu, upm, pass);
+      } else {
+        const int32_t u1 = pm_step((int32_t)sq[j].w[v]);
+        const uint32_t flipm = bit_mask((uint32_t)u1, 30) ^ bit_mask(own_q, i);   // proposal 2 differs from the spin
+        const int32_t u2 = pm_step(u1);
+        const uint32_t pass = (uint32_t)u2 - (uint32_t)thr;   // sign bit: u2 < thr
+        const uint32_t drawm = bop3<kAnd3>(flipm, upm, nzt);
+        nq.w[v] = bop3<kSel>(drawm, (uint32_t)u2, (uint32_t)u1);
+        keep = bop3<kKeep>(flipm, upm, pass);
+      }
+      ch_q = shift_in_sign(ch_q, keep);
+    }
+    sink(j, nq);
+  }
+  ch = SPT == 32 ? ch_q : (ch_q & ((1u << (SPT & 31)) - 1u)) << i0;
+  if (w == last) ch &= bg.lastmask;
+  Counts out;
+  out.nacc = __builtin_popcount(ch);
+  // sum over changes of dE = 4 - 2*ei
+  const int sum_ei = __builtin_popcount(ch & s0) + 2 * __builtin_popcount(ch & s1) + 4 * __builtin_popcount(ch & s2);
+  out.de = 4 * out.nacc - 2 * sum_ei;
+  return out;
+}
+
+// One colour pass's work of thread t of colour c on the planes in place:
+// its SPT site events (bits_events), the new seeds and changed bits written
+// back.
 template <int SPT, bool SINGLE = false>
 __host__ __device__ __forceinline__ Counts bits_thread(uint32_t *__restrict__ bits, int32_t *__restrict__ sseed,
                                                        const Geom &g, const BitGeom &bg, const Rule &r, int c,
                                                        uint32_t t) {
-  static_assert(SPT == 8 || SPT == 16 || SPT == 32, "8, 16 or 32 sites per thread");
   constexpr int TPW = 32 / SPT;
   const uint32_t wd = t / TPW;
   const int q = (int)(t % TPW);
@@ -282,67 +372,12 @@ __host__ __device__ __forceinline__ Counts bits_thread(uint32_t *__restrict__ bi
   for (int j = 0; j < SPT / 4; j++) sq[j] = *reinterpret_cast<const Words<4> *>(sseed + bit_seed_index(bg, c, t, 4 * j));
   fence_schedule();
 
-  // ---- the shifted x-neighbour word: site i's neighbour at a+1 (p = 1) or
-  // a-1 (p = 0); across a row's ends the periodic image (a = 0 or hx-1)
-  uint32_t xs;
-  if (p) {
-    const uint32_t e = xe & 1u;   // site 0 of word w+1, or of word 0 after the last word
-    const uint32_t at = w == last ? 1u << bg.r : 0x80000000u;
-    xs = ((x >> 1) & ~at) | (e ? at : 0u);
-  } else {
-    const uint32_t e = (xe >> (w == 0 ? bg.r : 31)) & 1u;   // site 31 of word w-1, or site hx-1
-    xs = (x << 1) | e;
-  }
-  // ---- unlike-neighbour counts ei = s0 + 2*s1 + 4*s2 of all 32 sites
-  const uint32_t d1 = own ^ up, d2 = own ^ down, d3 = own ^ x, d4 = own ^ xs;
-  const uint32_t t1 = d1 ^ d2, c1 = d1 & d2, t2 = d3 ^ d4, c2 = d3 & d4;
-  const uint32_t s0 = t1 ^ t2, c3 = t1 & t2;   // c3 excludes c1 and c2
-  const uint32_t s1 = c1 ^ c2 ^ c3, s2 = c1 & c2;
-  const uint32_t upw = ~(s1 | s2);   // ei <= 1: a change is uphill (dE = 4 or 2)
-
-  // ---- per site: proposal draw, second draw, accept; masks are 0 or ~0.
-  // At T = 0 both thresholds are 0, so no second draw passes; the draw
-  // itself is not taken (nzt clears it).
-  const uint32_t nzt = r.zero_t ? 0u : 0xffffffffu;
-  const int32_t thr2 = r.zero_t ? 0 : r.thr[2], thr4 = r.zero_t ? 0 : r.thr[4];
-  const int32_t half_m1 = r.half - 1;
-  // this thread's part of the word, shifted down so site i is bit i
-  const int i0 = q * SPT;
-  const uint32_t own_q = own >> i0, upw_q = upw >> i0, s0_q = s0 >> i0;
-  uint32_t ch_q = 0;
-#pragma unroll
-  for (int j = 0; j < SPT / 4; j++) {
-    Words<4> nq;
-#pragma unroll
-    for (int v = 0; v < 4; v++) {
-      const int i = 4 * j + v;
-      if constexpr (SINGLE) {
-        const int32_t s = (int32_t)sq[j].w[v];
-        const uint32_t upm = bit_mask(upw_q, i);
-        const int32_t thr = (int32_t)bop3<kSel>(bit_mask(s0_q, i), (uint32_t)thr2, (uint32_t)thr4);
-        const int32_t u = pm_step(s);
-        const uint32_t passm = less_mask(u, thr);
-        nq.w[v] = bop3<kSel>(upm & nzt, (uint32_t)u, (uint32_t)s);
-        ch_q = bop3<kOrAnd>(bop3<kKeep>(0xffffffffu, upm, passm), 1u << i, ch_q);
-        continue;
-      }
-      const int32_t u1 = pm_step((int32_t)sq[j].w[v]);
-      const uint32_t prm = less_mask(half_m1, u1);   // proposal 2 (u1 >= half)
-      const uint32_t flipm = prm ^ bit_mask(own_q, i);
-      const uint32_t upm = bit_mask(upw_q, i);
-      const int32_t thr = (int32_t)bop3<kSel>(bit_mask(s0_q, i), (uint32_t)thr2, (uint32_t)thr4);   // ei 1: dE 2
-      const int32_t u2 = pm_step(u1);
-      const uint32_t passm = less_mask(u2, thr);
-      const uint32_t drawm = bop3<kAnd3>(flipm, upm, nzt);
-      nq.w[v] = bop3<kSel>(drawm, (uint32_t)u2, (uint32_t)u1);
-      ch_q = bop3<kOrAnd>(bop3<kKeep>(flipm, upm, passm), 1u << i, ch_q);
-    }
-    *reinterpret_cast<Words<4> *>(sseed + bit_seed_index(bg, c, t, 4 * j)) = nq;
-  }
-  uint32_t ch = SPT == 32 ? ch_q : (ch_q & ((1u << (SPT & 31)) - 1u)) << i0;
-  if (w == last) ch &= bg.lastmask;
-
-  // ---- this thread's part of the word, and the counts
+  uint32_t ch;
+  const Counts out = bits_events<SPT, SINGLE>(own, up, down, x, xe, w, p, q, bg, r, sq, ch,
+                                              [&](int j, const Words<4> &nq) {
+                                                *reinterpret_cast<Words<4> *>(sseed + bit_seed_index(bg, c, t, 4 * j)) = nq;
+                                              });
+  // ---- this thread's part of the word
   if (ch) {
     const uint32_t nw = own ^ ch;
     if constexpr (SPT == 32) {
@@ -353,12 +388,138 @@ __host__ __device__ __forceinline__ Counts bits_thread(uint32_t *__restrict__ bi
       reinterpret_cast<uint8_t *>(bits + self)[q] = (uint8_t)(nw >> (8 * q));
     }
   }
-  Counts out;
-  out.nacc = __builtin_popcount(ch);
-  // sum over changes of dE = 4 - 2*ei
-  const int sum_ei = __builtin_popcount(ch & s0) + 2 * __builtin_popcount(ch & s1) + 4 * __builtin_popcount(ch & s2);
-  out.de = 4 * out.nacc - 2 * sum_ei;
   return out;
+}
+
+// ------------------------------------------------- two sweeps per launch
+// A lattice whose sub-grid rows hold 33..64 plane words (sq/4n, nx 2050 ..
+// 4096) can run two whole sweeps (colour passes 0, 1, 0, 1) in one launch,
+// reading each seed once and writing it once per two sweeps instead of per
+// sweep.  A workgroup owns a band of kPairBand rows starting at y0 and holds
+// the planes of the band and kPairHalo rows either side in LDS.  Pass p
+// (colour p & 1) updates the rows y0-3+p <= rho < y0+kPairBand+3-p: each
+// pass shrinks the span that is still exact by one row at each end, and the
+// band's own rows are exact after all four.  Rows are taken periodically,
+// so rho may lie outside 0..hy-1.  The workgroup's threads form kPairGroups
+// groups of kPairStage (8 sites each: the spt-8 layout); group g takes the
+// rows y0-3+c+g+kPairGroups*j (slot j) of colour c in every pass, so each
+// thread keeps the seeds of its rows in registers from the one load at the
+// start to the one store of the band's own rows after the colour's last
+// pass.  A barrier separates the passes (a pass reads only the other
+// colour's rows and writes only its own).  Planes and seeds go from one
+// pair of buffers to another, so no workgroup reads what another writes
+// during a launch.
+constexpr int kPairBand = 16;                            // rows a workgroup owns
+constexpr int kPairHalo = 4;                             // rows of input either side
+constexpr int kPairRows = kPairBand + 2 * kPairHalo;     // plane rows in LDS
+constexpr int kPairMaxPw = 64;                           // words per row, at most
+constexpr int kPairMinPw = 33;                           // and at least
+constexpr int kPairStage = 4 * kPairMaxPw;               // threads per group (8 sites each)
+constexpr int kPairGroups = 4;
+constexpr int kPairThreads = kPairGroups * kPairStage;   // the workgroup
+constexpr int kPairSlots0 = (kPairBand + 6 + kPairGroups - 1) / kPairGroups;   // colour-0 rows per thread
+constexpr int kPairSlots1 = (kPairBand + 4 + kPairGroups - 1) / kPairGroups;   // colour-1 rows per thread
+constexpr int kPairMinRows = 64;                         // rows (hy) a lattice needs for pairs
+
+struct PairLds {
+  uint32_t planes[2][kPairRows][kPairMaxPw];             // colour, row - (y0 - kPairHalo), word
+};
+
+// sq/4n with 33..64 plane words per row, at least kPairMinRows rows, and
+// shadow seeds (2 colours x hy*4*pw threads x 8 sites, 4 bytes each) below
+// 2^31 bytes, so a seed's byte offset fits 31 bits
+__host__ __device__ inline bool pair_fits(const Geom &g) {
+  const int pw = (g.hx + 31) / 32;
+  return g.lat == SPK_HIP_SQ_4N && pw >= kPairMinPw && pw <= kPairMaxPw && g.hy >= kPairMinRows &&
+         (int64_t)g.hy * pw * 4 * 8 * 2 * 4 < (int64_t(1) << 31);
+}
+
+__host__ __device__ __forceinline__ int pair_row(int rho, int hy) {
+  const int r = rho % hy;
+  return r < 0 ? r + hy : r;
+}
+
+// Row of slot j of group g for colour c, and whether pass p works on it.
+__host__ __device__ __forceinline__ int pair_slot_row(int y0, int c, int g, int j) {
+  return y0 - 3 + c + g + kPairGroups * j;
+}
+__host__ __device__ __forceinline__ bool pair_pass_row(int y0, int p, int rho) {
+  return rho >= y0 - 3 + p && rho < y0 + kPairBand + 3 - p;
+}
+
+// Global offset of local sites 4j..4j+3 of group thread tau on physical row
+// prow of colour c: bit_seed_index of the spt-8 layout (thread t =
+// prow*4*pw + tau) in 32-bit arithmetic, which pair_fits allows.
+__host__ __device__ __forceinline__ uint32_t pair_seed_index(const BitGeom &bg, int c, int prow, uint32_t tau, int j) {
+  const uint32_t t = (uint32_t)prow * (uint32_t)(4 * bg.pw) + tau;
+  return (uint32_t)c * (uint32_t)bg.seedc + ((((t >> 6) * 2u + (uint32_t)j) * 64u + (t & 63u)) * 4u);
+}
+
+// Plane word i (0 .. 2*kPairRows*pw - 1) of the band: its global offset, and
+// its place in LDS.
+__host__ __device__ __forceinline__ int64_t pair_plane_src(const Geom &g, const BitGeom &bg, int y0, int i, int &c,
+                                                           int &lr, int &w) {
+  const int per = kPairRows * bg.pw;
+  c = i >= per;
+  const int r = i - (c ? per : 0);
+  lr = r / bg.pw;
+  w = r - lr * bg.pw;
+  return ((int64_t)c * g.hy + pair_row(y0 - kPairHalo + lr, g.hy)) * bg.pw + w;
+}
+
+// Plane word i (0 .. 2*kPairBand*pw - 1) of the band's own rows out of LDS.
+__host__ __device__ __forceinline__ void pair_store_plane(const PairLds &L, uint32_t *__restrict__ bits_out,
+                                                          const Geom &g, const BitGeom &bg, int y0, int i) {
+  const int per = kPairBand * bg.pw;
+  const int c = i >= per;
+  const int r = i - (c ? per : 0);
+  const int rr = r / bg.pw, w = r - rr * bg.pw;
+  if (y0 + rr < g.hy) bits_out[((int64_t)c * g.hy + y0 + rr) * bg.pw + w] = L.planes[c][kPairHalo + rr][w];
+}
+
+// The five plane words pass p's thread tau (< 4*pw) reads at row rho: its
+// own colour's word, and the other colour's words above, below, beside and
+// next along (bits_events).  lr (rho's LDS row) must lie in 1..kPairRows-2.
+struct PairWords {
+  uint32_t own, up, down, x, xe;
+};
+
+__host__ __device__ __forceinline__ PairWords pair_words(const PairLds &L, const Geom &g, const BitGeom &bg, int y0,
+                                                         int p, int rho, uint32_t tau) {
+  const int c = p & 1, o = c ^ 1;
+  const int lr = rho - (y0 - kPairHalo);
+  const int w = (int)(tau >> 2), last = bg.pw - 1;
+  const int px = (pair_row(rho, g.hy) + c) & 1;   // x-neighbours at a and a+1 (px = 1) or a-1 (px = 0)
+  const int wn = px ? (w == last ? 0 : w + 1) : (w == 0 ? last : w - 1);
+  return PairWords{L.planes[c][lr][w], L.planes[o][lr - 1][w], L.planes[o][lr + 1][w], L.planes[o][lr][w],
+                   L.planes[o][lr][wn]};
+}
+
+// Pass p's work at row rho for thread tau (< 4*pw), given its plane words
+// (pair_words) and its seeds in sq: the 8 site events (bits_events), the
+// changed bits into the LDS planes, the new seeds back into sq and, after
+// the colour's last pass (p >= 2), for the band's own rows, to seed_out.
+// Returns the counts of the band's own rows.
+template <bool SINGLE>
+__host__ __device__ __forceinline__ Counts pair_thread(PairLds &L, const Geom &g, const BitGeom &bg, const Rule &r,
+                                                       int y0, int p, int rho, uint32_t tau, const PairWords &pwd,
+                                                       Words<4> (&sq)[2], int32_t *__restrict__ seed_out) {
+  const int c = p & 1;
+  const int lr = rho - (y0 - kPairHalo);
+  const int prow = pair_row(rho, g.hy);
+  const int w = (int)(tau >> 2), q = (int)(tau & 3);
+  const int px = (prow + c) & 1;
+  const bool owned = rho >= y0 && rho < y0 + kPairBand && rho < g.hy;
+  uint32_t ch;
+  const Counts cnt = bits_events<8, SINGLE>(pwd.own, pwd.up, pwd.down, pwd.x, pwd.xe, w, px, q, bg, r, sq, ch,
+                                            [&](int j, const Words<4> &nq) {
+                                              sq[j] = nq;   // sq[j] is not read after this
+                                              if (p >= 2 && owned)
+                                                *reinterpret_cast<Words<4> *>(seed_out +
+                                                                              pair_seed_index(bg, c, prow, tau, j)) = nq;
+                                            });
+  if (ch) reinterpret_cast<uint8_t *>(&L.planes[c][lr][w])[q] = (uint8_t)((pwd.own ^ ch) >> (8 * q));
+  return owned ? cnt : Counts{0, 0};
 }
 
 }  // namespace spk

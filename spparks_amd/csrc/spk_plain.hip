@@ -35,6 +35,108 @@ __global__ __launch_bounds__(1024) void k_sweep_bits(uint32_t *__restrict__ bits
   block_accumulate_pack(cnt.nacc, cnt.de, acc, blockIdx.x & (kAccSlots - 1));
 }
 
+// Two sweeps of a lattice that pair_fits, one band of kPairBand rows per
+// workgroup (spk_bits.h): planes and seeds from (bits_in, seed_in) to
+// (bits_out, seed_out); sweep 1's counts to the row at acc, sweep 2's to
+// the next row.  The grid is the bands rounded up to a multiple of 8.
+// Every thread issues the loads of its band's planes, then of its seeds, in
+// the order it uses them and all before its first use: each is waited for
+// just before that use while the later ones are in flight.  The loads are
+// unconditional (a slot outside every pass, or an idle lane, loads a row
+// that exists and is not used) so that the count of loads in flight is the
+// same on every path and those waits stay exact.
+template <bool SINGLE>
+__global__ __launch_bounds__(kPairThreads) void k_sweep_bits_pair(const uint32_t *__restrict__ bits_in,
+                                                                 const int32_t *__restrict__ seed_in,
+                                                                 uint32_t *__restrict__ bits_out,
+                                                                 int32_t *__restrict__ seed_out, Geom g, BitGeom bg,
+                                                                 Rule r, unsigned long long *__restrict__ acc) {
+  __shared__ PairLds L;
+  const int y0 = xcd_tile(blockIdx.x, gridDim.x) * kPairBand;
+  if (y0 >= g.hy) return;   // a padding workgroup: all its threads leave together
+  // the group is the same on all lanes of a wavefront (kPairStage is a
+  // multiple of 64); saying so keeps it and the rows in scalar registers
+  const int grp = __builtin_amdgcn_readfirstlane((int)(threadIdx.x / kPairStage));
+  const uint32_t tau = threadIdx.x % kPairStage;
+  const bool lane_on = tau < 4u * (uint32_t)bg.pw;
+  const uint32_t ta = lane_on ? tau : 0u;
+  // the planes: at most 3 words per thread (2 * kPairRows * 64 <= 3 * 1024)
+  constexpr int kPlaneLoads = (2 * kPairRows * kPairMaxPw + kPairThreads - 1) / kPairThreads;
+  const int nplane = 2 * kPairRows * bg.pw;
+  uint32_t pv[kPlaneLoads];
+#pragma unroll
+  for (int k = 0; k < kPlaneLoads; k++) {
+    int c, lr, w;
+    pv[k] = bits_in[pair_plane_src(g, bg, y0, min((int)threadIdx.x + k * kPairThreads, nplane - 1), c, lr, w)];
+  }
+  fence_schedule();   // the plane loads go out first: their wait leaves the seeds' in flight
+  // the seeds of the thread's rows, colour 0 (passes 0, 2) then colour 1
+  Words<4> s0[kPairSlots0][2], s1[kPairSlots1][2];
+  auto load = [&](int c, int j, Words<4>(&dst)[2]) {
+    const int rho = min(pair_slot_row(y0, c, grp, j), y0 + kPairBand + 2 - c);   // within pass c's rows
+    const int prow = pair_row(rho, g.hy);
+#pragma unroll
+    for (int h = 0; h < 2; h++)
+      dst[h] = *reinterpret_cast<const Words<4> *>(seed_in + pair_seed_index(bg, c, prow, ta, h));
+  };
+  // colour 1's first two slots now, the rest during pass 0, so that all of
+  // both colours' seeds are not in registers at once before pass 0 has
+  // used them
+  constexpr int kEarly1 = 2;
+#pragma unroll
+  for (int j = 0; j < kPairSlots0; j++) load(0, j, s0[j]);
+#pragma unroll
+  for (int j = 0; j < kEarly1; j++) load(1, j, s1[j]);
+  fence_schedule();
+  // (a thread past the last word writes that word again, with its value:
+  // no condition, so the compiler keeps the loads where they were issued)
+#pragma unroll
+  for (int k = 0; k < kPlaneLoads; k++) {
+    int c, lr, w;
+    (void)pair_plane_src(g, bg, y0, min((int)threadIdx.x + k * kPairThreads, nplane - 1), c, lr, w);
+    L.planes[c][lr][w] = pv[k];
+  }
+  __syncthreads();
+  int n1 = 0, d1 = 0, n2 = 0, d2 = 0;
+  // a pass over the thread's slots
+  auto pass = [&](int p, auto &seeds, int slots, auto &&between) {
+#pragma unroll
+    for (int j = 0; j < slots; j++) {
+      const int rho = pair_slot_row(y0, p & 1, grp, j);
+      if (lane_on && pair_pass_row(y0, p, rho)) {
+        const PairWords pwd = pair_words(L, g, bg, y0, p, rho, tau);
+        const Counts cnt = pair_thread<SINGLE>(L, g, bg, r, y0, p, rho, tau, pwd, seeds[j], seed_out);
+        if (p < 2) {
+          n1 += cnt.nacc;
+          d1 += cnt.de;
+        } else {
+          n2 += cnt.nacc;
+          d2 += cnt.de;
+        }
+      }
+    }
+    between();
+    __syncthreads();
+  };
+  auto nothing = [] {};
+  // colour 1's remaining slots go out once pass 0 has used colour 0's first
+  auto load_rest1 = [&] {
+#pragma unroll
+    for (int j = kEarly1; j < kPairSlots1; j++) load(1, j, s1[j]);
+  };
+  pass(0, s0, kPairSlots0, load_rest1);
+  pass(1, s1, kPairSlots1, nothing);
+  // the last colour-0 slot's rows (y0+13+g+4 >= y0+kPairBand+1) lie past
+  // pass 2's span: its seeds are not kept beyond pass 0
+  static_assert(-3 + kPairGroups * (kPairSlots0 - 1) >= kPairBand + 1, "last colour-0 slot outside pass 2");
+  pass(2, s0, kPairSlots0 - 1, nothing);
+  pass(3, s1, kPairSlots1, nothing);
+  for (int i = threadIdx.x; i < 2 * kPairBand * bg.pw; i += kPairThreads) pair_store_plane(L, bits_out, g, bg, y0, i);
+  block_accumulate(n1, d1, acc, blockIdx.x & (kAccSlots - 1));
+  __syncthreads();   // block_accumulate's LDS slots are reused
+  block_accumulate(n2, d2, acc + kAccRow, blockIdx.x & (kAccSlots - 1));
+}
+
 // u8 state -> planes, one thread per plane word of both colours
 __global__ void k_bits_pack(const uint8_t *__restrict__ spin, Geom g, BitGeom bg, uint32_t *__restrict__ bits) {
   const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -172,12 +274,19 @@ struct spk_hip_lattice {
   // bit planes of a two-state sq/4n lattice (spk_bits.h).  Once a sweeps
   // call has packed them they hold the state (bits_active) until a call
   // that needs the byte layout unpacks them (bits_leave).
-  int bitplanes = kBitSites;           // sites per thread on the planes (0: planes off)
+  // bitplanes: 0 planes off; 1 the default (two sweeps per launch where
+  // pair_fits, else kBitSites sites per thread); 8, 16, 32 sites per thread
+  // of the per-colour kernel
+  int bitplanes = 1;
   bool bits_active = false;
+  bool pairs = false;                  // planes in the spt-8 layout, sweeps two per launch
   BitGeom bg{};
   uint32_t *bits = nullptr;
   int32_t *sseed = nullptr;
   int64_t sseed_cap = 0;               // shadow seed slots allocated
+  uint32_t *bits2 = nullptr;           // the second planes and seeds of two-sweep launches
+  int32_t *sseed2 = nullptr;
+  int64_t sseed2_cap = 0;
   // lattices that fit one workgroup's LDS run every sweep in one launch
   int resident = 1;
   size_t max_lds = 0;                  // LDS bytes a workgroup may use
@@ -290,38 +399,54 @@ void drop_graphs(spk_hip_lattice *h) {
 }
 
 // Two-state sq/4n lattices on the per-colour path sweep on bit planes.
+// (bits_events takes the proposal from bit 30 of the first draw, so half
+// must be 2^30, which RandomPark's AM*s >= 0.5 gives)
 bool bits_eligible(const spk_hip_lattice *h) {
   return h->bitplanes && h->g.lat == SPK_HIP_SQ_4N && h->rule.nspins == 2 && h->stype == kU8 &&
+         h->rule.half == (1 << 30) &&
          (!variant_app(h->d.app) || h->d.app == SPK_HIP_ISING_SINGLE);
 }
 
 // Packs the state into the planes (allocated on first use) and routes the
 // colour passes there until bits_leave.
+// (Re)allocates a shadow seed array to hold `slots` slots.
+int ensure_shadow(spk_hip_lattice *h, int32_t *&p, int64_t &cap, int64_t slots) {
+  if (slots <= cap) return SPK_HIP_OK;
+  (void)hipStreamSynchronize(h->stream);
+  if (p) (void)hipFree(p);
+  cap = 0;
+  if (hipMalloc(&p, (size_t)slots * sizeof(int32_t)) != hipSuccess) {
+    p = nullptr;
+    return set_error(SPK_HIP_ENOMEM, "bit-plane shadow seeds not available");
+  }
+  cap = slots;
+  return SPK_HIP_OK;
+}
+
+int ensure_planes(uint32_t *&p, const spk_hip_lattice *h) {
+  if (p) return SPK_HIP_OK;   // 2*hy*pw words whatever the sites per thread
+  if (hipMalloc(&p, 2 * (size_t)h->g.hy * h->bg.pw * sizeof(uint32_t)) != hipSuccess) {
+    p = nullptr;
+    return set_error(SPK_HIP_ENOMEM, "bit-plane shadow state not available");
+  }
+  return SPK_HIP_OK;
+}
+
 int bits_enter(spk_hip_lattice *h) {
-  h->bg = bit_geom(h->g, h->bitplanes);
-  if (!h->bits) {
-    const size_t words = 2 * (size_t)h->g.hy * h->bg.pw;
-    if (hipMalloc(&h->bits, words * sizeof(uint32_t)) != hipSuccess) {
-      h->bits = nullptr;
-      return set_error(SPK_HIP_ENOMEM, "bit-plane shadow state not available");
-    }
-  }
-  if (2 * h->bg.seedc > h->sseed_cap) {
-    (void)hipStreamSynchronize(h->stream);
-    if (h->sseed) (void)hipFree(h->sseed);
-    h->sseed_cap = 0;
-    if (hipMalloc(&h->sseed, 2 * (size_t)h->bg.seedc * sizeof(int32_t)) != hipSuccess) {
-      h->sseed = nullptr;
-      return set_error(SPK_HIP_ENOMEM, "bit-plane shadow seeds not available");
-    }
-    h->sseed_cap = 2 * h->bg.seedc;
-  }
+  h->pairs = h->bitplanes == 1 && pair_fits(h->g);
+  h->bg = bit_geom(h->g, h->pairs ? 8 : h->bitplanes == 1 ? kBitSites : h->bitplanes);
+  int rc;
+  if ((rc = ensure_planes(h->bits, h)) || (rc = ensure_shadow(h, h->sseed, h->sseed_cap, 2 * h->bg.seedc)))
+    return rc;
+  if (h->pairs &&
+      ((rc = ensure_planes(h->bits2, h)) || (rc = ensure_shadow(h, h->sseed2, h->sseed2_cap, 2 * h->bg.seedc))))
+    return rc;
   const int64_t words = 2 * (int64_t)h->g.hy * h->bg.pw;
   hipLaunchKernelGGL(k_bits_pack, dim3(grid_for(words, 256)), dim3(256), 0, h->stream,
                      static_cast<const uint8_t *>(h->spin), h->g, h->bg, h->bits);
   hipLaunchKernelGGL(k_seeds_to_shadow, dim3(grid_for(h->bg.seedc / 2, 256)), dim3(256), 0, h->stream, h->seed,
                      h->g, h->bg, h->sseed);
-  int rc = check_hip(hipGetLastError(), "bit-plane pack");
+  rc = check_hip(hipGetLastError(), "bit-plane pack");
   if (!rc) h->bits_active = true;
   return rc;
 }
@@ -358,10 +483,42 @@ int ensure_pinned(spk_hip_lattice *h, int rows) {
   return SPK_HIP_OK;
 }
 
-// launches of sweeps whose accumulator rows start at acc; returns launches
+// Two sweeps in one launch (k_sweep_bits_pair), from the planes and seeds
+// at `from` (0: bits/sseed, 1: bits2/sseed2) to the others.
+int launch_pair(spk_hip_lattice *h, int from, unsigned long long *acc) {
+  const int bands = (h->g.hy + kPairBand - 1) / kPairBand;
+  const unsigned grid = (unsigned)((bands + kNumXcd - 1) / kNumXcd * kNumXcd);   // xcd_tile: a multiple of 8
+  const uint32_t *bi = from ? h->bits2 : h->bits;
+  const int32_t *si = from ? h->sseed2 : h->sseed;
+  uint32_t *bo = from ? h->bits : h->bits2;
+  int32_t *so = from ? h->sseed : h->sseed2;
+  auto go = [&](auto kern) {
+    hipLaunchKernelGGL(kern, dim3(grid), dim3(kPairThreads), 0, h->stream, bi, si, bo, so, h->g, h->bg, h->rule, acc);
+  };
+  if (h->d.app == SPK_HIP_ISING_SINGLE) go(k_sweep_bits_pair<true>);
+  else go(k_sweep_bits_pair<false>);
+  return check_hip(hipGetLastError(), "two-sweep launch");
+}
+
+// Sweep-kernel launches that launch_sweeps makes for n sweeps.
+int64_t sweep_launches(const spk_hip_lattice *h, int n) {
+  if (h->bits_active && h->pairs) return 2 * (n / 4) + (int64_t)h->g.K * (n % 4);
+  return (int64_t)h->g.K * n;
+}
+
+// launches of sweeps whose accumulator rows start at acc; returns launches.
+// On planes in the pair layout, every four sweeps are two two-sweep
+// launches, there and back, so the state ends in bits/sseed; the last n % 4
+// sweeps take the per-colour kernel.
 int launch_sweeps(spk_hip_lattice *h, int n, unsigned long long *acc, int64_t *nlaunch) {
   int rc;
-  for (int s = 0; s < n; s++)
+  int s = 0;
+  if (h->bits_active && h->pairs)
+    for (; s + 4 <= n; s += 4, *nlaunch += 2)
+      if ((rc = launch_pair(h, 0, acc + (size_t)s * kAccRow)) ||
+          (rc = launch_pair(h, 1, acc + (size_t)(s + 2) * kAccRow)))
+        return rc;
+  for (; s < n; s++)
     for (int c = 0; c < h->g.K; c++, (*nlaunch)++)
       if ((rc = launch_sweep(h, c, acc + (size_t)s * kAccRow))) return rc;
   return SPK_HIP_OK;
@@ -686,6 +843,8 @@ void spk_hip_destroy(spk_hip_lattice *h) {
   if (h->stage) (void)hipFree(h->stage);
   if (h->bits) (void)hipFree(h->bits);
   if (h->sseed) (void)hipFree(h->sseed);
+  if (h->bits2) (void)hipFree(h->bits2);
+  if (h->sseed2) (void)hipFree(h->sseed2);
   if (h->acc) (void)hipFree(h->acc);
   if (h->accc) (void)hipFree(h->accc);
   if (h->ev0) (void)hipEventDestroy(h->ev0);
@@ -849,7 +1008,7 @@ int spk_hip_sweeps(spk_hip_lattice *h, int nsweeps, int64_t *naccept_total, int6
         return rc;
       done += pieces[k];
     }
-    nlaunch = (int64_t)nsweeps * h->g.K;
+    for (int n : pieces) nlaunch += sweep_launches(h, n);
   } else {
     // every launch from the host: rows [0, nsweeps)
     if ((rc = ensure_acc(h, nsweeps))) return rc;
@@ -961,7 +1120,7 @@ int spk_hip_set_bitplanes(spk_hip_lattice *h, int enable) {
     return set_error(SPK_HIP_EINVAL, "bit planes: 0 (off), 1 (on), or 8, 16, 32 sites per thread");
   const int rc = bits_leave(h);   // the plane layout depends on the sites per thread
   if (rc) return rc;
-  h->bitplanes = enable == 1 ? kBitSites : enable;
+  h->bitplanes = enable;
   drop_graphs(h);   // captured launches name one kernel or the other
   return SPK_HIP_OK;
 }

@@ -246,7 +246,9 @@ class Lattice:
         _check(self.L.spk_hip_set_resident(self.h, int(bool(enable))))
 
     def set_bitplanes(self, enable):
-        """0/False: off; 1/True: on; 8, 16 or 32: on with that many sites per thread."""
+        """0/False: off; 1/True: on (two sweeps per launch where the rows
+        hold 33..64 plane words); 8, 16 or 32: the per-colour kernel with
+        that many sites per thread."""
         _check(self.L.spk_hip_set_bitplanes(self.h, int(enable)))
 
     def set_graph(self, chunk_sweeps):

@@ -6,12 +6,17 @@
 // usage: spk_emul app lat nx ny nz nspins kT nsweeps stype vec in.bin out.bin
 //   stype: 0 uint8, 1 uint16, 2 int32 spin storage; vec: sites per thread (4)
 //   SPK_EMUL_BITS=8|16|32 in the environment runs two-state sq/4n lattices
-//   on the bit-plane path (spk_bits.h), that many sites per thread.
+//   on the bit-plane path (spk_bits.h), that many sites per thread;
+//   SPK_EMUL_BITS=pair runs them two sweeps per launch (k_sweep_bits_pair's
+//   band passes), the sweeps after the last pair on the 8-sites-per-thread
+//   pass.
 //   in.bin : int32 spin[N], int32 seed[N] (reference order)
 //   out.bin: int32 spin[N], int32 seed[N], int64 naccept[nsweeps],
 //            int64 energy[nsweeps]
 #include <cstdio>
 #include <cstdlib>
+#include <cstring>
+#include <string>
 #include <algorithm>
 #include <utility>
 #include <vector>
@@ -97,8 +102,11 @@ static int run(int app, int nx, int ny, int nz, int q, double kt, int nsw, const
   if (bz && LAT == SPK_HIP_SQ_4N && r.nspins == 2 && sizeof(T) == 1 &&
       (app < SPK_HIP_POTTS_NEIGH || app == SPK_HIP_ISING_SINGLE)) {
     // bit-plane path (spk_bits.h): pack, every pass thread by thread, unpack
-    const int spt = atoi(bz);
-    if (spt != 8 && spt != 16 && spt != 32) { fprintf(stderr, "SPK_EMUL_BITS must be 8, 16 or 32\n"); return 16; }
+    const bool pairs = std::string(bz) == "pair";
+    const int spt = pairs ? 8 : atoi(bz);
+    if (spt != 8 && spt != 16 && spt != 32) { fprintf(stderr, "SPK_EMUL_BITS must be 8, 16, 32 or pair\n"); return 16; }
+    if (pairs && !pair_fits(g)) { fprintf(stderr, "lattice does not fit two sweeps per launch\n"); return 20; }
+    if (r.half != (1 << 30)) { fprintf(stderr, "proposal threshold is not 2^30\n"); return 24; }
     const BitGeom bg = bit_geom(g, spt);
     const int64_t words = (int64_t)g.hy * bg.pw;
     // every real site has exactly one shadow seed slot
@@ -152,11 +160,88 @@ static int run(int app, int nx, int ny, int nz, int q, double kt, int nsw, const
       const int64_t h = bit_seed_home(g, bg, c, t, i);
       if (sseed[sl] != (h < 0 ? 1 : k[h])) { fprintf(stderr, "shadow seed group mismatch\n"); return 19; }
     }
+    const bool single = app == SPK_HIP_ISING_SINGLE;
+    // two sweeps per launch: every band, pass by pass (the barrier), each
+    // group's threads in turn (a pass's threads touch disjoint words), the
+    // seeds of each thread's slots held in its "registers" from the band's
+    // start, from (bits, sseed) into (bits2, sseed2), as k_sweep_bits_pair
+    std::vector<uint32_t> bits2(pairs ? 2 * words : 0, 0xdeadbeefu);
+    std::vector<int32_t> sseed2(pairs ? 2 * bg.seedc : 0, 0x7ffffff1);
+    PairLds *L = pairs ? new PairLds : nullptr;
+    std::vector<Words<4>> regs(pairs ? (size_t)kPairThreads * (kPairSlots0 + kPairSlots1) * 2 : 0);
+    while (pairs && w + 2 <= nsw) {
+      long long de[2] = {0, 0}, nc[2] = {0, 0};
+      for (int y0 = 0; y0 < g.hy; y0 += kPairBand) {
+        memset(L, 0xa5, sizeof(PairLds));   // stale LDS must not matter
+        for (int i = 0; i < 2 * kPairRows * bg.pw; i++) {
+          int c, lr, wd;
+          const int64_t at = pair_plane_src(g, bg, y0, i, c, lr, wd);
+          if (at < 0 || at >= 2 * words || lr < 0 || lr >= kPairRows || wd >= bg.pw) {
+            fprintf(stderr, "pair plane index out of range\n");
+            return 21;
+          }
+          L->planes[c][lr][wd] = bits[at];
+        }
+        // the loads every thread issues at the band's start
+        auto reg = [&](int c, int grp, uint32_t tau, int j) -> Words<4> * {
+          const size_t k = ((size_t)(grp * kPairStage + tau) * (kPairSlots0 + kPairSlots1) + (c ? kPairSlots0 : 0) + j) * 2;
+          return &regs[k];
+        };
+        for (int grp = 0; grp < kPairGroups; grp++)
+          for (uint32_t tau = 0; tau < (uint32_t)kPairStage; tau++)
+            for (int c = 0; c < 2; c++)
+              for (int j = 0; j < (c ? kPairSlots1 : kPairSlots0); j++) {
+                const int rho = std::min(pair_slot_row(y0, c, grp, j), y0 + kPairBand + 2 - c);
+                const uint32_t ta = tau < (uint32_t)(4 * bg.pw) ? tau : 0u;
+                for (int h = 0; h < 2; h++) {
+                  const int64_t at = pair_seed_index(bg, c, pair_row(rho, g.hy), ta, h);
+                  if (at + 4 > 2 * bg.seedc) { fprintf(stderr, "pair seed index out of range\n"); return 22; }
+                  memcpy(&reg(c, grp, tau, j)[h], &sseed[at], sizeof(Words<4>));
+                }
+              }
+        for (int p = 0; p < 4; p++) {
+          const int c = p & 1;
+          const int slots = c ? kPairSlots1 : (p == 2 ? kPairSlots0 - 1 : kPairSlots0);
+          for (int grp = 0; grp < kPairGroups; grp++)
+            for (uint32_t tau = 0; tau < (uint32_t)(4 * bg.pw); tau++)
+              for (int j = 0; j < slots; j++) {
+                const int rho = pair_slot_row(y0, c, grp, j);
+                if (!pair_pass_row(y0, p, rho)) continue;
+                const int lr = rho - (y0 - kPairHalo);
+                if (lr < 1 || lr > kPairRows - 2) { fprintf(stderr, "pair row outside its LDS rows\n"); return 23; }
+                Words<4>(&sq)[2] = *reinterpret_cast<Words<4>(*)[2]>(reg(c, grp, tau, j));
+                const PairWords pwd = pair_words(*L, g, bg, y0, p, rho, tau);
+                const Counts cnt = single ? pair_thread<true>(*L, g, bg, r, y0, p, rho, tau, pwd, sq, sseed2.data())
+                                          : pair_thread<false>(*L, g, bg, r, y0, p, rho, tau, pwd, sq, sseed2.data());
+                nc[p >> 1] += cnt.nacc;
+                de[p >> 1] += cnt.de;
+              }
+        }
+        for (int i = 0; i < 2 * kPairBand * bg.pw; i++) pair_store_plane(*L, bits2.data(), g, bg, y0, i);
+      }
+      std::swap(bits, bits2);
+      std::swap(sseed, sseed2);
+      for (int h = 0; h < 2; h++, w++) {
+        na += nc[h];
+        e += 2 * de[h];
+        nacc[w] = na;
+        eng[w] = e;
+      }
+      for (int64_t q = 0; q < 2 * words; q++) {
+        const int c = q >= words;
+        const int64_t t = q - (c ? words : 0);
+        bits_unpack_word(reinterpret_cast<uint8_t *>(s.data()), g, c, (int)(t / bg.pw), (int)(t % bg.pw), bits[q]);
+      }
+      if (e != energy<LAT, T>(s, g)) { fprintf(stderr, "incremental energy drifted (two sweeps per launch)\n"); return 4; }
+    }
+    delete L;
+    if (pairs) {   // the seeds of the pairs' result, for the sweeps left and the output
+      for (int64_t X = 0; X < bg.seedc / 2; X++) bits_seed_group_out(sseed.data(), g, bg, X, k.data());
+    }
     for (; w < nsw; w++) {
       long long de = 0;
       for (int c = 0; c < 2; c++)
         for (uint32_t t = 0; t < (uint32_t)bg.threads; t++) {
-          const bool single = app == SPK_HIP_ISING_SINGLE;
           const Counts cnt = spt == 8    ? (single ? bits_thread<8, true>(bits.data(), sseed.data(), g, bg, r, c, t)
                                                : bits_thread<8>(bits.data(), sseed.data(), g, bg, r, c, t))
                              : spt == 16 ? (single ? bits_thread<16, true>(bits.data(), sseed.data(), g, bg, r, c, t)

@@ -154,3 +154,28 @@ def test_emulated_kernel_matches_oracle(orc, tmp_path, case):
         assert np.array_equal(ek, seeds), (stype, vec)
         assert np.array_equal(en, nacc), (stype, vec)
         assert np.array_equal(ee.astype(np.float64), eng), (stype, vec)
+
+
+PAIR_CASES = [
+    # two sweeps per launch (k_sweep_bits_pair): rows of 64 plane words
+    # (nx 4096) and of 34 and 41 words with the last word partly padding,
+    # bands that do not divide the rows (ny 66, 70, 80), the halo wrapping
+    # round the lattice, an odd sweep after the pairs, T = 0, Potts Q=2
+    ("ising", "sq/4n", (4096, 64, 1), 2, 2.269, 31, 5),
+    ("ising", "sq/4n", (2574, 70, 1), 2, 1.5, 32, 4),
+    ("potts", "sq/4n", (2114, 66, 1), 2, 0.9, 33, 3),
+    ("ising", "sq/4n", (4094, 64, 1), 2, 0.0, 34, 4),
+    ("ising/single", "sq/4n", (4096, 80, 1), 2, 2.269, 35, 6),
+    ("ising/single", "sq/4n", (2178, 64, 1), 2, 0.0, 36, 2),
+]
+
+
+@pytest.mark.parametrize("case", PAIR_CASES, ids=lambda c: "%s-%s-q%d-T%g" % (c[0], "x".join(map(str, c[2])), c[3], c[4]))
+def test_emulated_two_sweeps_per_launch_match_oracle(orc, tmp_path, case):
+    """The two-sweep band kernel's steps (pair_thread, the LDS planes and
+    seed ring, the band and halo rows) run stage by stage on the host
+    against the oracle: spins, seeds, Naccept and energy after every sweep
+    (sweep 1's counts from stages 0-1, sweep 2's from stages 2-3)."""
+    if not os.path.exists(EMUL):
+        pytest.skip("tests/emul/spk_emul not built")
+    run_emul(orc, tmp_path, case, {"SPK_EMUL_BITS": "pair"}, variants=[(0, 4)])

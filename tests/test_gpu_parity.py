@@ -313,8 +313,8 @@ with open(os.path.join(GOLD, "max_size_runs.json")) as _f:
 
 
 def run_against_fixture(r, mode):
-    """Runs fixture entry r on the GPU (mode: bits, bytes, bits-short or
-    default) and compares Naccept and energy after every sweep and the state
+    """Runs fixture entry r on the GPU (mode: bits, bytes, bits-short,
+    bits32 or default) and compares Naccept and energy after every sweep and the state
     checksum every r["every"] sweeps.  bits-short splits each stretch into
     calls of 20 and fewer sweeps (planes kept across them)."""
     import spparks_amd as S
@@ -322,6 +322,8 @@ def run_against_fixture(r, mode):
     with gpu_init(S, r["app"], r["lattice"], tuple(r["shape"]), r["nspins"], r["kT"], r["seed"]) as L:
         if mode == "bytes":
             L.set_bitplanes(False)
+        elif mode == "bits32":   # the per-colour bit-plane kernel, not two sweeps per launch
+            L.set_bitplanes(32)
         assert str(L.checksum()) == r["checksum"][0]
         base, nacc, eng = 0, [], []
         for i in range(r["sweeps"] // r["every"]):
@@ -339,9 +341,10 @@ def run_against_fixture(r, mode):
 
 
 @pytest.mark.parametrize("name,mode", [("ising-sq4n-4096", "bits"), ("ising-sq4n-4096", "bytes"),
-                                       ("ising-sq4n-4096", "bits-short"), ("potts-sc26n-256", "default")],
+                                       ("ising-sq4n-4096", "bits-short"), ("ising-sq4n-4096", "bits32"),
+                                       ("potts-sc26n-256", "default")],
                          ids=["ising-sq4n-4096-bits", "ising-sq4n-4096-bytes",
-                              "ising-sq4n-4096-bits-short", "potts-sc26n-256"])
+                              "ising-sq4n-4096-bits-short", "ising-sq4n-4096-bits32", "potts-sc26n-256"])
 def test_full_length_runs_match_oracle_fixture(name, mode):
     """The whole 1000-sweep (4096^2 Ising) and 200-sweep (256^3 Potts) runs:
     Naccept and energy after every sweep and the state checksum every
@@ -403,17 +406,20 @@ def test_two_state_bit_planes_bit_exact(orc, case, planes):
     assert np.array_equal(ge, eng)
 
 
-def test_bit_planes_equal_bytes_on_a_large_lattice():
-    """1026 x 512 Ising (sub-grid rows of 513 sites: a one-bit last word):
-    calls of 13, 32, 0, 5 and 20 sweeps with the planes kept across calls,
-    and download, energy, checksum, set_temperature, a seed upload and a
-    change of sites per thread between them: bit planes and the byte kernel
-    give the same spins, seeds, Naccept, energies and checksums."""
+@pytest.mark.parametrize("shape", [(1026, 512, 1), (2574, 530, 1)], ids=["1026x512", "2574x530-pairs"])
+def test_bit_planes_equal_bytes_on_a_large_lattice(shape):
+    """1026 x 512 Ising (sub-grid rows of 513 sites: a one-bit last word)
+    and 2574 x 530 (41-word rows, which sweep two sweeps per launch by
+    default, and a band that does not divide the rows): calls of 13, 32, 0,
+    5 and 20 sweeps with the planes kept across calls, and download,
+    energy, checksum, set_temperature, a seed upload and a change of sites
+    per thread between them: bit planes and the byte kernel give the same
+    spins, seeds, Naccept, energies and checksums."""
     import spparks_amd as S
     out = []
     for planes in (1, 8, 32, 0):
-        say("bit planes", planes)
-        with gpu_init(S, "ising", "sq/4n", (1026, 512, 1), 2, 2.269, 4321) as L:
+        say("bit planes", planes, shape)
+        with gpu_init(S, "ising", "sq/4n", shape, 2, 2.269, 4321) as L:
             L.set_bitplanes(planes)
             rows = []
             for n in (13, 32, 0):
@@ -446,6 +452,41 @@ def test_bit_planes_equal_bytes_on_a_large_lattice():
                     assert np.array_equal(np.asarray(x), np.asarray(y))
             else:
                 assert a == b
+
+
+PAIR_CASES = [
+    # two sweeps per launch: rows of 64 plane words (nx 4096) and of 34 and
+    # 41 words with a partly padded last word, bands that do not divide the
+    # rows (ny 66, 70, 80, 530), odd sweeps after the pairs, T = 0, Potts Q=2,
+    # ising/single
+    ("ising", "sq/4n", (4096, 64, 1), 2, 2.269, 31, 9),
+    ("ising", "sq/4n", (2574, 70, 1), 2, 1.5, 32, 6),
+    ("potts", "sq/4n", (2114, 66, 1), 2, 0.9, 33, 5),
+    ("ising", "sq/4n", (4094, 64, 1), 2, 0.0, 34, 4),
+    ("ising", "sq/4n", (4096, 530, 1), 2, 2.269, 37, 12),
+    ("ising/single", "sq/4n", (4096, 80, 1), 2, 2.269, 35, 6),
+    ("ising/single", "sq/4n", (2178, 64, 1), 2, 0.0, 36, 4),
+]
+
+
+@pytest.mark.parametrize("graph", [32, 0], ids=["graphs", "eager"])
+@pytest.mark.parametrize("case", PAIR_CASES, ids=lambda c: "%s-%s-q%d-T%g" % (c[0], "x".join(map(str, c[2])), c[3], c[4]))
+def test_two_sweeps_per_launch_bit_exact(orc, case, graph):
+    """Lattices with 33..64-word plane rows sweep two sweeps per launch by
+    default: spins, seeds, per-sweep Naccept and energy against the oracle,
+    through graph replay and eager launches."""
+    import spparks_amd as S
+    app, lat, shape, q, kt, seed, n = case
+    say("case", case, "graph", graph)
+    _, _, spin1, seeds1, nacc, eng = oracle_run(orc, app, lat, shape, q, kt, seed, n)
+    with gpu_init(S, app, lat, shape, q, kt, seed) as L:
+        L.set_graph(graph)
+        tot, gn, ge = L.sweeps(n)
+        gs1, gk1 = L.download()
+    assert np.array_equal(gn, nacc)
+    assert np.array_equal(ge, eng)
+    assert np.array_equal(gs1, spin1)
+    assert np.array_equal(gk1, seeds1)
 
 
 @pytest.mark.parametrize("case", [("potts", "sq/8n", (16, 12, 1), 100, 0.5, 56789),
