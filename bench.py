@@ -408,8 +408,12 @@ def run(args):
         bytes_per_launch = B * n // K
         B_kernel = B
     achieved = bytes_per_launch / per_launch_s / 1e9
+    # the measured traffic of the kernel that ran: two sweeps per launch is
+    # the default where it fits; the per-colour plane kernel and the byte
+    # kernel have entries of their own
     traffic, traffic_detail = load_traffic(args.workload + (
-        "-bytes" if lattice == "sq/4n" and not planes and not resident and q <= 2 else ""))
+        "-bytes" if lattice == "sq/4n" and not planes and not resident and q <= 2 else
+        "-percolour" if planes and launches is None else ""))
     if resident:
         traffic, traffic_detail = load_traffic(args.workload + "-resident")
     out = {
@@ -466,6 +470,15 @@ def run(args):
             roof["issue_floor_measurement"] = lat
         out["roofline"] = roof
         out["pct_hbm_peak"] = None
+    if launches is not None:
+        # two sweeps per launch move half the bytes of two per-colour sweeps
+        # and are bound by VALU issue instead: the issue time of a launch's
+        # instructions (committed SQ counters) against its measured time
+        v = load_traffic(args.workload + "-valu")[1]
+        if v:
+            out["roofline"]["valu_issue"] = {"floor_us_per_launch": v["issue_floor_us_per_launch"],
+                                             "frac": round(v["issue_floor_us_per_launch"] / (per_launch_s * 1e6), 4),
+                                             "measurement": v}
     out["setup"] = {"device_s": round(t_setup, 6),
                     "what": "set site range 1 Q and tagreset seeds for every site, on the device"}
     if not args.no_cpu_baseline:

@@ -101,16 +101,36 @@ __global__ __launch_bounds__(1024) void k_batch_step(char *__restrict__ pool, co
     bool two = false;
     if constexpr (sizeof(T) == 1) two = rule.nspins == 2;   // two states are stored as u8
     int nacc = 0, de = 0;
-    for (; t < tend; t++) {
-      const uint32_t ti = (uint32_t)(t - first) * kTile + (uint32_t)lane;
-      if (ti < npack) {
-        int a0, b, e;
-        pack_site(g, ti, a0, b, e);
-        const Counts cnt = two ? sweep_sites<LAT, T, kVec, true, uint32_t>(sp, sdp, g, rule, thr, c, a0, b, e)
-                               : sweep_sites<LAT, T, kVec, false, uint32_t>(sp, sdp, g, rule, thr, c, a0, b, e);
-        nacc += cnt.nacc;
-        de += cnt.de;
+    // the replica's tiles of this chunk; XD is the x-shift side when the
+    // colour fixes it (sq/8n, sc/26n: the colour's x parity), a template
+    // argument so that the shifts are not selected per lane
+    auto tiles = [&]<bool TWO, int XD>() {
+      for (; t < tend; t++) {
+        const uint32_t ti = (uint32_t)(t - first) * kTile + (uint32_t)lane;
+        if (ti < npack) {
+          int a0, b, e;
+          pack_site(g, ti, a0, b, e);
+          PackWork<T, TWO, uint32_t> w = pack_count<LAT, T, TWO, uint32_t, XD>(sp, sdp, g, rule, c, a0, b, e);
+          const Counts cnt = pack_decide<LAT, T, TWO, uint32_t>(w, sp, sdp, g, rule, thr, c);
+          nacc += cnt.nacc;
+          de += cnt.de;
+        }
       }
+    };
+    auto with_two = [&]<int XD>() {
+      if constexpr (sizeof(T) == 1) {
+        if (two) {
+          tiles.template operator()<true, XD>();
+          return;
+        }
+      }
+      tiles.template operator()<false, XD>();
+    };
+    if constexpr (LAT == SPK_HIP_SQ_8N || LAT == SPK_HIP_SC_26N) {
+      if (c & 1) with_two.template operator()<1>();
+      else with_two.template operator()<0>();
+    } else {
+      with_two.template operator()<-1>();
     }
     // the chunk's tiles of this replica pass: one sum per counter, into one
     // of the row's slots (one cache line each)

@@ -23,7 +23,7 @@ def test_every_traffic_lookup_reads_or_is_absent():
     b = _bench()
     names = []
     for wl in list(b.WORKLOADS) + ["ensemble-512"]:
-        names += [wl, wl + "-bytes", wl + "-resident", wl + "-latency"]
+        names += [wl, wl + "-bytes", wl + "-resident", wl + "-latency", wl + "-percolour", wl + "-valu"]
     for name in names:
         nbytes, detail = b.load_traffic(name)
         assert nbytes is None or (isinstance(nbytes, int) and nbytes > 0), name
@@ -33,6 +33,8 @@ def test_every_traffic_lookup_reads_or_is_absent():
     for name, rec in table.items():
         if name.endswith("-latency"):
             assert rec["issue_floor_us_per_pass"] > 0, name
+        elif name.endswith("-valu"):
+            assert rec["issue_floor_us_per_launch"] > 0, name
         else:
             assert rec["bytes_per_launch"] > 0, name
 
