@@ -1150,6 +1150,29 @@ __device__ __forceinline__ void block_accumulate_pack(int x, int y, unsigned lon
   }
 }
 
+// block_accumulate of two (x, y) pairs into slot `slot` of two rows (at
+// acc0 and acc1), with one barrier.
+__device__ __forceinline__ void block_accumulate2(int x0, int y0, int x1, int y1, unsigned long long *acc0,
+                                                  unsigned long long *acc1, int slot) {
+  __shared__ int red[4][16];
+  const int v[4] = {wave_sum(x0), wave_sum(y0), wave_sum(x1), wave_sum(y1)};
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  if (lane == 0)
+    for (int k = 0; k < 4; k++) red[k][wid] = v[k];
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int s[4] = {0, 0, 0, 0};   // a block's totals stay far below 2^31
+    const int nw = blockDim.x >> 6;
+    for (int w = 0; w < nw; w++)
+      for (int k = 0; k < 4; k++) s[k] += red[k][w];
+    unsigned long long *a0 = acc0 + (size_t)slot * kSlotStride, *a1 = acc1 + (size_t)slot * kSlotStride;
+    if (s[0]) atomicAdd(&a0[0], (unsigned long long)(long long)s[0]);
+    if (s[1]) atomicAdd(&a0[1], (unsigned long long)(long long)s[1]);
+    if (s[2]) atomicAdd(&a1[0], (unsigned long long)(long long)s[2]);
+    if (s[3]) atomicAdd(&a1[1], (unsigned long long)(long long)s[3]);
+  }
+}
+
 // Makes LDS values one wavefront wrote visible to all of its lanes: LDS
 // operations of a wavefront complete in order, so a wavefront-scope fence
 // and barrier suffice (no workgroup barrier).

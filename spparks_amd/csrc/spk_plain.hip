@@ -69,25 +69,6 @@ __global__ __launch_bounds__(kPairThreads) void k_sweep_bits_pair(const uint32_t
     int c, lr, w;
     pv[k] = bits_in[pair_plane_src(g, bg, y0, min((int)threadIdx.x + k * kPairThreads, nplane - 1), c, lr, w)];
   }
-  fence_schedule();   // the plane loads go out first: their wait leaves the seeds' in flight
-  // the seeds of the thread's rows, colour 0 (passes 0, 2) then colour 1
-  Words<4> s0[kPairSlots0][2], s1[kPairSlots1][2];
-  auto load = [&](int c, int j, Words<4>(&dst)[2]) {
-    const int rho = min(pair_slot_row(y0, c, grp, j), y0 + kPairBand + 2 - c);   // within pass c's rows
-    const int prow = pair_row(rho, g.hy);
-#pragma unroll
-    for (int h = 0; h < 2; h++)
-      dst[h] = *reinterpret_cast<const Words<4> *>(seed_in + pair_seed_index(bg, c, prow, ta, h));
-  };
-  // colour 1's first two slots now, the rest during pass 0, so that all of
-  // both colours' seeds are not in registers at once before pass 0 has
-  // used them
-  constexpr int kEarly1 = 2;
-#pragma unroll
-  for (int j = 0; j < kPairSlots0; j++) load(0, j, s0[j]);
-#pragma unroll
-  for (int j = 0; j < kEarly1; j++) load(1, j, s1[j]);
-  fence_schedule();
   // (a thread past the last word writes that word again, with its value:
   // no condition, so the compiler keeps the loads where they were issued)
 #pragma unroll
@@ -96,12 +77,30 @@ __global__ __launch_bounds__(kPairThreads) void k_sweep_bits_pair(const uint32_t
     (void)pair_plane_src(g, bg, y0, min((int)threadIdx.x + k * kPairThreads, nplane - 1), c, lr, w);
     L.planes[c][lr][w] = pv[k];
   }
+  // The seeds of the thread's rows, colour 0 (passes 0, 2) and colour 1
+  // (passes 1, 3), go out a few at a time: colour 0's first two slots now,
+  // then before each slot of pass 0 colour 0's slot two ahead and one of
+  // colour 1's.  Issuing all of them at the start had every workgroup's
+  // plane loads wait behind every other's seed loads (7 us of a 48 us launch).
+  Words<4> s0[kPairSlots0][2], s1[kPairSlots1][2];
+  auto load = [&](int c, int j, Words<4>(&dst)[2]) {
+    const int rho = min(pair_slot_row(y0, c, grp, j), y0 + kPairBand + 2 - c);   // within pass c's rows
+    const int prow = pair_row(rho, g.hy);
+#pragma unroll
+    for (int h = 0; h < 2; h++)
+      dst[h] = *reinterpret_cast<const Words<4> *>(seed_in + pair_seed_index(bg, c, prow, ta, h));
+  };
+  constexpr int kAhead = 2;
+  static_assert(kPairSlots1 <= kPairSlots0 - 1, "colour 1's slots go out during pass 0");
+#pragma unroll
+  for (int j = 0; j < kAhead; j++) load(0, j, s0[j]);
   __syncthreads();
   int n1 = 0, d1 = 0, n2 = 0, d2 = 0;
-  // a pass over the thread's slots
-  auto pass = [&](int p, auto &seeds, int slots, auto &&between) {
+  // a pass over the thread's slots; before(j) runs before slot j
+  auto pass = [&](int p, auto &seeds, int slots, auto &&before) {
 #pragma unroll
     for (int j = 0; j < slots; j++) {
+      before(j);
       const int rho = pair_slot_row(y0, p & 1, grp, j);
       if (lane_on && pair_pass_row(y0, p, rho)) {
         const PairWords pwd = pair_words(L, g, bg, y0, p, rho, tau);
@@ -115,16 +114,14 @@ __global__ __launch_bounds__(kPairThreads) void k_sweep_bits_pair(const uint32_t
         }
       }
     }
-    between();
     __syncthreads();
   };
-  auto nothing = [] {};
-  // colour 1's remaining slots go out once pass 0 has used colour 0's first
-  auto load_rest1 = [&] {
-#pragma unroll
-    for (int j = kEarly1; j < kPairSlots1; j++) load(1, j, s1[j]);
+  auto loads_ahead = [&](int j) {
+    if (j + kAhead < kPairSlots0) load(0, j + kAhead, s0[j + kAhead]);
+    if (j >= 1 && j - 1 < kPairSlots1) load(1, j - 1, s1[j - 1]);
   };
-  pass(0, s0, kPairSlots0, load_rest1);
+  auto nothing = [](int) {};
+  pass(0, s0, kPairSlots0, loads_ahead);
   pass(1, s1, kPairSlots1, nothing);
   // the last colour-0 slot's rows (y0+13+g+4 >= y0+kPairBand+1) lie past
   // pass 2's span: its seeds are not kept beyond pass 0
@@ -132,9 +129,7 @@ __global__ __launch_bounds__(kPairThreads) void k_sweep_bits_pair(const uint32_t
   pass(2, s0, kPairSlots0 - 1, nothing);
   pass(3, s1, kPairSlots1, nothing);
   for (int i = threadIdx.x; i < 2 * kPairBand * bg.pw; i += kPairThreads) pair_store_plane(L, bits_out, g, bg, y0, i);
-  block_accumulate(n1, d1, acc, blockIdx.x & (kAccSlots - 1));
-  __syncthreads();   // block_accumulate's LDS slots are reused
-  block_accumulate(n2, d2, acc + kAccRow, blockIdx.x & (kAccSlots - 1));
+  block_accumulate2(n1, d1, n2, d2, acc, acc + kAccRow, blockIdx.x & (kAccSlots - 1));
 }
 
 // u8 state -> planes, one thread per plane word of both colours
