@@ -78,10 +78,10 @@ __global__ __launch_bounds__(kPairThreads) void k_sweep_bits_pair(const uint32_t
     L.planes[c][lr][w] = pv[k];
   }
   // The seeds of the thread's rows, colour 0 (passes 0, 2) and colour 1
-  // (passes 1, 3), go out a few at a time: colour 0's first two slots now,
-  // then before each slot of pass 0 colour 0's slot two ahead and one of
-  // colour 1's.  Issuing all of them at the start had every workgroup's
-  // plane loads wait behind every other's seed loads (7 us of a 48 us launch).
+  // (passes 1, 3), go out a few at a time, two slots ahead of their use:
+  // colour 0's first two slots now, the others during pass 0 (below).
+  // Issuing all of them at the start had every workgroup's plane loads wait
+  // behind every other's seed loads (7 us of a 48 us launch).
   Words<4> s0[kPairSlots0][2], s1[kPairSlots1][2];
   auto load = [&](int c, int j, Words<4>(&dst)[2]) {
     const int rho = min(pair_slot_row(y0, c, grp, j), y0 + kPairBand + 2 - c);   // within pass c's rows
@@ -91,7 +91,6 @@ __global__ __launch_bounds__(kPairThreads) void k_sweep_bits_pair(const uint32_t
       dst[h] = *reinterpret_cast<const Words<4> *>(seed_in + pair_seed_index(bg, c, prow, ta, h));
   };
   constexpr int kAhead = 2;
-  static_assert(kPairSlots1 <= kPairSlots0 - 1, "colour 1's slots go out during pass 0");
 #pragma unroll
   for (int j = 0; j < kAhead; j++) load(0, j, s0[j]);
   __syncthreads();
@@ -116,13 +115,20 @@ __global__ __launch_bounds__(kPairThreads) void k_sweep_bits_pair(const uint32_t
     }
     __syncthreads();
   };
-  auto loads_ahead = [&](int j) {
+  // colour 1's first kAhead slots go out during pass 0's last slots and the
+  // rest during pass 1, two slots ahead of use, so that each pass reads
+  // about one colour's seeds rather than pass 0 reading both
+  static_assert(kPairSlots0 - kAhead >= kAhead, "colour 1's first slots go out after colour 0's");
+  auto loads_ahead0 = [&](int j) {
     if (j + kAhead < kPairSlots0) load(0, j + kAhead, s0[j + kAhead]);
-    if (j >= 1 && j - 1 < kPairSlots1) load(1, j - 1, s1[j - 1]);
+    if (j >= kPairSlots0 - kAhead) load(1, j - (kPairSlots0 - kAhead), s1[j - (kPairSlots0 - kAhead)]);
+  };
+  auto loads_ahead1 = [&](int j) {
+    if (j + kAhead < kPairSlots1) load(1, j + kAhead, s1[j + kAhead]);
   };
   auto nothing = [](int) {};
-  pass(0, s0, kPairSlots0, loads_ahead);
-  pass(1, s1, kPairSlots1, nothing);
+  pass(0, s0, kPairSlots0, loads_ahead0);
+  pass(1, s1, kPairSlots1, loads_ahead1);
   // the last colour-0 slot's rows (y0+13+g+4 >= y0+kPairBand+1) lie past
   // pass 2's span: its seeds are not kept beyond pass 0
   static_assert(-3 + kPairGroups * (kPairSlots0 - 1) >= kPairBand + 1, "last colour-0 slot outside pass 2");
