@@ -195,7 +195,7 @@ int spk_hip_set_resident(spk_hip_lattice *h, int enable);
 
 /* Sweeps run as replays of captured HIP graphs: a call of n sweeps replays
    n / chunk_sweeps graphs of chunk_sweeps sweeps (default 32, at most 1024),
-   then one graph per set bit of the remainder (16, 8, 4, 2, 1 sweeps).  A
+   then one graph of the remainder (n % chunk_sweeps sweeps).  A
    graph is captured on first use and kept until a setting that changes the
    launches (set_launch, set_bitplanes, set_temperature, set_graph) drops
    the cache.  chunk_sweeps = 0 launches every kernel from the host.

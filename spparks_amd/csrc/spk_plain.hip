@@ -303,7 +303,7 @@ struct spk_hip_lattice {
   int64_t energy = 0;                  // total energy, exact integer
   std::vector<unsigned long long> hacc;
   // graph replay: a call of n sweeps replays n / chunk graphs of `chunk`
-  // sweeps, then one graph per set bit of the remainder (16, 8, ... 1)
+  // sweeps, then one graph of the remainder
   int chunk = 32;                      // sweeps per full graph (0: eager launches only)
   struct Piece {
     int sweeps;                        // sweeps the graph runs
@@ -651,12 +651,11 @@ int piece_graph(spk_hip_lattice *h, int n, hipGraphExec_t *out) {
 }
 
 // Sweeps per graph replay of a call of n sweeps: n / chunk full graphs,
-// then one per set bit of the remainder, largest first.
+// then one of the remainder (so at most chunk + 1 graphs are ever cached,
+// and a call shorter than chunk is a single replay).
 std::vector<int> graph_pieces(int chunk, int n) {
   std::vector<int> p(chunk > 0 ? n / chunk : 0, chunk);
-  int rem = chunk > 0 ? n % chunk : 0;
-  for (int b = 1 << 30; b > 0; b >>= 1)
-    if (rem & b) p.push_back(b);
+  if (chunk > 0 && n % chunk) p.push_back(n % chunk);
   return p;
 }
 

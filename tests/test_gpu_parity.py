@@ -191,9 +191,9 @@ def test_launch_shape_independence(block):
                          ids=["ising-256x192-planes", "potts-sq8n-130x66", "potts-sc26n-i32"])
 def test_graph_pieces_of_every_size_equal_oracle(orc, case):
     """Calls of 1..7, 9, 16, 31 and 33 sweeps (graph replays of 32 and of
-    each power of two below it, some prepared ahead with prepare(), the bit
-    planes kept across calls) against one oracle run of the same total,
-    sweep by sweep."""
+    each remainder, some prepared ahead with prepare(), the bit planes kept
+    across calls) against one oracle run of the same total, sweep by
+    sweep."""
     import spparks_amd as S
     app, lat, shape, q, kt, seed = case
     calls = [1, 2, 3, 4, 5, 6, 7, 9, 16, 31, 33]
