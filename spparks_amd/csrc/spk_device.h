@@ -22,8 +22,10 @@ constexpr uint32_t kIA = 16807u;
 constexpr double kAM = 1.0 / 2147483647.0;   // same constant as AM*seed
 
 __host__ __device__ __forceinline__ int32_t pm_step(int32_t s) {
-  uint64_t x = (uint64_t)(uint32_t)s * kIA;            // < 2^46
-  uint32_t r = (uint32_t)(x & 0x7fffffffu) + (uint32_t)(x >> 31);
+  // y = 2x with x = 16807*s < 2^46: y's high word is x >> 31 and its low
+  // word 2*(x mod 2^31), so the fold takes no 64-bit shift
+  const uint64_t y = (uint64_t)(uint32_t)s * (2u * kIA);
+  uint32_t r = (uint32_t)(y >> 32) + ((uint32_t)y >> 1);
   if (r >= (uint32_t)kIM) r -= (uint32_t)kIM;
   return (int32_t)r;
 }
