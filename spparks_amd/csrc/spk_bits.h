@@ -26,6 +26,8 @@
 // lane) covers 1 KiB of consecutive bytes.
 #pragma once
 
+#include <type_traits>
+
 #include "spk_device.h"
 
 namespace spk {
@@ -243,6 +245,7 @@ constexpr uint32_t kSel = truth3([](int x, int y, int z) { return x ? y : z; });
 constexpr uint32_t kAnd3 = truth3([](int x, int y, int z) { return x & y & z; });           // x & y & z
 constexpr uint32_t kKeep = truth3([](int x, int y, int z) { return x & (!y | z); });        // x & (~y | z)
 constexpr uint32_t kOrAnd = truth3([](int x, int y, int z) { return (x & y) | z; });        // (x & y) | z
+constexpr uint32_t kAndNot2 = truth3([](int x, int y, int z) { return x & !y & !z; });      // x & ~y & ~z
 
 // -------------------------------------------------------------- the events
 // The SPT site events of one thread: part q (sites 32q/TPW .. +SPT-1) of
@@ -261,46 +264,62 @@ constexpr uint32_t kOrAnd = truth3([](int x, int y, int z) { return (x & y) | z;
 // 39-77) instead: the proposal is the flip, without a draw; a flip with
 // dE <= 0 is kept, and an uphill one at T > 0 takes the one draw and is
 // kept when it is below thr[dE].
-// sq holds the sites' seeds; sink(j, words) receives the new seeds of local
-// sites 4j..4j+3.  Returns the changed sites as a word mask (bit i = site
-// 32w+i) in ch, and (changes, their dE sum).
-template <int SPT, bool SINGLE, class Sink>
-__host__ __device__ __forceinline__ Counts bits_events(uint32_t own, uint32_t up, uint32_t down, uint32_t x,
-                                                       uint32_t xe, int w, int p, int q, const BitGeom &bg,
-                                                       const Rule &r, const Words<4> (&sq)[SPT / 4], uint32_t &ch,
-                                                       Sink &&sink) {
-  static_assert(SPT == 8 || SPT == 16 || SPT == 32, "8, 16 or 32 sites per thread");
-  const int last = bg.pw - 1;
-  // ---- the shifted x-neighbour word: site i's neighbour at a+1 (p = 1) or
-  // a-1 (p = 0); across a row's ends the periodic image (a = 0 or hx-1)
-  uint32_t xs;
-  if (p) {
-    const uint32_t e = xe & 1u;   // site 0 of word w+1, or of word 0 after the last word
-    const uint32_t at = w == last ? 1u << bg.r : 0x80000000u;
-    xs = ((x >> 1) & ~at) | (e ? at : 0u);
+
+// The x-neighbour word shifted into place: site i's neighbour at a+1
+// (p = 1) or a-1 (p = 0); across a row's ends the periodic image (a = 0 or
+// hx-1).  FULLW: every word of the row holds 32 sites (bg.r == 31), so the
+// shift is one funnel of x and xe.
+template <bool FULLW>
+__host__ __device__ __forceinline__ uint32_t bits_xshift(uint32_t x, uint32_t xe, int w, int p, const BitGeom &bg) {
+  if constexpr (FULLW) {
+    // p = 1: site 0 of xe enters at bit 31; p = 0: site 31 of xe at bit 0
+#ifdef __HIP_DEVICE_COMPILE__
+    return p ? __builtin_amdgcn_alignbit(xe, x, 1u) : __builtin_amdgcn_alignbit(x, xe, 31u);
+#else
+    return p ? (x >> 1) | (xe << 31) : (x << 1) | (xe >> 31);
+#endif
   } else {
+    const int last = bg.pw - 1;
+    if (p) {
+      const uint32_t e = xe & 1u;   // site 0 of word w+1, or of word 0 after the last word
+      const uint32_t at = w == last ? 1u << bg.r : 0x80000000u;
+      return ((x >> 1) & ~at) | (e ? at : 0u);
+    }
     const uint32_t e = (xe >> (w == 0 ? bg.r : 31)) & 1u;   // site 31 of word w-1, or site hx-1
-    xs = (x << 1) | e;
+    return (x << 1) | e;
   }
-  // ---- unlike-neighbour counts ei = s0 + 2*s1 + 4*s2 of all 32 sites
+}
+
+// Unlike-neighbour counts ei = s0 + 2*s1 + 4*s2 of all 32 sites of a word,
+// and upw: the sites with ei <= 1, where a change is uphill (dE = 4 or 2).
+struct BitsEi {
+  uint32_t s0, s1, s2, upw;
+};
+
+__host__ __device__ __forceinline__ BitsEi bits_ei(uint32_t own, uint32_t up, uint32_t down, uint32_t x, uint32_t xs) {
   const uint32_t d1 = own ^ up, d2 = own ^ down, d3 = own ^ x, d4 = own ^ xs;
   const uint32_t t1 = d1 ^ d2, c1 = d1 & d2, t2 = d3 ^ d4, c2 = d3 & d4;
   const uint32_t s0 = t1 ^ t2, c3 = t1 & t2;   // c3 excludes c1 and c2
   const uint32_t s1 = c1 ^ c2 ^ c3, s2 = c1 & c2;
-  const uint32_t upw = ~(s1 | s2);   // ei <= 1: a change is uphill (dE = 4 or 2)
+  return BitsEi{s0, s1, s2, ~(s1 | s2)};
+}
 
-  // ---- per site: proposal draw, second draw, accept; masks are 0 or ~0.
-  // At T = 0 both thresholds are 0, so no second draw passes; the draw
-  // itself is not taken (nzt clears it).  The proposal is 2 when the first
-  // draw is at least half = 2^30 (bit 30 of a value below 2^31; the caller
-  // checks half).  A site's "kept" flag is the sign bit of keep, and enters
-  // ch_q at bit 0 as the earlier sites move up (sites run from the last down,
-  // so site i ends at bit i).
+// The SPT events of sites 0..SPT-1 of own_q, upw_q and s0_q (the word's own
+// bits, upw and s0 shifted down so the thread's first site is bit 0).  Per
+// site: proposal draw, second draw, accept; masks are 0 or ~0.  At T = 0
+// both thresholds are 0, so no second draw passes; the draw itself is not
+// taken (nzt clears it).  The proposal is 2 when the first draw is at least
+// half = 2^30 (bit 30 of a value below 2^31; the caller checks half).  A
+// site's "kept" flag is the sign bit of keep, and enters the result at bit
+// 0 as the earlier sites move up (sites run from the last down, so site i
+// ends at bit i).  sq holds the sites' seeds; sink(j, words) receives the
+// new seeds of local sites 4j..4j+3.  Returns the kept sites (bit i = site
+// i; bits SPT and up are 0).
+template <int SPT, bool SINGLE, class Sink>
+__host__ __device__ __forceinline__ uint32_t bits_sites(uint32_t own_q, uint32_t upw_q, uint32_t s0_q, const Rule &r,
+                                                        const Words<4> (&sq)[SPT / 4], Sink &&sink) {
   const uint32_t nzt = r.zero_t ? 0u : 0xffffffffu;
   const int32_t thr2 = r.zero_t ? 0 : r.thr[2], thr4 = r.zero_t ? 0 : r.thr[4];
-  // this thread's part of the word, shifted down so site i is bit i
-  const int i0 = q * SPT;
-  const uint32_t own_q = own >> i0, upw_q = upw >> i0, s0_q = s0 >> i0;
   uint32_t ch_q = 0;
 #pragma unroll
   for (int j = SPT / 4 - 1; j >= 0; j--) {
@@ -330,12 +349,27 @@ __host__ __device__ __forceinline__ Counts bits_events(uint32_t own, uint32_t up
     }
     sink(j, nq);
   }
-  ch = SPT == 32 ? ch_q : (ch_q & ((1u << (SPT & 31)) - 1u)) << i0;
-  if (w == last) ch &= bg.lastmask;
+  return ch_q;
+}
+
+// All of the above for part q of word w: returns the changed sites as a
+// word mask (bit i = site 32w+i) in ch, and (changes, their dE sum).
+template <int SPT, bool SINGLE, class Sink>
+__host__ __device__ __forceinline__ Counts bits_events(uint32_t own, uint32_t up, uint32_t down, uint32_t x,
+                                                       uint32_t xe, int w, int p, int q, const BitGeom &bg,
+                                                       const Rule &r, const Words<4> (&sq)[SPT / 4], uint32_t &ch,
+                                                       Sink &&sink) {
+  static_assert(SPT == 8 || SPT == 16 || SPT == 32, "8, 16 or 32 sites per thread");
+  const BitsEi ei = bits_ei(own, up, down, x, bits_xshift<false>(x, xe, w, p, bg));
+  const int i0 = q * SPT;   // this thread's part of the word, shifted down so site i is bit i
+  const uint32_t ch_q = bits_sites<SPT, SINGLE>(own >> i0, ei.upw >> i0, ei.s0 >> i0, r, sq, sink);
+  ch = SPT == 32 ? ch_q : ch_q << i0;
+  if (w == bg.pw - 1) ch &= bg.lastmask;
   Counts out;
   out.nacc = __builtin_popcount(ch);
   // sum over changes of dE = 4 - 2*ei
-  const int sum_ei = __builtin_popcount(ch & s0) + 2 * __builtin_popcount(ch & s1) + 4 * __builtin_popcount(ch & s2);
+  const int sum_ei = __builtin_popcount(ch & ei.s0) + 2 * __builtin_popcount(ch & ei.s1) +
+                     4 * __builtin_popcount(ch & ei.s2);
   out.de = 4 * out.nacc - 2 * sum_ei;
   return out;
 }
@@ -455,6 +489,26 @@ __host__ __device__ __forceinline__ uint32_t pair_seed_index(const BitGeom &bg, 
   return (uint32_t)c * (uint32_t)bg.seedc + ((((t >> 6) * 2u + (uint32_t)j) * 64u + (t & 63u)) * 4u);
 }
 
+// Those 4 seeds in the shadow at base.  The byte offset is 32-bit (below
+// 2^31 by pair_fits), so the shadow's address stays scalar.  FULL (pw =
+// kPairMaxPw): a row's 4*pw threads are whole 64-thread blocks of the
+// layout, so the row's part of the offset (c, prow: the same on every
+// lane) goes into that scalar address and the thread's is the same on
+// every row.
+template <bool FULL, typename S>
+__host__ __device__ __forceinline__ auto pair_seeds(S *base, const BitGeom &bg, int c, int prow, uint32_t tau, int j) {
+  using B = std::conditional_t<std::is_const_v<S>, const char, char>;
+  using W = std::conditional_t<std::is_const_v<S>, const Words<4>, Words<4>>;
+  B *b = reinterpret_cast<B *>(base);
+  if constexpr (FULL) {
+    static_assert(4 * kPairMaxPw % 64 == 0, "rows of whole blocks");
+    b += ((int64_t)c * bg.seedc + (int64_t)prow * (4 * kPairMaxPw * 8)) * 4;
+    return reinterpret_cast<W *>(b + ((tau >> 6) * 2048u + (uint32_t)j * 1024u + (tau & 63u) * 16u));
+  } else {
+    return reinterpret_cast<W *>(b + pair_seed_index(bg, c, prow, tau, j) * 4u);
+  }
+}
+
 // Plane word i (0 .. 2*kPairRows*pw - 1) of the band: its global offset, and
 // its place in LDS.
 __host__ __device__ __forceinline__ int64_t pair_plane_src(const Geom &g, const BitGeom &bg, int y0, int i, int &c,
@@ -484,42 +538,65 @@ struct PairWords {
   uint32_t own, up, down, x, xe;
 };
 
+template <bool FULL>
 __host__ __device__ __forceinline__ PairWords pair_words(const PairLds &L, const Geom &g, const BitGeom &bg, int y0,
                                                          int p, int rho, uint32_t tau) {
   const int c = p & 1, o = c ^ 1;
   const int lr = rho - (y0 - kPairHalo);
   const int w = (int)(tau >> 2), last = bg.pw - 1;
   const int px = (pair_row(rho, g.hy) + c) & 1;   // x-neighbours at a and a+1 (px = 1) or a-1 (px = 0)
-  const int wn = px ? (w == last ? 0 : w + 1) : (w == 0 ? last : w - 1);
+  const int wn = FULL ? (w + (px ? 1 : kPairMaxPw - 1)) & (kPairMaxPw - 1)
+                      : px ? (w == last ? 0 : w + 1) : (w == 0 ? last : w - 1);
   return PairWords{L.planes[c][lr][w], L.planes[o][lr - 1][w], L.planes[o][lr + 1][w], L.planes[o][lr][w],
                    L.planes[o][lr][wn]};
 }
 
+// A thread's changes over one sweep's two passes, on the band's own rows:
+// n, and the sums over them of s0, s1 and s2 (ei = s0 + 2*s1 + 4*s2) in
+// 10-bit fields of e.  A thread owns at most 4 rows a pass, 64 sites a
+// sweep, so no field overflows.
+struct PairCounts {
+  uint32_t n = 0, e = 0;
+};
+
+__host__ __device__ __forceinline__ Counts pair_counts(const PairCounts &a) {
+  const int e0 = (int)(a.e & 1023u), e1 = (int)((a.e >> 10) & 1023u), e2 = (int)(a.e >> 20);
+  return Counts{(int)a.n, 4 * (int)a.n - 2 * (e0 + 2 * e1 + 4 * e2)};   // dE = 4 - 2*ei a change
+}
+
 // Pass p's work at row rho for thread tau (< 4*pw), given its plane words
-// (pair_words) and its seeds in sq: the 8 site events (bits_events), the
+// (pair_words) and its seeds in sq: the 8 site events (bits_sites), the
 // changed bits into the LDS planes, the new seeds back into sq and, after
 // the colour's last pass (p >= 2), for the band's own rows, to seed_out.
-// Returns the counts of the band's own rows.
-template <bool SINGLE>
-__host__ __device__ __forceinline__ Counts pair_thread(PairLds &L, const Geom &g, const BitGeom &bg, const Rule &r,
-                                                       int y0, int p, int rho, uint32_t tau, const PairWords &pwd,
-                                                       Words<4> (&sq)[2], int32_t *__restrict__ seed_out) {
+// The changes of the band's own rows go into acc.  FULL: pw = kPairMaxPw,
+// with 32 sites in the last word as in every other.
+template <bool SINGLE, bool FULL>
+__host__ __device__ __forceinline__ void pair_thread(PairLds &L, const Geom &g, const BitGeom &bg, const Rule &r,
+                                                     int y0, int p, int rho, uint32_t tau, const PairWords &pwd,
+                                                     Words<4> (&sq)[2], int32_t *__restrict__ seed_out,
+                                                     PairCounts &acc) {
   const int c = p & 1;
   const int lr = rho - (y0 - kPairHalo);
   const int prow = pair_row(rho, g.hy);
   const int w = (int)(tau >> 2), q = (int)(tau & 3);
   const int px = (prow + c) & 1;
   const bool owned = rho >= y0 && rho < y0 + kPairBand && rho < g.hy;
-  uint32_t ch;
-  const Counts cnt = bits_events<8, SINGLE>(pwd.own, pwd.up, pwd.down, pwd.x, pwd.xe, w, px, q, bg, r, sq, ch,
-                                            [&](int j, const Words<4> &nq) {
-                                              sq[j] = nq;   // sq[j] is not read after this
-                                              if (p >= 2 && owned)
-                                                *reinterpret_cast<Words<4> *>(seed_out +
-                                                                              pair_seed_index(bg, c, prow, tau, j)) = nq;
-                                            });
-  if (ch) reinterpret_cast<uint8_t *>(&L.planes[c][lr][w])[q] = (uint8_t)((pwd.own ^ ch) >> (8 * q));
-  return owned ? cnt : Counts{0, 0};
+  const BitsEi ei = bits_ei(pwd.own, pwd.up, pwd.down, pwd.x, bits_xshift<FULL>(pwd.x, pwd.xe, w, px, bg));
+  const int i0 = 8 * q;
+  const uint32_t own_q = pwd.own >> i0, upw_q = ei.upw >> i0, s0_q = ei.s0 >> i0, s1_q = ei.s1 >> i0;
+  uint32_t ch_q = bits_sites<8, SINGLE>(own_q, upw_q, s0_q, r, sq, [&](int j, const Words<4> &nq) {
+    sq[j] = nq;   // sq[j] is not read after this
+    if (p >= 2 && owned) *pair_seeds<FULL>(seed_out, bg, c, prow, tau, j) = nq;
+  });
+  if (!FULL && w == bg.pw - 1) ch_q &= bg.lastmask >> i0;
+  // the thread's byte of its own word; the word's other threads write the
+  // other bytes and use only their own bits of what they read
+  reinterpret_cast<uint8_t *>(&L.planes[c][lr][w])[q] = (uint8_t)(own_q ^ ch_q);
+  if (owned) {   // s2 = ~upw & ~s1 (ei 4), so s2 need not stay live
+    acc.n += (uint32_t)__builtin_popcount(ch_q);
+    acc.e += (uint32_t)__builtin_popcount(ch_q & s0_q) + ((uint32_t)__builtin_popcount(ch_q & s1_q) << 10) +
+             ((uint32_t)__builtin_popcount(bop3<kAndNot2>(ch_q, upw_q, s1_q)) << 20);
+  }
 }
 
 }  // namespace spk

@@ -44,8 +44,10 @@ __global__ __launch_bounds__(1024) void k_sweep_bits(uint32_t *__restrict__ bits
 // just before that use while the later ones are in flight.  The loads are
 // unconditional (a slot outside every pass, or an idle lane, loads a row
 // that exists and is not used) so that the count of loads in flight is the
-// same on every path and those waits stay exact.
-template <bool SINGLE>
+// same on every path and those waits stay exact.  FULL: rows of kPairMaxPw
+// whole words (sub-grid rows of 2048 sites: nx = 4096), so every lane
+// works and the row arithmetic is simpler (pair_thread).
+template <bool SINGLE, bool FULL>
 __global__ __launch_bounds__(kPairThreads) void k_sweep_bits_pair(const uint32_t *__restrict__ bits_in,
                                                                  const int32_t *__restrict__ seed_in,
                                                                  uint32_t *__restrict__ bits_out,
@@ -58,7 +60,7 @@ __global__ __launch_bounds__(kPairThreads) void k_sweep_bits_pair(const uint32_t
   // multiple of 64); saying so keeps it and the rows in scalar registers
   const int grp = __builtin_amdgcn_readfirstlane((int)(threadIdx.x / kPairStage));
   const uint32_t tau = threadIdx.x % kPairStage;
-  const bool lane_on = tau < 4u * (uint32_t)bg.pw;
+  const bool lane_on = FULL || tau < 4u * (uint32_t)bg.pw;
   const uint32_t ta = lane_on ? tau : 0u;
   // the planes: at most 3 words per thread (2 * kPairRows * 64 <= 3 * 1024)
   constexpr int kPlaneLoads = (2 * kPairRows * kPairMaxPw + kPairThreads - 1) / kPairThreads;
@@ -88,13 +90,13 @@ __global__ __launch_bounds__(kPairThreads) void k_sweep_bits_pair(const uint32_t
     const int prow = pair_row(rho, g.hy);
 #pragma unroll
     for (int h = 0; h < 2; h++)
-      dst[h] = *reinterpret_cast<const Words<4> *>(seed_in + pair_seed_index(bg, c, prow, ta, h));
+      dst[h] = *pair_seeds<FULL>(seed_in, bg, c, prow, ta, h);
   };
   constexpr int kAhead = 2;
 #pragma unroll
   for (int j = 0; j < kAhead; j++) load(0, j, s0[j]);
   __syncthreads();
-  int n1 = 0, d1 = 0, n2 = 0, d2 = 0;
+  PairCounts a1, a2;   // sweep 1 (passes 0, 1) and sweep 2
   // a pass over the thread's slots; before(j) runs before slot j
   auto pass = [&](int p, auto &seeds, int slots, auto &&before) {
 #pragma unroll
@@ -102,15 +104,8 @@ __global__ __launch_bounds__(kPairThreads) void k_sweep_bits_pair(const uint32_t
       before(j);
       const int rho = pair_slot_row(y0, p & 1, grp, j);
       if (lane_on && pair_pass_row(y0, p, rho)) {
-        const PairWords pwd = pair_words(L, g, bg, y0, p, rho, tau);
-        const Counts cnt = pair_thread<SINGLE>(L, g, bg, r, y0, p, rho, tau, pwd, seeds[j], seed_out);
-        if (p < 2) {
-          n1 += cnt.nacc;
-          d1 += cnt.de;
-        } else {
-          n2 += cnt.nacc;
-          d2 += cnt.de;
-        }
+        const PairWords pwd = pair_words<FULL>(L, g, bg, y0, p, rho, tau);
+        pair_thread<SINGLE, FULL>(L, g, bg, r, y0, p, rho, tau, pwd, seeds[j], seed_out, p < 2 ? a1 : a2);
       }
     }
     __syncthreads();
@@ -135,7 +130,8 @@ __global__ __launch_bounds__(kPairThreads) void k_sweep_bits_pair(const uint32_t
   pass(2, s0, kPairSlots0 - 1, nothing);
   pass(3, s1, kPairSlots1, nothing);
   for (int i = threadIdx.x; i < 2 * kPairBand * bg.pw; i += kPairThreads) pair_store_plane(L, bits_out, g, bg, y0, i);
-  block_accumulate2(n1, d1, n2, d2, acc, acc + kAccRow, blockIdx.x & (kAccSlots - 1));
+  const Counts c1 = pair_counts(a1), c2 = pair_counts(a2);
+  block_accumulate2(c1.nacc, c1.de, c2.nacc, c2.de, acc, acc + kAccRow, blockIdx.x & (kAccSlots - 1));
 }
 
 // u8 state -> planes, one thread per plane word of both colours
@@ -496,8 +492,9 @@ int launch_pair(spk_hip_lattice *h, int from, unsigned long long *acc) {
   auto go = [&](auto kern) {
     hipLaunchKernelGGL(kern, dim3(grid), dim3(kPairThreads), 0, h->stream, bi, si, bo, so, h->g, h->bg, h->rule, acc);
   };
-  if (h->d.app == SPK_HIP_ISING_SINGLE) go(k_sweep_bits_pair<true>);
-  else go(k_sweep_bits_pair<false>);
+  const bool full = h->bg.pw == kPairMaxPw && h->bg.r == 31;   // 64 words, the last one whole
+  if (h->d.app == SPK_HIP_ISING_SINGLE) full ? go(k_sweep_bits_pair<true, true>) : go(k_sweep_bits_pair<true, false>);
+  else full ? go(k_sweep_bits_pair<false, true>) : go(k_sweep_bits_pair<false, false>);
   return check_hip(hipGetLastError(), "two-sweep launch");
 }
 

@@ -18,6 +18,7 @@
 #include <cstring>
 #include <string>
 #include <algorithm>
+#include <type_traits>
 #include <utility>
 #include <vector>
 
@@ -168,6 +169,7 @@ static int run(int app, int nx, int ny, int nz, int q, double kt, int nsw, const
     std::vector<uint32_t> bits2(pairs ? 2 * words : 0, 0xdeadbeefu);
     std::vector<int32_t> sseed2(pairs ? 2 * bg.seedc : 0, 0x7ffffff1);
     PairLds *L = pairs ? new PairLds : nullptr;
+    const bool full = bg.pw == kPairMaxPw && bg.r == 31;   // as launch_pair picks k_sweep_bits_pair<., true>
     std::vector<Words<4>> regs(pairs ? (size_t)kPairThreads * (kPairSlots0 + kPairSlots1) * 2 : 0);
     while (pairs && w + 2 <= nsw) {
       long long de[2] = {0, 0}, nc[2] = {0, 0};
@@ -196,6 +198,12 @@ static int run(int app, int nx, int ny, int nz, int q, double kt, int nsw, const
                 for (int h = 0; h < 2; h++) {
                   const int64_t at = pair_seed_index(bg, c, pair_row(rho, g.hy), ta, h);
                   if (at + 4 > 2 * bg.seedc) { fprintf(stderr, "pair seed index out of range\n"); return 22; }
+                  // the kernel's address of those seeds (a scalar row part for full rows)
+                  const int64_t kat = reinterpret_cast<const int32_t *>(
+                                          full ? pair_seeds<true>(sseed.data(), bg, c, pair_row(rho, g.hy), ta, h)
+                                               : pair_seeds<false>(sseed.data(), bg, c, pair_row(rho, g.hy), ta, h)) -
+                                      sseed.data();
+                  if (kat != at) { fprintf(stderr, "pair seed address differs from the layout's index\n"); return 24; }
                   memcpy(&reg(c, grp, tau, j)[h], &sseed[at], sizeof(Words<4>));
                 }
               }
@@ -210,9 +218,17 @@ static int run(int app, int nx, int ny, int nz, int q, double kt, int nsw, const
                 const int lr = rho - (y0 - kPairHalo);
                 if (lr < 1 || lr > kPairRows - 2) { fprintf(stderr, "pair row outside its LDS rows\n"); return 23; }
                 Words<4>(&sq)[2] = *reinterpret_cast<Words<4>(*)[2]>(reg(c, grp, tau, j));
-                const PairWords pwd = pair_words(*L, g, bg, y0, p, rho, tau);
-                const Counts cnt = single ? pair_thread<true>(*L, g, bg, r, y0, p, rho, tau, pwd, sq, sseed2.data())
-                                          : pair_thread<false>(*L, g, bg, r, y0, p, rho, tau, pwd, sq, sseed2.data());
+                PairCounts a;
+                auto run = [&](auto single_c, auto full_c) {
+                  constexpr bool SG = decltype(single_c)::value, FL = decltype(full_c)::value;
+                  const PairWords pwd = pair_words<FL>(*L, g, bg, y0, p, rho, tau);
+                  pair_thread<SG, FL>(*L, g, bg, r, y0, p, rho, tau, pwd, sq, sseed2.data(), a);
+                };
+                using T_ = std::true_type;
+                using F_ = std::false_type;
+                if (single) full ? run(T_{}, T_{}) : run(T_{}, F_{});
+                else full ? run(F_{}, T_{}) : run(F_{}, F_{});
+                const Counts cnt = pair_counts(a);
                 nc[p >> 1] += cnt.nacc;
                 de[p >> 1] += cnt.de;
               }

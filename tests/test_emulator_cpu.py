@@ -157,11 +157,14 @@ def test_emulated_kernel_matches_oracle(orc, tmp_path, case):
 
 
 PAIR_CASES = [
-    # two sweeps per launch (k_sweep_bits_pair): rows of 64 plane words
-    # (nx 4096) and of 34 and 41 words with the last word partly padding,
-    # bands that do not divide the rows (ny 66, 70, 80), the halo wrapping
-    # round the lattice, an odd sweep after the pairs, T = 0, Potts Q=2
+    # two sweeps per launch (k_sweep_bits_pair): rows of 64 whole plane
+    # words (nx 4096: the kernel's full-row form), of 64 words with the last
+    # one partly padding (nx 4094) and of 34 and 41 words, bands that do not
+    # divide the rows (ny 66, 70, 80), the halo wrapping round the lattice,
+    # an odd sweep after the pairs, T = 0, Potts Q=2
     ("ising", "sq/4n", (4096, 64, 1), 2, 2.269, 31, 5),
+    ("potts", "sq/4n", (4096, 66, 1), 2, 0.9, 38, 4),
+    ("ising", "sq/4n", (4096, 64, 1), 2, 0.0, 39, 3),
     ("ising", "sq/4n", (2574, 70, 1), 2, 1.5, 32, 4),
     ("potts", "sq/4n", (2114, 66, 1), 2, 0.9, 33, 3),
     ("ising", "sq/4n", (4094, 64, 1), 2, 0.0, 34, 4),

@@ -455,11 +455,14 @@ def test_bit_planes_equal_bytes_on_a_large_lattice(shape):
 
 
 PAIR_CASES = [
-    # two sweeps per launch: rows of 64 plane words (nx 4096) and of 34 and
-    # 41 words with a partly padded last word, bands that do not divide the
-    # rows (ny 66, 70, 80, 530), odd sweeps after the pairs, T = 0, Potts Q=2,
+    # two sweeps per launch: rows of 64 whole plane words (nx 4096: the
+    # kernel's full-row form), of 64 words with a partly padded last one
+    # (nx 4094) and of 34 and 41 words, bands that do not divide the rows
+    # (ny 66, 70, 80, 530), odd sweeps after the pairs, T = 0, Potts Q=2,
     # ising/single
     ("ising", "sq/4n", (4096, 64, 1), 2, 2.269, 31, 9),
+    ("potts", "sq/4n", (4096, 66, 1), 2, 0.9, 38, 5),
+    ("ising", "sq/4n", (4096, 64, 1), 2, 0.0, 39, 4),
     ("ising", "sq/4n", (2574, 70, 1), 2, 1.5, 32, 6),
     ("potts", "sq/4n", (2114, 66, 1), 2, 0.9, 33, 5),
     ("ising", "sq/4n", (4094, 64, 1), 2, 0.0, 34, 4),
