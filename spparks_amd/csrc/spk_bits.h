@@ -246,6 +246,22 @@ constexpr uint32_t kAnd3 = truth3([](int x, int y, int z) { return x & y & z; })
 constexpr uint32_t kKeep = truth3([](int x, int y, int z) { return x & (!y | z); });        // x & (~y | z)
 constexpr uint32_t kOrAnd = truth3([](int x, int y, int z) { return (x & y) | z; });        // (x & y) | z
 constexpr uint32_t kAndNot2 = truth3([](int x, int y, int z) { return x & !y & !z; });      // x & ~y & ~z
+constexpr uint32_t kBothDiff = truth3([](int x, int y, int z) { return (x ^ y) & (x ^ z); });   // y and z unlike x
+constexpr uint32_t kXor3 = truth3([](int x, int y, int z) { return x ^ y ^ z; });            // x ^ y ^ z
+constexpr uint32_t kNor3 = truth3([](int x, int y, int z) { return !(x | y | z); });         // ~(x | y | z)
+
+// ~0 when bit 30 of x is set, else 0.  Opaque to the compiler, which would
+// otherwise fold it with the xor that follows into two shifts, an xor and
+// an arithmetic shift (four instructions for three).
+__host__ __device__ __forceinline__ uint32_t bit30_mask(uint32_t x) {
+#ifdef __HIP_DEVICE_COMPILE__
+  uint32_t m;
+  asm("v_bfe_i32 %0, %1, 30, 1" : "=v"(m) : "v"(x));
+  return m;
+#else
+  return bit_mask(x, 30);
+#endif
+}
 
 // -------------------------------------------------------------- the events
 // The SPT site events of one thread: part q (sites 32q/TPW .. +SPT-1) of
@@ -292,16 +308,19 @@ __host__ __device__ __forceinline__ uint32_t bits_xshift(uint32_t x, uint32_t xe
 
 // Unlike-neighbour counts ei = s0 + 2*s1 + 4*s2 of all 32 sites of a word,
 // and upw: the sites with ei <= 1, where a change is uphill (dE = 4 or 2).
+// s2 = ~upw & ~s1 (ei 4 alone has neither).
 struct BitsEi {
-  uint32_t s0, s1, s2, upw;
+  uint32_t s0, s1, upw;
 };
 
 __host__ __device__ __forceinline__ BitsEi bits_ei(uint32_t own, uint32_t up, uint32_t down, uint32_t x, uint32_t xs) {
-  const uint32_t d1 = own ^ up, d2 = own ^ down, d3 = own ^ x, d4 = own ^ xs;
-  const uint32_t t1 = d1 ^ d2, c1 = d1 & d2, t2 = d3 ^ d4, c2 = d3 & d4;
-  const uint32_t s0 = t1 ^ t2, c3 = t1 & t2;   // c3 excludes c1 and c2
-  const uint32_t s1 = c1 ^ c2 ^ c3, s2 = c1 & c2;
-  return BitsEi{s0, s1, s2, ~(s1 | s2)};
+  // per neighbour pair (up, down) and (x, xs): t = one of the two unlike
+  // own, c = both; ei = t1 + t2 + 2*(c1 + c2), and c3 = t1 & t2 is the
+  // carry of t1 + t2 (it excludes c1 and c2)
+  const uint32_t t1 = up ^ down, c1 = bop3<kBothDiff>(own, up, down);
+  const uint32_t t2 = x ^ xs, c2 = bop3<kBothDiff>(own, x, xs);
+  const uint32_t c3 = t1 & t2;
+  return BitsEi{t1 ^ t2, bop3<kXor3>(c1, c2, c3), bop3<kNor3>(c1, c2, c3)};
 }
 
 // The SPT events of sites 0..SPT-1 of own_q, upw_q and s0_q (the word's own
@@ -338,7 +357,7 @@ __host__ __device__ __forceinline__ uint32_t bits_sites(uint32_t own_q, uint32_t
         keep = bop3<kKeep>(0xffffffffu, upm, pass);
       } else {
         const int32_t u1 = pm_step((int32_t)sq[j].w[v]);
-        const uint32_t flipm = bit_mask((uint32_t)u1, 30) ^ bit_mask(own_q, i);   // proposal 2 differs from the spin
+        const uint32_t flipm = bit30_mask((uint32_t)u1) ^ bit_mask(own_q, i);   // proposal 2 differs from the spin
         const int32_t u2 = pm_step(u1);
         const uint32_t pass = (uint32_t)u2 - (uint32_t)thr;   // sign bit: u2 < thr
         const uint32_t drawm = bop3<kAnd3>(flipm, upm, nzt);
@@ -369,7 +388,7 @@ __host__ __device__ __forceinline__ Counts bits_events(uint32_t own, uint32_t up
   out.nacc = __builtin_popcount(ch);
   // sum over changes of dE = 4 - 2*ei
   const int sum_ei = __builtin_popcount(ch & ei.s0) + 2 * __builtin_popcount(ch & ei.s1) +
-                     4 * __builtin_popcount(ch & ei.s2);
+                     4 * __builtin_popcount(bop3<kAndNot2>(ch, ei.upw, ei.s1));
   out.de = 4 * out.nacc - 2 * sum_ei;
   return out;
 }
@@ -457,6 +476,7 @@ constexpr int kPairMinRows = 64;                         // rows (hy) a lattice 
 
 struct PairLds {
   uint32_t planes[2][kPairRows][kPairMaxPw];             // colour, row - (y0 - kPairHalo), word
+  Words<4> parked[2][kPairThreads];                      // one slot of seeds per thread, through pass 1
 };
 
 // sq/4n with 33..64 plane words per row, at least kPairMinRows rows, and

@@ -123,11 +123,26 @@ __global__ __launch_bounds__(kPairThreads) void k_sweep_bits_pair(const uint32_t
   };
   auto nothing = [](int) {};
   pass(0, s0, kPairSlots0, loads_ahead0);
+  // colour 0's last slot that pass 2 uses waits in LDS through pass 1, when
+  // both colours' seeds are held (its registers go to pass 1's work), and
+  // comes back one slot before pass 2 needs it; the single-flip events need
+  // fewer registers and keep it
+  constexpr int kParked = kPairSlots0 - 2;
+  if constexpr (!SINGLE) {
+#pragma unroll
+    for (int h = 0; h < 2; h++) L.parked[h][threadIdx.x] = s0[kParked][h];
+  }
   pass(1, s1, kPairSlots1, loads_ahead1);
   // the last colour-0 slot's rows (y0+13+g+4 >= y0+kPairBand+1) lie past
   // pass 2's span: its seeds are not kept beyond pass 0
   static_assert(-3 + kPairGroups * (kPairSlots0 - 1) >= kPairBand + 1, "last colour-0 slot outside pass 2");
-  pass(2, s0, kPairSlots0 - 1, nothing);
+  auto unpark = [&](int j) {
+    if (!SINGLE && j == kParked - 1) {
+#pragma unroll
+      for (int h = 0; h < 2; h++) s0[kParked][h] = L.parked[h][threadIdx.x];
+    }
+  };
+  pass(2, s0, kPairSlots0 - 1, unpark);
   pass(3, s1, kPairSlots1, nothing);
   for (int i = threadIdx.x; i < 2 * kPairBand * bg.pw; i += kPairThreads) pair_store_plane(L, bits_out, g, bg, y0, i);
   const Counts c1 = pair_counts(a1), c2 = pair_counts(a2);
