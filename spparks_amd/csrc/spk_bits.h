@@ -551,24 +551,59 @@ __host__ __device__ __forceinline__ void pair_store_plane(const PairLds &L, uint
   if (y0 + rr < g.hy) bits_out[((int64_t)c * g.hy + y0 + rr) * bg.pw + w] = L.planes[c][kPairHalo + rr][w];
 }
 
-// The five plane words pass p's thread tau (< 4*pw) reads at row rho: its
+// Row rho's x-neighbour side for colour c: 1 when a site's x-neighbours
+// sit at a and a+1, 0 for a-1 and a.  FULL rows (the form needs hy even):
+// rho's row has rho's parity and y0 is even, so for group grp's rows
+// (y0-3+c+grp+4j) the side is (1 + grp) & 1 in every pass.
+template <bool FULL>
+__host__ __device__ __forceinline__ int pair_side(const Geom &g, int rho, int c, int grp) {
+  if constexpr (FULL) return (1 + grp) & 1;
+  return (pair_row(rho, g.hy) + c) & 1;
+}
+
+// LDS byte offsets of thread tau's words at a row of colour c of group
+// grp: its word w (own, above, below, beside), the x-neighbour word, and
+// its byte of its own word (w*4 + q = tau), all from the start of plane row
+// grp, so that what remains of a row's offset (its LDS row less grp: 1 + c
+// + 4j for slot j) is the same on every lane.  FULL: the same for every row
+// of the group (pair_side), so computed once.
+struct PairLane {
+  uint32_t w4, wn4, b;
+};
+
+template <bool FULL>
+__host__ __device__ __forceinline__ PairLane pair_lane(const Geom &g, const BitGeom &bg, int rho, int c, int grp,
+                                                       uint32_t tau) {
+  const int w = (int)(tau >> 2), last = bg.pw - 1;
+  const int px = pair_side<FULL>(g, rho, c, grp);
+  const int wn = FULL ? (w + (px ? 1 : kPairMaxPw - 1)) & (kPairMaxPw - 1)
+                      : px ? (w == last ? 0 : w + 1) : (w == 0 ? last : w - 1);
+  const uint32_t rowg = (uint32_t)grp * (kPairMaxPw * 4);
+  return PairLane{rowg + (uint32_t)w * 4u, rowg + (uint32_t)wn * 4u, rowg + tau};
+}
+
+// Byte `at` of the plane rows of colour c from LDS row lr0.
+__host__ __device__ __forceinline__ uint8_t *pair_lds(PairLds &L, int c, int lr0, uint32_t at) {
+  return reinterpret_cast<uint8_t *>(&L.planes[c][lr0][0]) + at;
+}
+__host__ __device__ __forceinline__ uint32_t pair_lds_word(const PairLds &L, int c, int lr0, uint32_t at) {
+  return *reinterpret_cast<const uint32_t *>(reinterpret_cast<const uint8_t *>(&L.planes[c][lr0][0]) + at);
+}
+
+// The five plane words pass p's thread reads at row rho of group grp: its
 // own colour's word, and the other colour's words above, below, beside and
-// next along (bits_events).  lr (rho's LDS row) must lie in 1..kPairRows-2.
+// next along (bits_events).  rho's LDS row must lie in 1..kPairRows-2.
 struct PairWords {
   uint32_t own, up, down, x, xe;
 };
 
-template <bool FULL>
-__host__ __device__ __forceinline__ PairWords pair_words(const PairLds &L, const Geom &g, const BitGeom &bg, int y0,
-                                                         int p, int rho, uint32_t tau) {
+__host__ __device__ __forceinline__ PairWords pair_words(const PairLds &L, int y0, int p, int rho, int grp,
+                                                         const PairLane &ln) {
   const int c = p & 1, o = c ^ 1;
-  const int lr = rho - (y0 - kPairHalo);
-  const int w = (int)(tau >> 2), last = bg.pw - 1;
-  const int px = (pair_row(rho, g.hy) + c) & 1;   // x-neighbours at a and a+1 (px = 1) or a-1 (px = 0)
-  const int wn = FULL ? (w + (px ? 1 : kPairMaxPw - 1)) & (kPairMaxPw - 1)
-                      : px ? (w == last ? 0 : w + 1) : (w == 0 ? last : w - 1);
-  return PairWords{L.planes[c][lr][w], L.planes[o][lr - 1][w], L.planes[o][lr + 1][w], L.planes[o][lr][w],
-                   L.planes[o][lr][wn]};
+  const int lr0 = rho - (y0 - kPairHalo) - grp;
+  return PairWords{pair_lds_word(L, c, lr0, ln.w4), pair_lds_word(L, o, lr0 - 1, ln.w4),
+                   pair_lds_word(L, o, lr0 + 1, ln.w4), pair_lds_word(L, o, lr0, ln.w4),
+                   pair_lds_word(L, o, lr0, ln.wn4)};
 }
 
 // A thread's changes over one sweep's two passes, on the band's own rows:
@@ -589,17 +624,17 @@ __host__ __device__ __forceinline__ Counts pair_counts(const PairCounts &a) {
 // changed bits into the LDS planes, the new seeds back into sq and, after
 // the colour's last pass (p >= 2), for the band's own rows, to seed_out.
 // The changes of the band's own rows go into acc.  FULL: pw = kPairMaxPw,
-// with 32 sites in the last word as in every other.
+// with 32 sites in the last word as in every other, and hy even.
 template <bool SINGLE, bool FULL>
 __host__ __device__ __forceinline__ void pair_thread(PairLds &L, const Geom &g, const BitGeom &bg, const Rule &r,
-                                                     int y0, int p, int rho, uint32_t tau, const PairWords &pwd,
-                                                     Words<4> (&sq)[2], int32_t *__restrict__ seed_out,
-                                                     PairCounts &acc) {
+                                                     int y0, int p, int rho, int grp, uint32_t tau, const PairLane &ln,
+                                                     const PairWords &pwd, Words<4> (&sq)[2],
+                                                     int32_t *__restrict__ seed_out, PairCounts &acc) {
   const int c = p & 1;
-  const int lr = rho - (y0 - kPairHalo);
+  const int lr0 = rho - (y0 - kPairHalo) - grp;
   const int prow = pair_row(rho, g.hy);
   const int w = (int)(tau >> 2), q = (int)(tau & 3);
-  const int px = (prow + c) & 1;
+  const int px = pair_side<FULL>(g, rho, c, grp);
   const bool owned = rho >= y0 && rho < y0 + kPairBand && rho < g.hy;
   const BitsEi ei = bits_ei(pwd.own, pwd.up, pwd.down, pwd.x, bits_xshift<FULL>(pwd.x, pwd.xe, w, px, bg));
   const int i0 = 8 * q;
@@ -611,7 +646,7 @@ __host__ __device__ __forceinline__ void pair_thread(PairLds &L, const Geom &g, 
   if (!FULL && w == bg.pw - 1) ch_q &= bg.lastmask >> i0;
   // the thread's byte of its own word; the word's other threads write the
   // other bytes and use only their own bits of what they read
-  reinterpret_cast<uint8_t *>(&L.planes[c][lr][w])[q] = (uint8_t)(own_q ^ ch_q);
+  *pair_lds(L, c, lr0, ln.b) = (uint8_t)(own_q ^ ch_q);
   if (owned) {   // s2 = ~upw & ~s1 (ei 4), so s2 need not stay live
     acc.n += (uint32_t)__builtin_popcount(ch_q);
     acc.e += (uint32_t)__builtin_popcount(ch_q & s0_q) + ((uint32_t)__builtin_popcount(ch_q & s1_q) << 10) +

@@ -45,8 +45,9 @@ __global__ __launch_bounds__(1024) void k_sweep_bits(uint32_t *__restrict__ bits
 // unconditional (a slot outside every pass, or an idle lane, loads a row
 // that exists and is not used) so that the count of loads in flight is the
 // same on every path and those waits stay exact.  FULL: rows of kPairMaxPw
-// whole words (sub-grid rows of 2048 sites: nx = 4096), so every lane
-// works and the row arithmetic is simpler (pair_thread).
+// whole words (sub-grid rows of 2048 sites: nx = 4096) and an even number
+// of them, so every lane works and the row arithmetic is simpler
+// (pair_side, pair_thread).
 template <bool SINGLE, bool FULL>
 __global__ __launch_bounds__(kPairThreads) void k_sweep_bits_pair(const uint32_t *__restrict__ bits_in,
                                                                  const int32_t *__restrict__ seed_in,
@@ -97,6 +98,13 @@ __global__ __launch_bounds__(kPairThreads) void k_sweep_bits_pair(const uint32_t
   for (int j = 0; j < kAhead; j++) load(0, j, s0[j]);
   __syncthreads();
   PairCounts a1, a2;   // sweep 1 (passes 0, 1) and sweep 2
+  // full rows: the thread's LDS offsets are the same on every row; opaque,
+  // so the compiler keeps them rather than recomputing them a row at a time
+  PairLane lane_full{};
+  if constexpr (FULL) {
+    lane_full = pair_lane<true>(g, bg, 0, 0, grp, tau);
+    asm volatile("" : "+v"(lane_full.w4), "+v"(lane_full.wn4), "+v"(lane_full.b));
+  }
   // a pass over the thread's slots; before(j) runs before slot j
   auto pass = [&](int p, auto &seeds, int slots, auto &&before) {
 #pragma unroll
@@ -104,8 +112,9 @@ __global__ __launch_bounds__(kPairThreads) void k_sweep_bits_pair(const uint32_t
       before(j);
       const int rho = pair_slot_row(y0, p & 1, grp, j);
       if (lane_on && pair_pass_row(y0, p, rho)) {
-        const PairWords pwd = pair_words<FULL>(L, g, bg, y0, p, rho, tau);
-        pair_thread<SINGLE, FULL>(L, g, bg, r, y0, p, rho, tau, pwd, seeds[j], seed_out, p < 2 ? a1 : a2);
+        const PairLane ln = FULL ? lane_full : pair_lane<false>(g, bg, rho, p & 1, grp, tau);
+        const PairWords pwd = pair_words(L, y0, p, rho, grp, ln);
+        pair_thread<SINGLE, FULL>(L, g, bg, r, y0, p, rho, grp, tau, ln, pwd, seeds[j], seed_out, p < 2 ? a1 : a2);
       }
     }
     __syncthreads();
@@ -507,7 +516,8 @@ int launch_pair(spk_hip_lattice *h, int from, unsigned long long *acc) {
   auto go = [&](auto kern) {
     hipLaunchKernelGGL(kern, dim3(grid), dim3(kPairThreads), 0, h->stream, bi, si, bo, so, h->g, h->bg, h->rule, acc);
   };
-  const bool full = h->bg.pw == kPairMaxPw && h->bg.r == 31;   // 64 words, the last one whole
+  // 64 words, the last one whole, and an even number of rows
+  const bool full = h->bg.pw == kPairMaxPw && h->bg.r == 31 && h->g.hy % 2 == 0;
   if (h->d.app == SPK_HIP_ISING_SINGLE) full ? go(k_sweep_bits_pair<true, true>) : go(k_sweep_bits_pair<true, false>);
   else full ? go(k_sweep_bits_pair<false, true>) : go(k_sweep_bits_pair<false, false>);
   return check_hip(hipGetLastError(), "two-sweep launch");

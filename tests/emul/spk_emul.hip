@@ -169,7 +169,8 @@ static int run(int app, int nx, int ny, int nz, int q, double kt, int nsw, const
     std::vector<uint32_t> bits2(pairs ? 2 * words : 0, 0xdeadbeefu);
     std::vector<int32_t> sseed2(pairs ? 2 * bg.seedc : 0, 0x7ffffff1);
     PairLds *L = pairs ? new PairLds : nullptr;
-    const bool full = bg.pw == kPairMaxPw && bg.r == 31;   // as launch_pair picks k_sweep_bits_pair<., true>
+    // as launch_pair picks k_sweep_bits_pair<., true>
+    const bool full = bg.pw == kPairMaxPw && bg.r == 31 && g.hy % 2 == 0;
     std::vector<Words<4>> regs(pairs ? (size_t)kPairThreads * (kPairSlots0 + kPairSlots1) * 2 : 0);
     while (pairs && w + 2 <= nsw) {
       long long de[2] = {0, 0}, nc[2] = {0, 0};
@@ -221,8 +222,13 @@ static int run(int app, int nx, int ny, int nz, int q, double kt, int nsw, const
                 PairCounts a;
                 auto run = [&](auto single_c, auto full_c) {
                   constexpr bool SG = decltype(single_c)::value, FL = decltype(full_c)::value;
-                  const PairWords pwd = pair_words<FL>(*L, g, bg, y0, p, rho, tau);
-                  pair_thread<SG, FL>(*L, g, bg, r, y0, p, rho, tau, pwd, sq, sseed2.data(), a);
+                  if (FL && pair_side<true>(g, rho, c, grp) != pair_side<false>(g, rho, c, grp)) {
+                    fprintf(stderr, "full-row x side differs from the row's\n");
+                    exit(25);
+                  }
+                  const PairLane ln = pair_lane<FL>(g, bg, rho, c, grp, tau);
+                  const PairWords pwd = pair_words(*L, y0, p, rho, grp, ln);
+                  pair_thread<SG, FL>(*L, g, bg, r, y0, p, rho, grp, tau, ln, pwd, sq, sseed2.data(), a);
                 };
                 using T_ = std::true_type;
                 using F_ = std::false_type;
