@@ -469,6 +469,9 @@ PAIR_CASES = [
     ("ising", "sq/4n", (4096, 530, 1), 2, 2.269, 37, 12),
     ("ising/single", "sq/4n", (4096, 80, 1), 2, 2.269, 35, 6),
     ("ising/single", "sq/4n", (2178, 64, 1), 2, 0.0, 36, 4),
+    # 16 bands and 37 sweeps: a graph of 32 sweeps (16 launches), then one
+    # of 4 sweeps and a per-colour sweep
+    ("ising", "sq/4n", (4096, 256, 1), 2, 2.269, 40, 37),
 ]
 
 
