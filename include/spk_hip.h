@@ -174,11 +174,12 @@ int spk_hip_set_launch(spk_hip_lattice *h, int block);
    one-workgroup kernel) unpacks them first.  By default (enable = 1) a
    lattice whose sub-grid rows hold 33..64 plane words (nx 2050..4096) and
    at least 64 rows runs two whole sweeps per launch: a workgroup owns a
-   band of 16 rows, keeps it with 4 halo rows either side in LDS, and runs
-   the four colour passes as a pipeline over its rows, so each seed is read
-   and written once per two sweeps; the last n % 4 sweeps of a call (and
-   every sweep of other lattices) take the per-colour kernel with 32 sites
-   per thread.  enable = 0 keeps such lattices on the byte-per-site kernel;
+   band of 16 rows, keeps its planes with 4 halo rows either side in LDS
+   and its threads' seeds in registers through the four colour passes, so
+   each seed is read and written once per two sweeps; the last n % 4 sweeps
+   of a call take the per-colour kernel with 8 sites per thread (the same
+   layout), and every sweep of other lattices the per-colour kernel with 32
+   sites per thread.  enable = 0 keeps such lattices on the byte-per-site kernel;
    8, 16 or 32 selects the per-colour kernel with that many sites per thread
    for every sweep.  Results are identical either way.  No counterpart in
    the reference (a storage choice of this implementation). */
