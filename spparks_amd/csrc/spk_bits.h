@@ -456,9 +456,10 @@ __host__ __device__ __forceinline__ Counts bits_thread(uint32_t *__restrict__ bi
 // so rho may lie outside 0..hy-1.  The workgroup's threads form kPairGroups
 // groups of kPairStage (8 sites each: the spt-8 layout); group g takes the
 // rows y0-3+c+g+kPairGroups*j (slot j) of colour c in every pass, so each
-// thread keeps the seeds of its rows in registers from the one load at the
-// start to the one store of the band's own rows after the colour's last
-// pass.  A barrier separates the passes (a pass reads only the other
+// thread keeps the seeds of its rows in registers from their one load (two
+// slots ahead of first use) to the one store of the band's own rows after
+// the colour's last pass; outside the ising/single rule one colour-0 slot
+// waits in LDS (PairLds::parked) through pass 1.  A barrier separates the passes (a pass reads only the other
 // colour's rows and writes only its own).  Planes and seeds go from one
 // pair of buffers to another, so no workgroup reads what another writes
 // during a launch.

@@ -39,12 +39,12 @@ __global__ __launch_bounds__(1024) void k_sweep_bits(uint32_t *__restrict__ bits
 // workgroup (spk_bits.h): planes and seeds from (bits_in, seed_in) to
 // (bits_out, seed_out); sweep 1's counts to the row at acc, sweep 2's to
 // the next row.  The grid is the bands rounded up to a multiple of 8.
-// Every thread issues the loads of its band's planes, then of its seeds, in
-// the order it uses them and all before its first use: each is waited for
-// just before that use while the later ones are in flight.  The loads are
-// unconditional (a slot outside every pass, or an idle lane, loads a row
-// that exists and is not used) so that the count of loads in flight is the
-// same on every path and those waits stay exact.  FULL: rows of kPairMaxPw
+// Every thread issues the loads of its band's planes first, then the loads
+// of its seed slots in the order it uses them, two slots ahead of their use:
+// each is waited for just before that use while the later ones are in
+// flight.  The seed loads are unconditional (a slot outside every pass, or
+// an idle lane, loads a row that exists and is not used) so that the count
+// of loads in flight is the same on every path and those waits stay exact.  FULL: rows of kPairMaxPw
 // whole words (sub-grid rows of 2048 sites: nx = 4096) and an even number
 // of them, so every lane works and the row arithmetic is simpler
 // (pair_side, pair_thread).
